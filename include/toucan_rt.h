@@ -1,0 +1,153 @@
+/* C-ABI of the MI355X Toucan cycle-simulation runtime (libtoucan_rt.so).
+ *
+ * This is the lower boundary under toucanGPUSim::ToucanSimulator. It replaces
+ * the reference's C++-linkage kernel shim (include/SimEval.h:21-39,
+ * src/SimEval.cu:1058-1343) and the launch code of src/ToucanSim.cu:16-267,
+ * with plain pointers and sizes, an opaque per-design context instead of
+ * process globals, and negative status codes instead of exit()/assert().
+ * Every entry point names the reference function it replaces.
+ *
+ * Threading: a context is used by one host thread at a time. All device work
+ * of a context is ordered on the context's own stream.
+ */
+#ifndef TOUCAN_RT_H
+#define TOUCAN_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TGS_OK 0
+#define TGS_E_ARG (-1)       /* bad argument */
+#define TGS_E_IO (-2)        /* file missing or unreadable (ToucanSim.cu:108-122 returns -1) */
+#define TGS_E_FORMAT (-3)    /* truncated / malformed program (reference: undefined) */
+#define TGS_E_LIMIT (-4)     /* value pool or LDS over the limit (ToucanSim.cu:153,194-197) */
+#define TGS_E_HIP (-5)       /* HIP runtime error (reference: gpuErrchk exit, SimEval.h:11-19) */
+#define TGS_E_NODEV (-6)     /* no device / no cooperative launch (ToucanSim.cu:16-43) */
+#define TGS_E_BUDGET (-7)    /* HBM budget exhausted (serving admission) */
+#define TGS_E_STATE (-8)     /* call out of order (e.g. eval before load) */
+#define TGS_E_INVALID (-9)   /* program violates a format invariant (SimEval.cu:92-97,1183,1292-1311) */
+#define TGS_E_UNSUPPORTED (-10)
+
+/* kernel families */
+#define TGS_KERNEL_PLAIN 0   /* reference execution model: block per partition, 32-lane micropart per wave */
+#define TGS_KERNEL_WAVE64 1  /* persistent wave64 kernel, program resident in LDS */
+#define TGS_KERNEL_BATCH 2   /* instance-per-lane batched kernel */
+
+/* pools */
+#define TGS_POOL_REG 0
+#define TGS_POOL_MEM 1
+#define TGS_POOL_EXCHANGE 2
+
+typedef struct tgs_ctx tgs_ctx;
+
+typedef struct tgs_program_stats {
+  uint64_t n_parts[2];
+  uint64_t n_top, n_mid, n_last;          /* LUT op records */
+  uint64_t n_vecread, n_vecop, n_memread; /* word-level records */
+  uint64_t n_regread, n_exgread, n_regwrite, n_exgwrite, n_memwrite, n_print, n_stop;
+  uint64_t n_microparts, n_exec_levels;
+  uint64_t b_prog;       /* F1 record bytes read per cycle (shared by instances) */
+  uint64_t b_state;      /* state nibbles moved per cycle per instance, memory ops counted enabled */
+  uint64_t max_value_pool;
+  uint64_t reg_pool, mem_pool, exchange_pool;
+} tgs_program_stats;
+
+typedef struct tgs_profile {
+  double kernel_ms;       /* sum of kernel durations measured with HIP events on the context stream */
+  uint64_t launches;      /* kernel launches measured */
+  uint64_t cycles;        /* simulated cycles covered by those launches (summed over instances) */
+  uint32_t grid, block;   /* launch shape of the last launch */
+  uint32_t lds_bytes;     /* dynamic LDS of the last launch */
+  uint32_t kernel;        /* kernel family of the last launch */
+  uint32_t resident;      /* 1 if the wave64 program fit in LDS */
+  uint64_t prog_bytes;    /* device program image bytes */
+  uint64_t barrier_ticks; /* wave64: summed ticks spent waiting at grid barriers (rank 0 block) */
+  uint64_t total_ticks;   /* wave64: summed ticks of the cycle loop (rank 0 block) */
+} tgs_profile;
+
+/* ---- program inspection (host only, no device needed) ---- */
+
+/* Parse and validate a design image; fill counts. The byte model is the one
+ * the roofline uses (record sizes of include/ToucanGPUGenDataTypes.h). */
+int tgs_program_stats_from_bytes(const uint8_t *design, size_t size, tgs_program_stats *out,
+                                 char *err, size_t errlen);
+
+/* ---- context ---- */
+
+/* setupGPU (src/ToucanSim.cu:16-100): select device, check cooperative launch. */
+int tgs_create(int device, tgs_ctx **out);
+/* cleanup (src/ToucanSim.cu:302-304, a no-op there): frees all device memory. */
+void tgs_destroy(tgs_ctx *ctx);
+/* Text of the last error on this context (empty if none). */
+int tgs_last_error(const tgs_ctx *ctx, char *buf, size_t len);
+
+/* init (src/ToucanSim.cu:102-204) + copy_netlist_to_gpu (src/SimEval.cu:1114-1343):
+ * parse, Init, optional Randomize(0), validate, build device images, upload.
+ * symbols may be NULL (then randomize must be 0). */
+int tgs_load(tgs_ctx *ctx, const uint8_t *design, size_t design_size, const uint8_t *symbols,
+             size_t symbols_size, int randomize, int enable_print);
+int tgs_load_files(tgs_ctx *ctx, const char *design_path, const char *symbol_path, int randomize,
+                   int enable_print);
+
+/* Choose the kernel family used by eval / eval_free_running. */
+int tgs_set_kernel(tgs_ctx *ctx, int kernel);
+/* setEnablePrint (src/SimEval.cu:1095-1097). */
+int tgs_set_print(tgs_ctx *ctx, int enable);
+
+/* Number of independent stimulus instances held resident (default 1).
+ * Resets every instance to the loaded initial state. */
+int tgs_set_instances(tgs_ctx *ctx, uint32_t n);
+
+/* Primary-input stimulus: before every cycle, per instance, s = xorshift64(s)
+ * once per 64 inputs; input i takes bit (i mod 64) of draw i/64. Instance k
+ * starts from splitmix64(seed + k). n == 0 disables stimulus. The reference
+ * has no stimulus path; this is the synthetic workload driver. */
+int tgs_set_stimulus(tgs_ctx *ctx, const uint32_t *input_ids, uint32_t n, uint64_t seed);
+
+/* eval (src/ToucanSim.cu:206-230): one cycle of every instance. *stopped gets
+ * the sticky stop flag of instance 0 (get_eval_done, src/SimEval.cu:1083-1087). */
+int tgs_eval(tgs_ctx *ctx, int *stopped);
+/* eval_free_running (src/ToucanSim.cu:232-267): `cycles` cycles of every instance. */
+int tgs_eval_free_running(tgs_ctx *ctx, uint32_t cycles, int *stopped);
+/* Batched run: instance k advances cycles[k] cycles (uneven counts). */
+int tgs_run_batch(tgs_ctx *ctx, const uint32_t *cycles, uint32_t n, int *any_stopped);
+
+/* read_reg_from_gpu / write_reg_to_gpu (src/SimEval.cu:1058-1081), batched:
+ * one nibble per id, no per-nibble memcpy. */
+int tgs_read_regs(tgs_ctx *ctx, uint32_t instance, const uint32_t *ids, uint32_t n, uint8_t *out);
+int tgs_write_regs(tgs_ctx *ctx, uint32_t instance, const uint32_t *ids, uint32_t n, const uint8_t *in);
+/* Whole-pool access (checkpoint/restore of the architectural state). */
+int tgs_read_pool(tgs_ctx *ctx, uint32_t instance, int pool, uint64_t offset, uint64_t n, uint8_t *out);
+int tgs_write_pool(tgs_ctx *ctx, uint32_t instance, int pool, uint64_t offset, uint64_t n, const uint8_t *in);
+int tgs_pool_size(const tgs_ctx *ctx, int pool, uint64_t *size);
+
+/* get_eval_done (src/SimEval.cu:1083-1087) and get_real_cycles (:1089-1093). */
+int tgs_get_stop(tgs_ctx *ctx, uint32_t instance, int *stopped);
+int tgs_get_cycles(tgs_ctx *ctx, uint32_t instance, uint64_t *cycles);
+/* Cycle at which the stop flag was first raised (UINT64_MAX if never). */
+int tgs_get_stop_cycle(tgs_ctx *ctx, uint32_t instance, uint64_t *cycle);
+
+/* Print log of fired print ops: (cycle, message id) pairs in firing order of
+ * cycles. Returns the number of entries available via *count. The reference
+ * printf()s from the device (src/SimEval.cu:696-704). */
+int tgs_get_print_log(tgs_ctx *ctx, uint32_t instance, uint64_t *cycles, uint32_t *msgs, uint64_t cap,
+                      uint64_t *count);
+
+/* copy_profile_data (src/SimEval.cu:1100-1104) and printProfile
+ * (src/ToucanSim.cu:306-325): launch shape and measured kernel time. */
+int tgs_get_profile(tgs_ctx *ctx, tgs_profile *out);
+int tgs_reset_profile(tgs_ctx *ctx);
+
+/* Program statistics of the loaded design. */
+int tgs_get_program_stats(tgs_ctx *ctx, tgs_program_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOUCAN_RT_H */
