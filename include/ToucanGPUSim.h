@@ -1,0 +1,64 @@
+// Simulator API: drop-in for the reference's include/ToucanGPUSim.h:10-74.
+//
+// Same class, method names, argument meanings and return conventions
+// (init returns 0 / -1; eval* return the sticky stop flag). The device side
+// is the MI355X runtime behind include/toucan_rt.h; the object owns one
+// runtime context instead of the reference's process globals, so several
+// simulators may live in one process.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <fstream>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "SimConfigs.h"
+#include "ToucanGPUGenDataTypes.h"
+
+struct tgs_ctx;
+
+namespace toucanGPUSim {
+
+class ToucanSimulator {
+ public:
+  SimDesignInfo design;
+  SimDebugInfo symbols;
+
+  int init(const int gpu_id, const std::string designBinFilename, const std::string symbolBinFilename,
+           bool shouldRandomize, bool enablePrint);
+  void enableVCDDump(const std::string vcdFilename);
+  bool eval();
+  bool eval_free_running(uint32_t max_cycles);
+  void setRegister(const std::string &signalName, uint64_t signalValue);
+  uint64_t readRegister(const std::string &signalName);
+  uint64_t readSignal(const std::string &signalName);
+  void cleanup();
+  void printProfile();
+
+  // Extensions: kernel family (TGS_KERNEL_*), stimulus, last runtime error.
+  int setKernel(int kernel);
+  int setStimulus(const std::vector<uint32_t> &inputIds, uint64_t seed);
+  std::string lastError() const;
+  tgs_ctx *context() const { return ctx; }
+
+  ~ToucanSimulator() { cleanup(); }
+
+  uint64_t cycle_cnt = 0;
+
+ private:
+  tgs_ctx *ctx = nullptr;
+  bool dumpVCD = false;
+  std::ofstream *vcdOfs = nullptr;
+  std::vector<std::string> vcdNames;
+  std::vector<std::vector<std::tuple<uint32_t, uint32_t>>> vcdRegs;
+  std::vector<uint32_t> vcdWidth;
+  std::vector<std::string> vcdIds;
+  std::vector<std::vector<uint8_t>> vcdLast;
+  void dumpVcdWorker(uint64_t cycle);
+  void flushPrints();
+  uint64_t printsSeen = 0;
+};
+
+}  // namespace toucanGPUSim

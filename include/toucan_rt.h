@@ -77,6 +77,11 @@ typedef struct tgs_profile {
 int tgs_program_stats_from_bytes(const uint8_t *design, size_t size, tgs_program_stats *out,
                                  char *err, size_t errlen);
 
+/* Init (src/SimInit.cpp:16-26) and optionally Randomize(seed) (:28-64) of a
+ * design, host only: writes the initial register and memory pools. */
+int tgs_initial_state(const uint8_t *design, size_t design_size, const uint8_t *symbols, size_t symbols_size,
+                      int randomize, uint32_t seed, uint8_t *reg, size_t reg_size, uint8_t *mem, size_t mem_size);
+
 /* ---- context ---- */
 
 /* setupGPU (src/ToucanSim.cu:16-100): select device, check cooperative launch. */

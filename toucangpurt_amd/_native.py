@@ -151,7 +151,7 @@ def gen_lib():
 
 # Exported symbols of libtoucan_rt.so, as declared in include/toucan_rt.h.
 RT_SYMBOLS = [
-    "tgs_program_stats_from_bytes", "tgs_create", "tgs_destroy", "tgs_last_error", "tgs_load",
+    "tgs_program_stats_from_bytes", "tgs_initial_state", "tgs_create", "tgs_destroy", "tgs_last_error", "tgs_load",
     "tgs_load_files", "tgs_set_kernel", "tgs_set_print", "tgs_set_instances", "tgs_set_stimulus",
     "tgs_eval", "tgs_eval_free_running", "tgs_run_batch", "tgs_read_regs", "tgs_write_regs",
     "tgs_read_pool", "tgs_write_pool", "tgs_pool_size", "tgs_get_stop", "tgs_get_cycles",
@@ -171,6 +171,8 @@ def rt_lib():
         vp = ctypes.c_void_p
         lib.tgs_program_stats_from_bytes.argtypes = [_u8p, ctypes.c_size_t, ctypes.POINTER(ProgramStats),
                                                      ctypes.c_char_p, ctypes.c_size_t]
+        lib.tgs_initial_state.argtypes = [_u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32,
+                                          _u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t]
         lib.tgs_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
         lib.tgs_destroy.argtypes = [vp]
         lib.tgs_destroy.restype = None

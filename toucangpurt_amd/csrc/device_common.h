@@ -1,0 +1,113 @@
+// Device helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "device_layout.h"
+
+namespace tgs {
+
+// Control block of one instance (device memory, u32 words).
+enum Ctl : uint32_t {
+  C_STOP = 0,        // sticky stop flag (SimEval.cu:194 shouldStop)
+  C_BARRIER = 1,     // grid-barrier arrival counter, zeroed before each launch
+  C_LOGCOUNT = 2,    // print-log entries appended (may exceed capacity)
+  C_STOPCYC = 4,     // u64: first cycle a stop op fired (UINT64_MAX if none)
+  C_BARTICKS = 6,    // u64: rank-0 ticks spent in grid barriers (last launch)
+  C_TOTTICKS = 8,    // u64: rank-0 ticks of the cycle loop (last launch)
+  C_WORDS = 16
+};
+
+constexpr uint32_t kMaxDraws = 64;  // up to 4096 stimulus inputs
+
+// Arguments every cycle kernel takes.
+struct CommonArgs {
+  const uint8_t *lut;      // 4642-entry table, zero padded to a word multiple
+  const uint8_t *consts;   // constant-vector pools of all partitions
+  uint8_t *reg, *mem, *exg;
+  uint32_t regSize, memSize, exgSize;
+  uint64_t *xs;            // xorshift64 stimulus state
+  const uint32_t *inIds;   // stimulus input register ids
+  uint32_t nIn;
+  uint32_t *ctl;
+  uint64_t *logCycle;
+  uint32_t *logMsg;
+  uint32_t logCap;
+  uint32_t printEnable;
+  uint64_t cycleBase;      // cycles completed before this launch
+  uint32_t nCycles;
+};
+
+// xorshift64, Marsaglia (2003), shifts (13, 7, 17).
+__device__ __forceinline__ uint64_t xorshift64(uint64_t x) {
+  x ^= x << 13;
+  x ^= x >> 7;
+  x ^= x << 17;
+  return x;
+}
+
+// Advance the stimulus generator one cycle; every workgroup runs the same
+// sequence so no cross-workgroup traffic is needed for the draws.
+__device__ __forceinline__ void stimulusDraws(uint64_t &xs, uint32_t nIn, uint64_t *draws) {
+  for (uint32_t d = 0; d * 64 < nIn; ++d) {
+    xs = xorshift64(xs);
+    draws[d] = xs;
+  }
+}
+
+__device__ __forceinline__ uint32_t stimBit(const uint64_t *draws, uint32_t input) {
+  return uint32_t((draws[input >> 6] >> (input & 63)) & 1u);
+}
+
+// Print op (SimEval.cu:696-704): append (cycle, message) to the instance log.
+__device__ __forceinline__ void logPrint(const CommonArgs &a, uint64_t cycle, uint32_t msg) {
+  uint32_t k = atomicAdd(&a.ctl[C_LOGCOUNT], 1u);
+  if (k < a.logCap) {
+    a.logCycle[k] = cycle;
+    a.logMsg[k] = msg;
+  }
+}
+
+// Stop op (SimEval.cu:706-722): sticky flag plus the first cycle it fired.
+__device__ __forceinline__ void raiseStop(const CommonArgs &a, uint64_t cycle) {
+  atomicOr(&a.ctl[C_STOP], 1u);
+  atomicMin(reinterpret_cast<unsigned long long *>(&a.ctl[C_STOPCYC]), (unsigned long long)cycle);
+}
+
+// LUT lookup (SimEval.cu:349-356,406-413) with the u16 index arithmetic of the
+// reference; positions past the 4642-entry table read 0.
+__device__ __forceinline__ uint32_t lutIndex(uint32_t lut, uint32_t a, uint32_t b, uint32_t c) {
+  return (lut + (((a & 0xFFFFu) << 8) | (b << 4) | c)) & 0xFFFFu;
+}
+
+// Memory address of SimEval.cu:561-570 / 680-690 from 8 value-pool nibbles.
+__device__ __forceinline__ uint64_t memAddress(const uint8_t *vp, uint32_t addrVec, uint32_t multi, uint64_t base) {
+  uint32_t addr = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) addr |= uint32_t(vp[addrVec + j]) << (4 * j);
+  if (multi) addr <<= 2;
+  return base + addr;
+}
+
+// Grid-wide barrier over a monotonically increasing arrival counter. Every
+// wave publishes its global writes (agent-scope release) before the
+// workgroup arrives, and re-acquires after the barrier opens, so register,
+// exchange and memory pools written before the barrier are visible to every
+// XCD after it. `target` = (barrier index + 1) * gridDim.x.
+__device__ __forceinline__ uint64_t gridBarrier(uint32_t *counter, uint32_t target, bool measure) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  uint64_t waited = 0;
+  if (threadIdx.x == 0) {
+    const uint64_t t0 = measure ? __builtin_amdgcn_s_memrealtime() : 0;
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(1);
+    if (measure) waited = __builtin_amdgcn_s_memrealtime() - t0;
+  }
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return waited;
+}
+
+}  // namespace tgs

@@ -1,0 +1,83 @@
+// Device image layouts shared by the host image builder (image.cpp) and the
+// gfx950 kernels (kernels_*.hip). Plain C structs, 4-byte fields only.
+#pragma once
+#include <stdint.h>
+
+namespace tgs {
+
+constexpr uint32_t kLutSize = 4642;       // include/ToucanGPUGenDataTypes.h LUT, SimEval.cu:33
+constexpr uint32_t kLutWords = (kLutSize + 3) / 4;
+constexpr uint32_t kMaxMidLevels = 158;   // SimEval.cu:92 (+2 <= 160)
+
+// ---------------------------------------------------------------------------
+// Plain image: the reference's device layout (SimEval.cu:69-148,1143-1281).
+// Per partition one byte stream of 16-byte aligned chunks:
+//   [L0 regRead | L0 exgRead | stimulus reads][microparts ...][last level]
+// A LUT micropart is {u32 magic, u32 nMid, u8 levelSizes[160], top, mid, last}
+// and the other kinds are {u32 magic, u32 n, records}; records keep the
+// on-disk F1 layouts. Stimulus reads are a chunk of {u32 input, u16 slot, pad}.
+constexpr uint32_t kMagicLut = 0xfb709394u;      // SimEval.cu:36
+constexpr uint32_t kMagicVecRead = 0xfb809917u;  // SimEval.cu:37
+constexpr uint32_t kMagicMemRead = 0xccccccccu;  // SimEval.cu:38
+constexpr uint32_t kMagicVecOp = 0xbeef1234u;    // SimEval.cu:39
+constexpr uint32_t kPlainHeaderBytes = 8 + 160;
+
+struct PlainPart {
+  uint32_t net;          // byte offset of the partition stream in the image
+  uint32_t nRegRead;     // chunk offsets below are relative to `net`
+  uint32_t nExgRead;
+  uint32_t nStimRead;
+  uint32_t offRegRead, offExgRead, offStimRead;
+  uint32_t nLevels;
+  uint32_t levelBase;    // index into the level table (pairs {nMParts, mpartBase})
+  uint32_t offLast;      // regWrite/exgWrite/memWrite/print/stop chunks
+  uint32_t hasRegWrite, hasExgWrite, nMemWrite, nPrint, nStop;
+  uint32_t offRegWrite, offExgWrite, offMemWrite, offPrint, offStop;
+  uint32_t constBase;    // byte offset of the constant-vector pool in the const image
+  uint32_t nConst;
+};
+
+// ---------------------------------------------------------------------------
+// Wave64 image: one u32 word stream; all offsets are word offsets relative to
+// the partition's first word. The per-partition header:
+enum W64Hdr : uint32_t {
+  H_VP = 0,         // value-pool size (slots)
+  H_NRR, H_NER, H_NSR,  // L0 register / exchange / stimulus reads
+  H_OFF_L0,         // 2 words per read: {source, slot}
+  H_NLEV,           // exec levels
+  H_OFF_LEV,        // per level {task begin, task end} indices into the task table
+  H_OFF_TASK,       // task table: word offset of each task stream
+  H_RW_REG, H_RW_DC,  // register write: reg, dat | count << 16 (count 0 = none)
+  H_EW_EXG, H_EW_DC,  // exchange write
+  H_NMW, H_NPR, H_NST,  // last-level memory writes, prints, stops
+  H_OFF_LAST,       // memwrite: 4 words each, print: 1, stop: 1
+  H_CONST_BASE,     // byte offset of the constant-vector pool in the const image
+  H_NCONST,
+  H_WORDS           // header size
+};
+
+// Task kinds (low byte of a task's first word).
+enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3 };
+// T_LUT2: two on-disk 32-lane micropartitions in one wave (lanes 0-31 = A,
+//   lanes 32-63 = B). Words:
+//   w0  kind | nMid << 8
+//   w1  nTopA | nTopB << 8 | nLastA << 16 | nLastB << 24
+//   ceil(nMid/2) words of middle sizes, (nA | nB << 8) per 16 bits
+//   top ops   (nTopA + nTopB) x 2 words: lut | op0 << 16, op1 | op2 << 16
+//   middle    per level (nA + nB) words: the packed on-disk op
+//   last ops  (nLastA + nLastB) words: result | shuffleId << 16
+// T_VECREAD: w0 kind | n << 8, then n x 5 words:
+//   vecBase | vecLength << 16, offset | isConst << 16, idx0 | idx1 << 16,
+//   idx2 | idx3 << 16, outRange | result << 16
+// T_VECOP: w0 kind | n << 8, then n x 3 words:
+//   v1 | v2 << 16, len | op << 16 | constFlags << 24, result
+// T_MEMREAD: w0 kind | n << 8, then n x 4 words:
+//   memBase lo, memBase hi, en | addrVec << 16, result | multi << 16
+// Last level: memwrite 4 words {base lo, base hi, addrVec | dat << 16,
+//   en | multi << 16}; print 1 word {en | msg << 16}; stop 1 word {en}.
+
+// Per-launch schedule of the persistent wave64 kernel: for each workgroup and
+// region, the partitions it evaluates (indices into the partition table).
+// sched[g * 2 + r] .. sched[g * 2 + r + 1] index `parts`.
+
+}  // namespace tgs

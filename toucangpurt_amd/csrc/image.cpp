@@ -1,0 +1,647 @@
+// Host-side program checks, statistics and device-image builders.
+#include "image.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <queue>
+#include <unordered_map>
+
+using namespace toucanGPUSim;
+
+namespace tgs {
+namespace {
+
+uint32_t up4(uint32_t x) { return (x + 3u) & ~3u; }
+
+bool setErr(std::string *err, const std::string &m) {
+  if (err) *err = m;
+  return false;
+}
+
+// Non-LUT micropart kind by the reference's priority (SimEval.cu:129-147):
+// vecRead, then vecArithAndLogic, else memRead.
+int otherKind(const CGMicroPartInfo &m) {
+  if (!m.vecRead.empty()) return 1;
+  if (!m.vecArithAndLogic.empty()) return 2;
+  if (!m.memRead.empty()) return 3;
+  return 0;
+}
+
+// Slots read / written by one micropartition, per lane (for the hazard rule).
+struct Access {
+  std::vector<std::pair<uint32_t, uint32_t>> reads, writes;  // (slot, lane)
+};
+
+void accessOf(const CGMicroPartInfo &m, Access &a) {
+  auto rd = [&](uint32_t s, uint32_t lane) { a.reads.emplace_back(s, lane); };
+  auto rdImm = [&](uint32_t s, uint32_t lane) {
+    if (s >= 16) rd(s, lane);
+  };
+  if (m.isLUTPart) {
+    for (uint32_t i = 0; i < m.topLevel.size(); ++i) {
+      const auto &o = m.topLevel[i];
+      rdImm(o.op2, i);
+      if (o.lutIndex != 0) {
+        rdImm(o.op0, i);
+        rdImm(o.op1, i);
+      }
+    }
+    // A LUT micropart reads all its top operands before its last level writes.
+    for (uint32_t i = 0; i < m.lastLevel.size(); ++i) a.writes.emplace_back(m.lastLevel[i].result, 0);
+    return;
+  }
+  switch (otherKind(m)) {
+    case 1:
+      for (uint32_t i = 0; i < m.vecRead.size(); ++i) {
+        const auto &o = m.vecRead[i];
+        rdImm(o.index0, i); rdImm(o.index1, i); rdImm(o.index2, i); rdImm(o.index3, i);
+        rdImm(o.outRangeValue, i);
+        if (!o.isConstVec)
+          for (uint32_t k = 0; k < o.vecLength; ++k) rd(uint32_t(o.vecBase) + k, i);
+        a.writes.emplace_back(o.result, i);
+      }
+      break;
+    case 2:
+      for (uint32_t i = 0; i < m.vecArithAndLogic.size(); ++i) {
+        const auto &o = m.vecArithAndLogic[i];
+        if (!(o.isV1V2Const & 2))
+          for (uint32_t k = 0; k < o.vecLength; ++k) rd(uint32_t(o.vec1Base) + k, i);
+        if (!(o.isV1V2Const & 1))
+          for (uint32_t k = 0; k < o.vecLength; ++k) rd(uint32_t(o.vec2Base) + k, i);
+        if (o.opName <= VEC_ARITH_MUL)
+          for (uint32_t k = 0; k < o.vecLength; ++k) a.writes.emplace_back(uint16_t(o.result + k), i);
+        else
+          a.writes.emplace_back(o.result, i);
+      }
+      break;
+    case 3:
+      for (uint32_t i = 0; i < m.memRead.size(); ++i) {
+        const auto &o = m.memRead[i];
+        rdImm(o.en, i);
+        for (uint32_t k = 0; k < 8; ++k) rd(uint32_t(o.addrVec) + k, i);
+        a.writes.emplace_back(o.result, i);
+      }
+      break;
+    default:
+      break;
+  }
+}
+
+}  // namespace
+
+int validateProgram(const SimDesignInfo &d, std::string *err) {
+  auto bad = [&](const std::string &m) {
+    setErr(err, m);
+    return TGS_E_INVALID;
+  };
+  if (d.lut.size() != kLutSize) return bad("LUT table must have 4642 entries (SimEval.cu:1122)");
+  if (d.regionPartitionIds.size() != 2) return bad("exactly 2 regions are supported (SimEval.cu:1292-1295)");
+  uint32_t expect = 0;
+  for (const auto &r : d.regionPartitionIds)
+    for (uint32_t id : r)
+      if (id != expect++) return bad("region partition ids must be contiguous from 0 (SimEval.cu:1298-1302)");
+  if (expect != d.parts.size()) return bad("regions must cover every partition");
+  if (d.regionPartitionIds[0].empty() && d.regionPartitionIds[1].empty()) return bad("design has no partitions");
+  const uint32_t regN = up4(d.regPoolSize), exgN = up4(d.exchangePoolSize);
+
+  for (size_t pi = 0; pi < d.parts.size(); ++pi) {
+    const auto &p = d.parts[pi];
+    const std::string at = "partition " + std::to_string(pi) + ": ";
+    if (p.valuePoolSize > 65535) {
+      setErr(err, at + "value pool over 65535 slots (ToucanSim.cu:153)");
+      return TGS_E_LIMIT;
+    }
+    const uint32_t vp = p.valuePoolSize, vpw = up4(vp);
+    auto slot = [&](uint32_t s) { return s < vp; };
+    auto opnd = [&](uint32_t s) { return s < 16 || s < vp; };
+    if (p.ops_l0_regRead.empty() == p.ops_l0_exchangeRead.empty())
+      return bad(at + "exactly one of regRead / exchangeRead must be non-empty (SimEval.cu:1183)");
+    std::unordered_map<uint32_t, uint64_t> l0;
+    for (const auto &o : p.ops_l0_regRead) {
+      if (o.reg >= d.regPoolSize || !slot(o.result)) return bad(at + "register read out of range");
+      auto ins = l0.emplace(o.result, (uint64_t(1) << 32) | o.reg);
+      if (!ins.second && ins.first->second != ((uint64_t(1) << 32) | o.reg)) return bad(at + "two L0 reads write one slot");
+    }
+    for (const auto &o : p.ops_l0_exchangeRead) {
+      if (o.exchange >= d.exchangePoolSize || !slot(o.result)) return bad(at + "exchange read out of range");
+      auto ins = l0.emplace(o.result, (uint64_t(2) << 32) | o.exchange);
+      if (!ins.second && ins.first->second != ((uint64_t(2) << 32) | o.exchange)) return bad(at + "two L0 reads write one slot");
+    }
+    for (size_t li = 0; li < p.exec_mParts.size(); ++li) {
+      const auto &lv = p.exec_mParts[li];
+      std::unordered_map<uint32_t, uint64_t> writer;  // slot -> (mpart << 32 | lane)
+      std::vector<Access> acc(lv.size());
+      for (size_t mi = 0; mi < lv.size(); ++mi) {
+        const auto &m = lv[mi];
+        const std::string mat = at + "level " + std::to_string(li) + " micropart " + std::to_string(mi) + ": ";
+        if (m.isLUTPart) {
+          if (m.topLevel.size() > 32 || m.lastLevel.size() > 32) return bad(mat + "more than 32 lanes (SimEval.cu:93-94)");
+          if (m.middleLevels.size() + 2 > 160) return bad(mat + "more than 158 middle levels (SimEval.cu:92)");
+          for (const auto &lvl : m.middleLevels)
+            if (lvl.size() > 32) return bad(mat + "more than 32 lanes (SimEval.cu:95-97)");
+          for (const auto &o : m.topLevel)
+            if (!opnd(o.op2) || (o.lutIndex && (!opnd(o.op0) || !opnd(o.op1)))) return bad(mat + "top operand out of range");
+          for (const auto &o : m.lastLevel)
+            if (!slot(o.result)) return bad(mat + "last-level slot out of range");
+        } else {
+          int k = otherKind(m);
+          if (k == 0) return bad(mat + "empty micropart (SimEval.cu:143)");
+          size_t n = k == 1 ? m.vecRead.size() : k == 2 ? m.vecArithAndLogic.size() : m.memRead.size();
+          if (n > 32) return bad(mat + "more than 32 ops in one micropart");
+          if (k == 1)
+            for (const auto &o : m.vecRead) {
+              if (!opnd(o.index0) || !opnd(o.index1) || !opnd(o.index2) || !opnd(o.index3) || !opnd(o.outRangeValue) ||
+                  !slot(o.result))
+                return bad(mat + "vecRead operand out of range");
+              uint32_t end = uint32_t(o.vecBase) + o.vecLength;
+              if (o.vecLength && (o.isConstVec ? end > p.constVecPool.size() : end > vp))
+                return bad(mat + "vecRead vector out of range");
+            }
+          if (k == 2)
+            for (const auto &o : m.vecArithAndLogic) {
+              if (o.vecLength > 32) return bad(mat + "vector op longer than 32 nibbles (128 bits)");
+              uint32_t e1 = uint32_t(o.vec1Base) + o.vecLength, e2 = uint32_t(o.vec2Base) + o.vecLength;
+              if ((o.isV1V2Const & 2) ? e1 > p.constVecPool.size() : e1 > vp) return bad(mat + "vector operand out of range");
+              if ((o.isV1V2Const & 1) ? e2 > p.constVecPool.size() : e2 > vp) return bad(mat + "vector operand out of range");
+              uint32_t we = o.opName <= VEC_ARITH_MUL ? uint32_t(o.result) + o.vecLength : uint32_t(o.result) + 1;
+              if (we > vp) return bad(mat + "vector result out of range");
+            }
+          if (k == 3)
+            for (const auto &o : m.memRead)
+              if (!opnd(o.en) || uint32_t(o.addrVec) + 8 > vp || !slot(o.result)) return bad(mat + "memRead operand out of range");
+        }
+        accessOf(m, acc[mi]);
+        for (const auto &w : acc[mi].writes) {
+          uint64_t tag = (uint64_t(mi) << 32) | w.second;
+          auto ins = writer.emplace(w.first, tag);
+          if (!ins.second && ins.first->second != tag)
+            return bad(mat + "slot " + std::to_string(w.first) + " written twice in one exec level");
+        }
+      }
+      for (size_t mi = 0; mi < lv.size(); ++mi)
+        for (const auto &r : acc[mi].reads) {
+          auto it = writer.find(r.first);
+          if (it == writer.end()) continue;
+          uint32_t wm = uint32_t(it->second >> 32), wl = uint32_t(it->second);
+          if (wm != mi || (!lv[mi].isLUTPart && wl != r.second))
+            return bad(at + "level " + std::to_string(li) + ": slot " + std::to_string(r.first) +
+                       " read and written by concurrent ops");
+        }
+    }
+    const auto &ew = p.op_last_exchangeWrite;
+    if (ew.count) {
+      uint32_t w = (uint32_t(ew.count) + 3) / 4;
+      if (((ew.dat >> 2) + w) * 4 > vpw || ((ew.exchange >> 2) + w) * 4 > exgN) return bad(at + "exchange write out of range");
+    } else {
+      const auto &rw = p.op_last_regWrite;
+      if (rw.count) {
+        uint32_t w = (uint32_t(rw.count) + 3) / 4;
+        if (((rw.dat >> 2) + w) * 4 > vpw || ((rw.reg >> 2) + w) * 4 > regN) return bad(at + "register write out of range");
+      }
+      for (const auto &o : p.ops_last_memWrite)
+        if (!opnd(o.en) || !opnd(o.dat) || uint32_t(o.addrVec) + 8 > vp) return bad(at + "memWrite operand out of range");
+      for (const auto &o : p.ops_last_print)
+        if (!opnd(o.en) || o.msg >= d.printMsgs.size()) return bad(at + "print operand out of range");
+      for (const auto &o : p.ops_last_stop)
+        if (!opnd(o.en)) return bad(at + "stop operand out of range");
+    }
+  }
+  return TGS_OK;
+}
+
+void programStats(const SimDesignInfo &d, uint64_t nInputs, tgs_program_stats *s) {
+  std::memset(s, 0, sizeof(*s));
+  if (d.regionPartitionIds.size() == 2) {
+    s->n_parts[0] = d.regionPartitionIds[0].size();
+    s->n_parts[1] = d.regionPartitionIds[1].size();
+  }
+  uint64_t bs = 0;
+  for (const auto &p : d.parts) {
+    s->n_regread += p.ops_l0_regRead.size();
+    s->n_exgread += p.ops_l0_exchangeRead.size();
+    s->n_exec_levels += p.exec_mParts.size();
+    s->max_value_pool = std::max<uint64_t>(s->max_value_pool, p.valuePoolSize);
+    for (const auto &lv : p.exec_mParts)
+      for (const auto &m : lv) {
+        ++s->n_microparts;
+        if (m.isLUTPart) {
+          s->n_top += m.topLevel.size();
+          for (const auto &l : m.middleLevels) s->n_mid += l.size();
+          s->n_last += m.lastLevel.size();
+        } else {
+          int k = otherKind(m);
+          if (k == 1) s->n_vecread += m.vecRead.size();
+          if (k == 2) s->n_vecop += m.vecArithAndLogic.size();
+          if (k == 3) s->n_memread += m.memRead.size();
+        }
+      }
+    if (p.op_last_exchangeWrite.count) {
+      ++s->n_exgwrite;
+      bs += p.op_last_exchangeWrite.count;
+    } else {
+      if (p.op_last_regWrite.count) {
+        ++s->n_regwrite;
+        bs += p.op_last_regWrite.count;
+      }
+      s->n_memwrite += p.ops_last_memWrite.size();
+      s->n_print += p.ops_last_print.size();
+      s->n_stop += p.ops_last_stop.size();
+    }
+  }
+  s->b_prog = 8 * s->n_top + 4 * s->n_mid + 4 * s->n_last + 20 * s->n_vecread + 10 * s->n_vecop + 24 * s->n_memread +
+              8 * s->n_regread + 8 * s->n_exgread + 8 * s->n_regwrite + 8 * s->n_exgwrite + 24 * s->n_memwrite +
+              4 * s->n_print + 2 * s->n_stop;
+  s->b_state = s->n_regread + s->n_exgread + bs + s->n_memread + s->n_memwrite + nInputs;
+  s->reg_pool = d.regPoolSize;
+  s->mem_pool = d.memPoolSize;
+  s->exchange_pool = d.exchangePoolSize;
+}
+
+int buildStimMap(const SimDesignInfo &d, const uint32_t *ids, uint32_t n, StimMap *m, std::string *err) {
+  m->ids.clear();
+  m->inputOf.clear();
+  if (n == 0) return TGS_OK;
+  m->inputOf.assign(d.regPoolSize, -1);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (ids[i] >= d.regPoolSize) {
+      setErr(err, "stimulus input id out of range");
+      return TGS_E_ARG;
+    }
+    if (m->inputOf[ids[i]] >= 0) {
+      setErr(err, "duplicate stimulus input id");
+      return TGS_E_ARG;
+    }
+    m->inputOf[ids[i]] = static_cast<int32_t>(i);
+    m->ids.push_back(ids[i]);
+  }
+  for (const auto &p : d.parts) {
+    if (p.op_last_exchangeWrite.count || !p.op_last_regWrite.count) continue;
+    uint32_t w = (uint32_t(p.op_last_regWrite.count) + 3) / 4;
+    uint32_t lo = (p.op_last_regWrite.reg >> 2) * 4, hi = lo + 4 * w;
+    for (uint32_t r = lo; r < hi && r < d.regPoolSize; ++r)
+      if (m->inputOf[r] >= 0) {
+        setErr(err, "a stimulus input is overwritten by a register write");
+        m->ids.clear();
+        m->inputOf.clear();
+        return TGS_E_ARG;
+      }
+  }
+  return TGS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Plain image (reference layout, SimEval.cu:69-148,1143-1281)
+
+namespace {
+void align16(std::vector<uint8_t> &b) {
+  while (b.size() % 16) b.push_back(0);
+}
+template <typename T>
+void put(std::vector<uint8_t> &b, const T &v) {
+  const uint8_t *p = reinterpret_cast<const uint8_t *>(&v);
+  b.insert(b.end(), p, p + sizeof(T));
+}
+template <typename T>
+void putVec(std::vector<uint8_t> &b, const std::vector<T> &v) {
+  for (const auto &e : v) put(b, e);
+}
+}  // namespace
+
+PlainImage buildPlainImage(const SimDesignInfo &d, const StimMap &stim) {
+  PlainImage img;
+  img.n0 = d.regionPartitionIds[0].size();
+  img.nTotal = d.parts.size();
+  img.maxRegion = std::max<uint32_t>(d.regionPartitionIds[0].size(), d.regionPartitionIds[1].size());
+  for (const auto &p : d.parts) {
+    PlainPart pp{};
+    align16(img.net);
+    pp.net = static_cast<uint32_t>(img.net.size());
+    std::vector<uint8_t> b;
+    img.maxVp = std::max(img.maxVp, p.valuePoolSize);
+    std::vector<CGRegReadMetaInfo> rr;
+    std::vector<std::pair<uint32_t, uint32_t>> sr;
+    for (const auto &o : p.ops_l0_regRead) {
+      int32_t in = stim.lookup(o.reg);
+      if (in >= 0) sr.emplace_back(uint32_t(in), o.result);
+      else rr.push_back(o);
+    }
+    pp.nRegRead = rr.size();
+    pp.offRegRead = b.size();
+    putVec(b, rr);
+    align16(b);
+    pp.nExgRead = p.ops_l0_exchangeRead.size();
+    pp.offExgRead = b.size();
+    putVec(b, p.ops_l0_exchangeRead);
+    align16(b);
+    pp.nStimRead = sr.size();
+    pp.offStimRead = b.size();
+    for (auto &s : sr) {
+      put(b, s.first);
+      put(b, s.second);
+    }
+    align16(b);
+    pp.nLevels = p.exec_mParts.size();
+    pp.levelBase = img.levels.size() / 2;
+    for (const auto &lv : p.exec_mParts) {
+      img.levels.push_back(lv.size());
+      img.levels.push_back(img.mparts.size());
+      for (const auto &m : lv) {
+        img.mparts.push_back(b.size());
+        if (m.isLUTPart) {
+          put(b, kMagicLut);
+          put(b, uint32_t(m.middleLevels.size()));
+          uint8_t sizes[160] = {0};
+          sizes[0] = static_cast<uint8_t>(m.topLevel.size());
+          for (size_t l = 0; l < m.middleLevels.size(); ++l) sizes[1 + l] = static_cast<uint8_t>(m.middleLevels[l].size());
+          sizes[1 + m.middleLevels.size()] = static_cast<uint8_t>(m.lastLevel.size());
+          b.insert(b.end(), sizes, sizes + 160);
+          putVec(b, m.topLevel);
+          for (const auto &l : m.middleLevels) putVec(b, l);
+          putVec(b, m.lastLevel);
+        } else {
+          int k = otherKind(m);
+          if (k == 1) {
+            put(b, kMagicVecRead);
+            put(b, uint32_t(m.vecRead.size()));
+            putVec(b, m.vecRead);
+          } else if (k == 2) {
+            put(b, kMagicVecOp);
+            put(b, uint32_t(m.vecArithAndLogic.size()));
+            putVec(b, m.vecArithAndLogic);
+          } else {
+            put(b, kMagicMemRead);
+            put(b, uint32_t(m.memRead.size()));
+            putVec(b, m.memRead);
+          }
+        }
+        align16(b);
+      }
+    }
+    pp.offLast = b.size();
+    pp.hasExgWrite = p.op_last_exchangeWrite.count != 0;
+    pp.hasRegWrite = !pp.hasExgWrite && p.op_last_regWrite.count != 0;
+    pp.offRegWrite = b.size();
+    put(b, p.op_last_regWrite);
+    align16(b);
+    pp.offExgWrite = b.size();
+    put(b, p.op_last_exchangeWrite);
+    align16(b);
+    pp.nMemWrite = p.ops_last_memWrite.size();
+    pp.offMemWrite = b.size();
+    putVec(b, p.ops_last_memWrite);
+    align16(b);
+    pp.nPrint = p.ops_last_print.size();
+    pp.offPrint = b.size();
+    putVec(b, p.ops_last_print);
+    align16(b);
+    pp.nStop = p.ops_last_stop.size();
+    pp.offStop = b.size();
+    putVec(b, p.ops_last_stop);
+    align16(b);
+    pp.constBase = img.consts.size();
+    pp.nConst = p.constVecPool.size();
+    img.consts.insert(img.consts.end(), p.constVecPool.begin(), p.constVecPool.end());
+    img.net.insert(img.net.end(), b.begin(), b.end());
+    img.parts.push_back(pp);
+  }
+  align16(img.net);
+  if (img.consts.empty()) img.consts.push_back(0);
+  return img;
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 image
+
+Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32_t nWaves) {
+  Wave64Image img;
+  img.n0 = d.regionPartitionIds[0].size();
+  img.nTotal = d.parts.size();
+  for (const auto &p : d.parts) {
+    std::vector<uint32_t> w(H_WORDS, 0);
+    img.maxVp = std::max(img.maxVp, p.valuePoolSize);
+    w[H_VP] = p.valuePoolSize;
+    // L0: register reads, exchange reads, stimulus reads; 2 words each
+    std::vector<std::pair<uint32_t, uint32_t>> rr, sr;
+    for (const auto &o : p.ops_l0_regRead) {
+      int32_t in = stim.lookup(o.reg);
+      if (in >= 0) sr.emplace_back(uint32_t(in), o.result);
+      else rr.emplace_back(o.reg, o.result);
+    }
+    w[H_NRR] = rr.size();
+    w[H_NER] = p.ops_l0_exchangeRead.size();
+    w[H_NSR] = sr.size();
+    w[H_OFF_L0] = w.size();
+    for (auto &e : rr) { w.push_back(e.first); w.push_back(e.second); }
+    for (const auto &o : p.ops_l0_exchangeRead) { w.push_back(o.exchange); w.push_back(o.result); }
+    for (auto &e : sr) { w.push_back(e.first); w.push_back(e.second); }
+
+    // tasks per exec level
+    std::vector<std::vector<uint32_t>> tasks(p.exec_mParts.size());  // task streams (relative words)
+    std::vector<std::vector<uint32_t>> levelTasks;                   // per level: task word vectors
+    std::vector<std::vector<std::vector<uint32_t>>> streams(p.exec_mParts.size());
+    uint64_t cost = 4 + (rr.size() + sr.size() + p.ops_l0_exchangeRead.size()) / 64;
+    for (size_t li = 0; li < p.exec_mParts.size(); ++li) {
+      const auto &lv = p.exec_mParts[li];
+      auto &st = streams[li];
+      std::vector<const CGMicroPartInfo *> luts;
+      std::vector<const CGMicroPartVecRead *> vr;
+      std::vector<const CGMicroPartVecArithOrLogic *> vo;
+      std::vector<const CGMicroPartMemRead *> mr;
+      for (const auto &m : lv) {
+        if (m.isLUTPart) luts.push_back(&m);
+        else {
+          int k = otherKind(m);
+          if (k == 1) for (const auto &o : m.vecRead) vr.push_back(&o);
+          if (k == 2) for (const auto &o : m.vecArithAndLogic) vo.push_back(&o);
+          if (k == 3) for (const auto &o : m.memRead) mr.push_back(&o);
+        }
+      }
+      std::stable_sort(luts.begin(), luts.end(), [](const CGMicroPartInfo *a, const CGMicroPartInfo *b) {
+        return a->middleLevels.size() > b->middleLevels.size();
+      });
+      uint64_t levelCost = 0;
+      for (size_t i = 0; i < luts.size();) {
+        const CGMicroPartInfo *A = luts[i];
+        const CGMicroPartInfo *B = nullptr;
+        if (i + 1 < luts.size() && luts[i + 1]->middleLevels.size() == A->middleLevels.size()) B = luts[i + 1];
+        i += B ? 2 : 1;
+        uint32_t nMid = A->middleLevels.size();
+        std::vector<uint32_t> t;
+        t.push_back(T_LUT2 | nMid << 8);
+        uint32_t ntA = A->topLevel.size(), ntB = B ? B->topLevel.size() : 0;
+        uint32_t nlA = A->lastLevel.size(), nlB = B ? B->lastLevel.size() : 0;
+        t.push_back(ntA | ntB << 8 | nlA << 16 | nlB << 24);
+        for (uint32_t l = 0; l < nMid; l += 2) {
+          uint32_t v = 0;
+          for (uint32_t k = 0; k < 2 && l + k < nMid; ++k) {
+            uint32_t na = A->middleLevels[l + k].size(), nb = B ? B->middleLevels[l + k].size() : 0;
+            v |= (na | nb << 8) << (16 * k);
+          }
+          t.push_back(v);
+        }
+        auto tops = [&](const CGMicroPartInfo *m) {
+          if (!m) return;
+          for (const auto &o : m->topLevel) {
+            t.push_back(uint32_t(o.lutIndex) | uint32_t(o.op0) << 16);
+            t.push_back(uint32_t(o.op1) | uint32_t(o.op2) << 16);
+          }
+        };
+        tops(A);
+        tops(B);
+        for (uint32_t l = 0; l < nMid; ++l) {
+          for (const auto &o : A->middleLevels[l]) t.push_back(o.getPacked());
+          if (B) for (const auto &o : B->middleLevels[l]) t.push_back(o.getPacked());
+        }
+        auto lasts = [&](const CGMicroPartInfo *m) {
+          if (!m) return;
+          for (const auto &o : m->lastLevel) t.push_back(uint32_t(o.result) | uint32_t(o.shuffleId) << 16);
+        };
+        lasts(A);
+        lasts(B);
+        st.push_back(std::move(t));
+        levelCost += nMid + 2;
+      }
+      for (size_t i = 0; i < vr.size(); i += 64) {
+        std::vector<uint32_t> t;
+        uint32_t n = std::min<size_t>(64, vr.size() - i);
+        t.push_back(T_VECREAD | n << 8);
+        for (uint32_t k = 0; k < n; ++k) {
+          const auto &o = *vr[i + k];
+          t.push_back(uint32_t(o.vecBase) | uint32_t(o.vecLength) << 16);
+          t.push_back(uint32_t(o.offset) | uint32_t(o.isConstVec ? 1 : 0) << 16);
+          t.push_back(uint32_t(o.index0) | uint32_t(o.index1) << 16);
+          t.push_back(uint32_t(o.index2) | uint32_t(o.index3) << 16);
+          t.push_back(uint32_t(o.outRangeValue) | uint32_t(o.result) << 16);
+        }
+        st.push_back(std::move(t));
+        levelCost += 4;
+      }
+      for (size_t i = 0; i < vo.size(); i += 64) {
+        std::vector<uint32_t> t;
+        uint32_t n = std::min<size_t>(64, vo.size() - i);
+        t.push_back(T_VECOP | n << 8);
+        uint32_t maxLen = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+          const auto &o = *vo[i + k];
+          t.push_back(uint32_t(o.vec1Base) | uint32_t(o.vec2Base) << 16);
+          t.push_back(uint32_t(o.vecLength) | uint32_t(o.opName) << 16 | uint32_t(o.isV1V2Const) << 24);
+          t.push_back(o.result);
+          maxLen = std::max<uint32_t>(maxLen, o.vecLength);
+        }
+        st.push_back(std::move(t));
+        levelCost += 4 + maxLen;
+      }
+      for (size_t i = 0; i < mr.size(); i += 64) {
+        std::vector<uint32_t> t;
+        uint32_t n = std::min<size_t>(64, mr.size() - i);
+        t.push_back(T_MEMREAD | n << 8);
+        for (uint32_t k = 0; k < n; ++k) {
+          const auto &o = *mr[i + k];
+          t.push_back(uint32_t(o.memBase));
+          t.push_back(uint32_t(o.memBase >> 32));
+          t.push_back(uint32_t(o.en) | uint32_t(o.addrVec) << 16);
+          t.push_back(uint32_t(o.result) | uint32_t(o.hasMultipleWriter ? 1 : 0) << 16);
+        }
+        st.push_back(std::move(t));
+        levelCost += 8;
+      }
+      uint64_t nt = st.size();
+      cost += 2 + (nt ? ((nt + nWaves - 1) / nWaves) * (levelCost / std::max<uint64_t>(nt, 1) + 1) : 0);
+    }
+    w[H_NLEV] = p.exec_mParts.size();
+    w[H_OFF_LEV] = w.size();
+    uint32_t taskIdx = 0;
+    for (size_t li = 0; li < streams.size(); ++li) {
+      w.push_back(taskIdx);
+      taskIdx += streams[li].size();
+      w.push_back(taskIdx);
+    }
+    w[H_OFF_TASK] = w.size();
+    size_t taskTab = w.size();
+    w.resize(w.size() + taskIdx, 0);
+    uint32_t ti = 0;
+    for (auto &lvStreams : streams)
+      for (auto &t : lvStreams) {
+        w[taskTab + ti++] = w.size();
+        w.insert(w.end(), t.begin(), t.end());
+      }
+    // last level
+    const auto &ew = p.op_last_exchangeWrite;
+    const auto &rw = p.op_last_regWrite;
+    w[H_EW_EXG] = ew.exchange;
+    w[H_EW_DC] = uint32_t(ew.dat) | uint32_t(ew.count) << 16;
+    bool exg = ew.count != 0;
+    w[H_RW_REG] = rw.reg;
+    w[H_RW_DC] = exg ? 0 : (uint32_t(rw.dat) | uint32_t(rw.count) << 16);
+    w[H_NMW] = exg ? 0 : p.ops_last_memWrite.size();
+    w[H_NPR] = exg ? 0 : p.ops_last_print.size();
+    w[H_NST] = exg ? 0 : p.ops_last_stop.size();
+    w[H_OFF_LAST] = w.size();
+    if (!exg) {
+      for (const auto &o : p.ops_last_memWrite) {
+        w.push_back(uint32_t(o.memBase));
+        w.push_back(uint32_t(o.memBase >> 32));
+        w.push_back(uint32_t(o.addrVec) | uint32_t(o.dat) << 16);
+        w.push_back(uint32_t(o.en) | uint32_t(o.hasMultipleWriter ? 1 : 0) << 16);
+      }
+      for (const auto &o : p.ops_last_print) w.push_back(uint32_t(o.en) | uint32_t(o.msg) << 16);
+      for (const auto &o : p.ops_last_stop) w.push_back(o.en);
+    }
+    w[H_CONST_BASE] = img.consts.size();
+    w[H_NCONST] = p.constVecPool.size();
+    img.consts.insert(img.consts.end(), p.constVecPool.begin(), p.constVecPool.end());
+    cost += 2 + ((exg ? ew.count : rw.count) / 4 + p.ops_last_memWrite.size()) / 64;
+    img.partOff.push_back(img.words.size());
+    img.partWords.push_back(w.size());
+    img.partCost.push_back(cost);
+    img.words.insert(img.words.end(), w.begin(), w.end());
+    while (img.words.size() % 4) img.words.push_back(0);  // 16-byte aligned partitions
+  }
+  if (img.consts.empty()) img.consts.push_back(0);
+  return img;
+}
+
+Wave64Schedule scheduleWave64(const Wave64Image &img, uint32_t groups) {
+  Wave64Schedule s;
+  s.groups = groups;
+  std::vector<std::vector<uint32_t>> per(groups * 2);
+  std::vector<uint64_t> load(groups, 0);
+  std::vector<uint32_t> words(groups, 0);
+  for (int r = 0; r < 2; ++r) {
+    uint32_t lo = r == 0 ? 0 : img.n0, hi = r == 0 ? img.n0 : img.nTotal;
+    std::vector<uint32_t> ids(hi - lo);
+    std::iota(ids.begin(), ids.end(), lo);
+    std::stable_sort(ids.begin(), ids.end(), [&](uint32_t a, uint32_t b) { return img.partCost[a] > img.partCost[b]; });
+    // min-heap on (region load, total words, group)
+    using E = std::tuple<uint64_t, uint32_t, uint32_t>;
+    std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
+    for (uint32_t g = 0; g < groups; ++g) pq.emplace(0, words[g], g);
+    for (uint32_t id : ids) {
+      auto [l, wd, g] = pq.top();
+      pq.pop();
+      per[g * 2 + r].push_back(id);
+      l += img.partCost[id];
+      words[g] += img.partWords[id];
+      load[g] = std::max<uint64_t>(load[g], 0) + img.partCost[id];
+      pq.emplace(l, words[g], g);
+    }
+  }
+  s.off.push_back(0);
+  for (uint32_t g = 0; g < groups; ++g) {
+    uint32_t lds = 0;
+    for (int r = 0; r < 2; ++r) {
+      for (uint32_t id : per[g * 2 + r]) {
+        s.parts.push_back(id);
+        s.ldsOff.push_back(lds);
+        lds += img.partWords[id];
+      }
+      s.off.push_back(s.parts.size());
+    }
+    s.maxGroupWords = std::max(s.maxGroupWords, lds);
+    s.maxGroupCost = std::max(s.maxGroupCost, load[g]);
+  }
+  return s;
+}
+
+}  // namespace tgs

@@ -1,0 +1,64 @@
+// Host-side program checks, statistics and device-image builders.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ToucanGPUGenDataTypes.h"
+#include "device_layout.h"
+#include "toucan_rt.h"
+
+namespace tgs {
+
+// Format invariants the reference asserts (SimEval.cu:92-97,1122,1157-1160,
+// 1183,1292-1311; ToucanSim.cu:153) plus index ranges and the exec-level
+// hazard rule: micropartitions of one exec level run concurrently, so none may
+// write a slot another one reads or writes. Returns TGS_OK or TGS_E_INVALID.
+int validateProgram(const toucanGPUSim::SimDesignInfo &d, std::string *err);
+
+// Record and nibble counts; b_prog / b_state follow the F1 record sizes.
+void programStats(const toucanGPUSim::SimDesignInfo &d, uint64_t nInputs, tgs_program_stats *s);
+
+// Stimulus: regPool id -> input index (or -1). Empty when stimulus is off.
+struct StimMap {
+  std::vector<uint32_t> ids;
+  std::vector<int32_t> inputOf;  // sized regPoolSize when ids non-empty
+  int32_t lookup(uint32_t reg) const { return inputOf.empty() ? -1 : inputOf[reg]; }
+};
+// Rejects ids out of range, duplicates, and inputs covered by a register write.
+int buildStimMap(const toucanGPUSim::SimDesignInfo &d, const uint32_t *ids, uint32_t n, StimMap *m,
+                 std::string *err);
+
+struct PlainImage {
+  std::vector<uint8_t> net;
+  std::vector<PlainPart> parts;
+  std::vector<uint32_t> levels;   // pairs {nMParts, mpartBase}
+  std::vector<uint32_t> mparts;   // byte offsets within the partition stream
+  std::vector<uint8_t> consts;
+  uint32_t maxVp = 0, maxRegion = 0;
+  uint32_t n0 = 0, nTotal = 0;
+};
+PlainImage buildPlainImage(const toucanGPUSim::SimDesignInfo &d, const StimMap &stim);
+
+struct Wave64Image {
+  std::vector<uint32_t> words;
+  std::vector<uint32_t> partOff, partWords;
+  std::vector<uint64_t> partCost;
+  std::vector<uint8_t> consts;
+  uint32_t maxVp = 0;
+  uint32_t n0 = 0, nTotal = 0;
+};
+Wave64Image buildWave64Image(const toucanGPUSim::SimDesignInfo &d, const StimMap &stim, uint32_t nWaves);
+
+struct Wave64Schedule {
+  uint32_t groups = 0;
+  std::vector<uint32_t> off;    // groups*2+1 entries into parts/ldsOff
+  std::vector<uint32_t> parts;  // partition ids
+  std::vector<uint32_t> ldsOff; // word offset of each scheduled partition in the group's LDS program area
+  uint32_t maxGroupWords = 0;   // largest per-group program (words)
+  uint64_t maxGroupCost = 0;
+};
+// Longest-processing-time assignment of partitions to `groups` workgroups per region.
+Wave64Schedule scheduleWave64(const Wave64Image &img, uint32_t groups);
+
+}  // namespace tgs

@@ -1,0 +1,262 @@
+"""Python mirror of toucanGPUSim::ToucanSimulator over the C-ABI (libtoucan_rt.so).
+
+Method names and meanings follow include/ToucanGPUSim.h:16-34 of the
+reference (init / eval / eval_free_running / setRegister / readRegister /
+readSignal / cleanup), so parity tests read like the reference's own usage
+in main.cpp:43-60. Errors the reference reports by return code keep that
+form (``init`` returns -1 on a missing file or an LDS limit); HIP and format
+errors raise ``SimError`` instead of exiting the process.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _native
+from ._native import (KERNEL_BATCH, KERNEL_PLAIN, KERNEL_WAVE64, POOL_EXCHANGE, POOL_MEM, POOL_REG,
+                      TGS_E_IO, TGS_E_LIMIT, TGS_OK)
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class SimError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def program_stats(design: bytes) -> dict:
+    """Record counts and the per-cycle byte model of a serialized design (no GPU needed)."""
+    lib = _native.rt_lib()
+    st = _native.ProgramStats()
+    err = ctypes.create_string_buffer(512)
+    rc = lib.tgs_program_stats_from_bytes(_native.as_u8p(design), len(design), ctypes.byref(st), err, len(err))
+    if rc != TGS_OK:
+        raise SimError(rc, err.value.decode())
+    return {n: (list(getattr(st, n)) if n == "n_parts" else getattr(st, n)) for n, _ in _native.ProgramStats._fields_}
+
+
+class Simulator:
+    """One design resident on one GPU, with any number of stimulus instances."""
+
+    def __init__(self, gpu_id=0):
+        self._lib = _native.rt_lib()
+        self._h = ctypes.c_void_p()
+        rc = self._lib.tgs_create(gpu_id, ctypes.byref(self._h))
+        if rc != TGS_OK:
+            raise SimError(rc, "no usable GPU (cooperative launch required)")
+        self.symbols = None
+        self.cycle_cnt = 0
+        self._design = None
+        self._symbols_blob = None
+        self._vcd = None
+
+    # -- helpers -------------------------------------------------------------
+    def _err(self):
+        buf = ctypes.create_string_buffer(1024)
+        self._lib.tgs_last_error(self._h, buf, len(buf))
+        return buf.value.decode(errors="replace")
+
+    def _check(self, rc):
+        if rc != TGS_OK:
+            raise SimError(rc, self._err())
+
+    # -- reference entry points ---------------------------------------------
+    def init(self, gpu_id, designBin, symbolBin, shouldRandomize=False, enablePrint=False):
+        """ToucanSimulator::init (src/ToucanSim.cu:102-204). Paths or bytes; returns 0 / -1."""
+        design = designBin
+        symbols = symbolBin
+        try:
+            if isinstance(design, (str, os.PathLike)):
+                with open(design, "rb") as f:
+                    design = f.read()
+            if isinstance(symbols, (str, os.PathLike)):
+                with open(symbols, "rb") as f:
+                    symbols = f.read()
+        except OSError:
+            return -1
+        rc = self._lib.tgs_load(self._h, _native.as_u8p(design), len(design),
+                                _native.as_u8p(symbols) if symbols else None, len(symbols) if symbols else 0,
+                                1 if shouldRandomize else 0, 1 if enablePrint else 0)
+        if rc in (TGS_E_IO, TGS_E_LIMIT):
+            return -1
+        self._check(rc)
+        self._design = design
+        self._symbols_blob = symbols
+        self.symbols = parse_symbols(symbols) if symbols else None
+        self.cycle_cnt = 0
+        return 0
+
+    def eval(self):
+        """One cycle (src/ToucanSim.cu:206-230); returns the sticky stop flag."""
+        st = ctypes.c_int()
+        self._check(self._lib.tgs_eval(self._h, ctypes.byref(st)))
+        self.cycle_cnt += 1
+        if self._vcd is not None:
+            self._vcd.dump(self.cycle_cnt)
+        return bool(st.value)
+
+    def eval_free_running(self, max_cycles):
+        """N cycles in one launch (src/ToucanSim.cu:232-267); returns the stop flag."""
+        st = ctypes.c_int()
+        self._check(self._lib.tgs_eval_free_running(self._h, int(max_cycles), ctypes.byref(st)))
+        self.cycle_cnt += int(max_cycles)
+        return bool(st.value)
+
+    def setRegister(self, name, value, instance=0):
+        """write_reg_to_gpu (src/SimEval.cu:1072-1081): 4 bits per fragment, last fragment first."""
+        locs = self.symbols["reg"][name]
+        if len(locs) > 16:
+            raise ValueError("register wider than 64 bits")
+        ids = np.array([l[0] for l in locs], dtype=np.uint32)
+        vals = np.zeros(len(locs), dtype=np.uint8)
+        v = int(value)
+        for k in range(len(locs) - 1, -1, -1):
+            vals[k] = v & 0xF
+            v >>= 4
+        if v != 0:
+            raise ValueError("Given value is wider than register")
+        self._check(self._lib.tgs_write_regs(self._h, instance, ids.ctypes.data_as(_u32p), len(ids),
+                                             vals.ctypes.data_as(_u8p)))
+
+    def readRegister(self, name, instance=0):
+        """read_reg_from_gpu (src/SimEval.cu:1058-1069): r = r << 4 | nibble, in list order."""
+        locs = self.symbols["reg"][name]
+        if len(locs) > 16:
+            raise ValueError("register wider than 64 bits")
+        ids = np.array([l[0] for l in locs], dtype=np.uint32)
+        out = np.zeros(len(locs), dtype=np.uint8)
+        self._check(self._lib.tgs_read_regs(self._h, instance, ids.ctypes.data_as(_u32p), len(ids),
+                                            out.ctypes.data_as(_u8p)))
+        r = 0
+        for x in out:
+            r = (r << 4) | int(x)
+        return r
+
+    def readSignal(self, name):
+        """Unimplemented in the reference (src/ToucanSim.cu:291-300): returns UINT64_MAX."""
+        return (1 << 64) - 1
+
+    def enableVCDDump(self, path):
+        """VCD writer (unsupport/SimVCD.cpp:99-241) over register snapshots after each eval()."""
+        from .vcd import VcdWriter
+        self._vcd = VcdWriter(self, path)
+
+    def cleanup(self):
+        if self._vcd is not None:
+            self._vcd.close()
+            self._vcd = None
+        if getattr(self, "_h", None):
+            self._lib.tgs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.cleanup()
+        except Exception:
+            pass
+
+    # -- extensions over the reference -------------------------------------
+    def set_kernel(self, kernel):
+        self._check(self._lib.tgs_set_kernel(self._h, kernel))
+
+    def set_print(self, enable):
+        self._check(self._lib.tgs_set_print(self._h, 1 if enable else 0))
+
+    def set_instances(self, n):
+        self._check(self._lib.tgs_set_instances(self._h, int(n)))
+
+    def set_stimulus(self, input_ids, seed):
+        ids = np.ascontiguousarray(input_ids, dtype=np.uint32)
+        self._check(self._lib.tgs_set_stimulus(self._h, ids.ctypes.data_as(_u32p), ids.size, int(seed)))
+
+    def run_batch(self, cycles):
+        c = np.ascontiguousarray(cycles, dtype=np.uint32)
+        st = ctypes.c_int()
+        self._check(self._lib.tgs_run_batch(self._h, c.ctypes.data_as(_u32p), c.size, ctypes.byref(st)))
+        return bool(st.value)
+
+    def pool_size(self, pool):
+        n = ctypes.c_uint64()
+        self._check(self._lib.tgs_pool_size(self._h, pool, ctypes.byref(n)))
+        return n.value
+
+    def read_pool(self, pool, instance=0):
+        n = self.pool_size(pool)
+        out = np.zeros(n, dtype=np.uint8)
+        self._check(self._lib.tgs_read_pool(self._h, instance, pool, 0, n, out.ctypes.data_as(_u8p)))
+        return out
+
+    def write_pool(self, pool, data, instance=0, offset=0):
+        a = np.ascontiguousarray(data, dtype=np.uint8)
+        self._check(self._lib.tgs_write_pool(self._h, instance, pool, offset, a.size, a.ctypes.data_as(_u8p)))
+
+    def stopped(self, instance=0):
+        st = ctypes.c_int()
+        self._check(self._lib.tgs_get_stop(self._h, instance, ctypes.byref(st)))
+        return bool(st.value)
+
+    def stop_cycle(self, instance=0):
+        c = ctypes.c_uint64()
+        self._check(self._lib.tgs_get_stop_cycle(self._h, instance, ctypes.byref(c)))
+        return c.value
+
+    def cycles(self, instance=0):
+        c = ctypes.c_uint64()
+        self._check(self._lib.tgs_get_cycles(self._h, instance, ctypes.byref(c)))
+        return c.value
+
+    def print_log(self, instance=0):
+        n = ctypes.c_uint64()
+        self._check(self._lib.tgs_get_print_log(self._h, instance, None, None, 0, ctypes.byref(n)))
+        k = n.value
+        cyc = np.zeros(max(k, 1), dtype=np.uint64)
+        msg = np.zeros(max(k, 1), dtype=np.uint32)
+        self._check(self._lib.tgs_get_print_log(self._h, instance, cyc.ctypes.data_as(_u64p),
+                                                msg.ctypes.data_as(_u32p), k, ctypes.byref(n)))
+        return sorted(zip(cyc[:k].tolist(), msg[:k].tolist()))
+
+    def profile(self):
+        p = _native.Profile()
+        self._check(self._lib.tgs_get_profile(self._h, ctypes.byref(p)))
+        return {n: getattr(p, n) for n, _ in _native.Profile._fields_}
+
+    def reset_profile(self):
+        self._check(self._lib.tgs_reset_profile(self._h))
+
+    def program_stats(self):
+        st = _native.ProgramStats()
+        self._check(self._lib.tgs_get_program_stats(self._h, ctypes.byref(st)))
+        return {n: (list(getattr(st, n)) if n == "n_parts" else getattr(st, n)) for n, _ in _native.ProgramStats._fields_}
+
+
+def parse_symbols(blob: bytes) -> dict:
+    """GPUSimIOSymbols.bin (src/Serializer.cpp:264-354) -> {'reg','signal','mem'} dicts."""
+    pos = 0
+
+    def u64():
+        nonlocal pos
+        v = int.from_bytes(blob[pos:pos + 8], "little")
+        pos += 8
+        return v
+
+    def u32():
+        nonlocal pos
+        v = int.from_bytes(blob[pos:pos + 4], "little")
+        pos += 4
+        return v
+
+    out = {}
+    for key, k in (("reg", 2), ("signal", 3), ("mem", 3)):
+        m = {}
+        for _ in range(u64()):
+            n = u64()
+            name = blob[pos:pos + n].decode()
+            pos += n
+            m[name] = [tuple(u32() for _ in range(k)) for _ in range(u64())]
+        out[key] = m
+    if pos != len(blob):
+        raise ValueError("trailing bytes in symbol file")
+    return out
