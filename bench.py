@@ -1,21 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark: simulated cycles/s of the 1M-LUT4 design on one MI355X.
+"""Benchmark: simulated cycles/s of the 1M-LUT4 design on one MI355X, and the
+batched stimulus-cycles/s of the 200K-LUT4 design, each beside the plain
+one-instance-per-launch kernel.
 
-Workload (single mode, default): seeded synthetic design of 1,000,000 LUT4s
+Single leg (the headline `value`): seeded synthetic design of 1,000,000 LUT4s
 plus 20,000 word ops at depth 64, 50,000 registers, 16 SRAMs of 4096 x 64
 bit, 128 xorshift64-driven inputs, 100,000 cycles. A step is one
-eval_free_running call of --chunk cycles (default 10,000); --steps 10 covers
-the 100,000 cycles. Inputs are resident in HBM before timing starts.
+eval_free_running call of --chunk cycles (default 10,000); the default 10
+steps cover the 100,000 cycles. Inputs are resident in HBM before timing.
 
-Batched mode (--mode batched): the 200,000-LUT4 design run as --instances
-stimulus instances with uneven cycle counts; reports aggregate
-stimulus-cycles/s.
+Batched leg: the 200,000-LUT4 design (depth 48, 8 SRAMs of 1024 x 32 bit)
+run as --instances instances (default 1,024) with distinct xorshift64 seeds
+and cycle counts drawn uniformly from [1,000, 50,000], in one persistent
+launch. An autotuner first sweeps the partition size (256..4,096 LUT4s) and
+the workgroup size on a short run; the logical design is identical for every
+partition size, so results do not depend on its choice.
 
-Baselines reported beside the value:
-  * plain: the reference-execution-model kernel on the same design (a
-    bounded cycle count, rate is per cycle);
-  * cpu_baseline: the C oracle on the GPU box's host, 1 thread, on a bounded
-    number of cycles of the same workload (checker code, measured only).
+Baselines: the plain kernel (reference execution model) on the same design,
+timed on a bounded cycle count (its rate is per cycle), and the C oracle on
+the host (1 thread, bounded sample) as cpu_baseline. The oracle is the
+checker; it is timed, never used to produce a GPU result.
 """
 import argparse
 import json
@@ -34,23 +38,29 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["single", "batched"], default="single")
-    ap.add_argument("--chunk", type=int, default=10_000, help="cycles per step (single mode)")
+    ap.add_argument("--mode", choices=["single", "batched", "both"], default="both")
+    ap.add_argument("--chunk", type=int, default=10_000, help="cycles per step (single leg)")
     ap.add_argument("--plain-cycles", type=int, default=1_000)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--instances", type=int, default=1024)
+    ap.add_argument("--min-cycles", type=int, default=1_000)
+    ap.add_argument("--max-cycles", type=int, default=50_000)
+    ap.add_argument("--tune-cycles", type=int, default=100)
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
 
-def cpu_baseline(design, inputs, seed, seconds):
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def oracle_rate(design, inputs, seed, seconds):
     """Oracle cycles/s on the host (1 thread), bounded to about `seconds`."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    prog = O.Program(design)
-    st = O.State(prog)
+    st = O.State(O.Program(design))
     st.set_stimulus(inputs, seed)
     t0 = time.perf_counter()
     st.run(2)
@@ -58,104 +68,197 @@ def cpu_baseline(design, inputs, seed, seconds):
     n = max(2, int(seconds / per))
     t0 = time.perf_counter()
     st.run(n)
+    return n, time.perf_counter() - t0, st
+
+
+def plain_rate(d, cycles, seed):
+    from toucangpurt_amd import KERNEL_PLAIN, rt
+    ps = rt.Simulator(0)
+    assert ps.init(0, d.design, d.symbols, False, False) == 0
+    ps.set_kernel(KERNEL_PLAIN)
+    ps.set_stimulus(d.inputs, seed)
+    ps.eval_free_running(10)
+    ps.reset_profile()
+    t1 = time.perf_counter()
+    ps.eval_free_running(cycles)
+    dt = time.perf_counter() - t1
+    prof = ps.profile()
+    ps.cleanup()
+    return {"value": cycles / dt, "unit": "simulated cycles/s", "cycles": cycles, "seconds": dt,
+            "kernel_ms": prof["kernel_ms"], "grid": prof["grid"], "block": prof["block"]}
+
+
+def run_single(args):
+    import numpy as np
+    from toucangpurt_amd import KERNEL_WAVE64, POOL_REG, gen, rt
+    d = gen.generate("lut1m", seed=args.seed)
+    log(f"single: design generated {d.stats['n_parts']} partitions")
+    sim = rt.Simulator(0)
+    assert sim.init(0, d.design, d.symbols, False, False) == 0
+    sim.set_kernel(KERNEL_WAVE64)
+    sim.set_stimulus(d.inputs, args.seed)
+    stats = sim.program_stats()
+    for _ in range(args.warmup):
+        sim.eval_free_running(args.chunk)
+    sim.reset_profile()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.eval_free_running(args.chunk)
     dt = time.perf_counter() - t0
-    return n, dt, st
+    prof = sim.profile()
+    sim.cleanup()
+    cycles = args.steps * args.chunk
+    launch_ms = prof["kernel_ms"] / max(prof["launches"], 1)
+    cyc_per_launch = prof["cycles"] / max(prof["launches"], 1)
+    bytes_per_cycle = stats["b_prog"] + stats["b_state"]
+    achieved = bytes_per_cycle * cyc_per_launch / (launch_ms / 1e3) / 1e9
+    log(f"single: {cycles / dt:.0f} cycles/s, launch {launch_ms:.1f} ms")
+    out = {
+        "value": cycles / dt, "dt": dt, "cycles": cycles,
+        "config": {"workload": "lut1m: 1,000,000 LUT4 + 20,000 word ops, depth 64, 50,000 regs, "
+                               "16 SRAM 4096x64b, 128 inputs, 100,000 cycles",
+                   "cycles": cycles, "cycles_per_step": args.chunk, "kernel": "wave64", "grid": prof["grid"],
+                   "block": prof["block"], "lds_bytes": prof["lds_bytes"],
+                   "program_resident_in_lds": bool(prof["resident"])},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cycle": bytes_per_cycle,
+                     "b_prog": stats["b_prog"], "b_state": stats["b_state"], "launch_ms": launch_ms,
+                     "cycles_per_launch": cyc_per_launch},
+        "barrier_stall_frac": (prof["barrier_ticks"] / prof["total_ticks"]) if prof["total_ticks"] else None,
+    }
+    if not args.no_plain:
+        out["plain"] = plain_rate(d, args.plain_cycles, args.seed)
+        log(f"single: plain {out['plain']['value']:.0f} cycles/s")
+    if not args.no_cpu:
+        n, cdt, st = oracle_rate(d.design, d.inputs, args.seed, args.cpu_seconds)
+        out["cpu"] = {"value": n / cdt, "unit": "simulated cycles/s", "cores": 1, "kind": "port",
+                      "sample": f"{n + 2} cycles of the 1M-LUT4 design with the same stimulus, C oracle, 1 thread"}
+        chk = rt.Simulator(0)
+        assert chk.init(0, d.design, d.symbols, False, False) == 0
+        chk.set_kernel(KERNEL_WAVE64)
+        chk.set_stimulus(d.inputs, args.seed)
+        chk.eval_free_running(n + 2)
+        out["parity"] = bool(np.array_equal(chk.read_pool(POOL_REG), st.pool(0)))
+        chk.cleanup()
+        log(f"single: cpu {out['cpu']['value']:.1f} cycles/s, parity {out['parity']}")
+    return out
+
+
+def run_batched(args):
+    import numpy as np
+    from toucangpurt_amd import KERNEL_BATCH, POOL_REG, gen, rt
+    from toucangpurt_amd.rt import SimError
+    n = args.instances
+    rng = np.random.default_rng(args.seed)
+    cycles = rng.integers(args.min_cycles, args.max_cycles + 1, size=n).astype(np.uint32)
+    # autotune: partition size x workgroup size, on a short uniform run
+    best = None
+    tuning = []
+    designs = {}
+    for part in (256, 512, 1024, 2048, 4096):
+        d = gen.generate("lut200k", seed=args.seed, part_luts=part)
+        designs[part] = d
+        for block in (256, 512):
+            sim = rt.Simulator(0)
+            assert sim.init(0, d.design, d.symbols, False, False) == 0
+            try:
+                sim.set_launch(KERNEL_BATCH, block, 0)
+                sim.set_kernel(KERNEL_BATCH)
+                sim.set_instances(n)
+                sim.set_stimulus(d.inputs, args.seed)
+                sim.run_batch(np.full(n, 10, np.uint32))
+                t0 = time.perf_counter()
+                sim.run_batch(np.full(n, args.tune_cycles, np.uint32))
+                rate = n * args.tune_cycles / (time.perf_counter() - t0)
+            except SimError as e:
+                tuning.append({"part_luts": part, "block": block, "error": str(e)[:80]})
+                sim.cleanup()
+                continue
+            tuning.append({"part_luts": part, "block": block, "rate": rate})
+            log(f"batched tune part_luts={part} block={block}: {rate:.3e} stimulus-cycles/s")
+            if best is None or rate > best[0]:
+                best = (rate, part, block)
+            sim.cleanup()
+    _, part, block = best
+    d = designs[part]
+    sim = rt.Simulator(0)
+    assert sim.init(0, d.design, d.symbols, False, False) == 0
+    sim.set_launch(KERNEL_BATCH, block, 0)
+    sim.set_kernel(KERNEL_BATCH)
+    sim.set_instances(n)
+    sim.set_stimulus(d.inputs, args.seed)
+    sim.reset_profile()
+    t0 = time.perf_counter()
+    sim.run_batch(cycles)
+    dt = time.perf_counter() - t0
+    prof = sim.profile()
+    total = int(cycles.sum())
+    stats = sim.program_stats()
+    value = total / dt
+    log(f"batched: {value:.3e} stimulus-cycles/s over {total} instance-cycles in {dt:.2f} s")
+    out = {"value": value, "unit": "stimulus-cycles/s", "instances": n, "total_cycles": total, "seconds": dt,
+           "kernel_ms": prof["kernel_ms"], "part_luts": part, "block": block, "grid": prof["grid"],
+           "lds_bytes": prof["lds_bytes"], "tuning": tuning,
+           "workload": "lut200k: 200,000 LUT4, depth 48, 8 SRAM 1024x32b, 1,024 instances, "
+                       f"cycles uniform in [{args.min_cycles}, {args.max_cycles}]",
+           "bytes_per_cycle_shared": stats["b_prog"], "bytes_per_cycle_per_instance": stats["b_state"]}
+    # roofline: program bytes once per launch-cycle, state bytes per live instance-cycle
+    launch_cycles = int(cycles.max())
+    algo = stats["b_prog"] * launch_cycles + stats["b_state"] * total
+    achieved = algo / (prof["kernel_ms"] / 1e3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS}
+    # spot parity: the instance with the fewest cycles against a fresh single-instance run
+    k = int(np.argmin(cycles))
+    got = sim.read_pool(POOL_REG, k)
+    sim.cleanup()
+    if not args.no_plain:
+        out["plain"] = plain_rate(d, args.plain_cycles, args.seed)
+        out["speedup_vs_plain"] = value / out["plain"]["value"]
+        log(f"batched: plain {out['plain']['value']:.0f} cycles/s")
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        st = O.State(O.Program(d.design))
+        st.set_stimulus(d.inputs, args.seed, k)
+        t0 = time.perf_counter()
+        st.run(int(cycles[k]))
+        cdt = time.perf_counter() - t0
+        out["cpu"] = {"value": int(cycles[k]) / cdt, "unit": "stimulus-cycles/s", "cores": 1, "kind": "port",
+                      "sample": f"instance {k}: {int(cycles[k])} cycles of the 200K-LUT4 design, C oracle, 1 thread"}
+        out["parity_instance"] = k
+        out["parity"] = bool(np.array_equal(got, st.pool(0)))
+        log(f"batched: cpu {out['cpu']['value']:.1f} cycles/s, parity {out['parity']}")
+    return out
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding is described "
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is described "
               "in DESIGN.md and not built", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
-    from toucangpurt_amd import KERNEL_PLAIN, KERNEL_WAVE64, POOL_REG, gen, rt
-
-    if args.mode == "single":
-        d = gen.generate("lut1m", seed=args.seed)
-        workload = "lut1m: 1,000,000 LUT4 + 20,000 word ops, depth 64, 50,000 regs, 16 SRAM 4096x64b, 128 inputs"
-        sim = rt.Simulator(0)
-        assert sim.init(0, d.design, d.symbols, False, False) == 0
-        sim.set_kernel(KERNEL_WAVE64)
-        sim.set_stimulus(d.inputs, args.seed)
-        stats = sim.program_stats()
-        for _ in range(args.warmup):
-            sim.eval_free_running(args.chunk)
-        sim.reset_profile()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            sim.eval_free_running(args.chunk)
-        dt = time.perf_counter() - t0
-        prof = sim.profile()
-        cycles = args.steps * args.chunk
-        value = cycles / dt
-        launch_ms = prof["kernel_ms"] / max(prof["launches"], 1)
-        cyc_per_launch = prof["cycles"] / max(prof["launches"], 1)
-        bytes_per_cycle = stats["b_prog"] + stats["b_state"]
-        achieved = bytes_per_cycle * cyc_per_launch / (launch_ms / 1e3) / 1e9
-        stall = prof["barrier_ticks"] / prof["total_ticks"] if prof["total_ticks"] else None
-        sim.cleanup()
-
-        plain = None
-        if not args.no_plain:
-            ps = rt.Simulator(0)
-            assert ps.init(0, d.design, d.symbols, False, False) == 0
-            ps.set_kernel(KERNEL_PLAIN)
-            ps.set_stimulus(d.inputs, args.seed)
-            ps.eval_free_running(10)
-            t1 = time.perf_counter()
-            ps.eval_free_running(args.plain_cycles)
-            pdt = time.perf_counter() - t1
-            plain = {"value": args.plain_cycles / pdt, "cycles": args.plain_cycles, "seconds": pdt,
-                     "kernel_ms": ps.profile()["kernel_ms"]}
-            ps.cleanup()
-
-        cpu = None
-        parity = None
-        if not args.no_cpu:
-            n, cdt, st = cpu_baseline(d.design, d.inputs, args.seed, args.cpu_seconds)
-            cpu = {"value": n / cdt, "unit": "simulated cycles/s", "cores": 1, "kind": "port",
-                   "sample": f"{n + 2} cycles of the same design and stimulus on the C oracle, 1 thread"}
-            chk = rt.Simulator(0)
-            assert chk.init(0, d.design, d.symbols, False, False) == 0
-            chk.set_kernel(KERNEL_WAVE64)
-            chk.set_stimulus(d.inputs, args.seed)
-            chk.eval_free_running(n + 2)
-            parity = bool(np.array_equal(chk.read_pool(POOL_REG), st.pool(0)))
-            chk.cleanup()
-
-        out = {
-            "metric": "simulated cycles/sec @1M-LUT design",
-            "value": value,
-            "unit": "simulated cycles/s",
-            "n_gpus": 1,
-            "steps": args.steps,
-            "warmup": args.warmup,
-            "ms_per_step": dt / args.steps * 1e3,
-            "higher_is_better": True,
-            "scaling": "weak",
-            "vs_baseline": None,
-            "dtype": "u8",
-            "data": "synthetic",
-            "config": {"workload": workload, "cycles": cycles, "cycles_per_step": args.chunk,
-                       "kernel": "wave64", "grid": prof["grid"], "block": prof["block"],
-                       "lds_bytes": prof["lds_bytes"], "program_resident_in_lds": bool(prof["resident"])},
-            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "bytes_per_cycle": bytes_per_cycle, "b_prog": stats["b_prog"],
-                         "b_state": stats["b_state"], "launch_ms": launch_ms,
-                         "cycles_per_launch": cyc_per_launch},
-            "barrier_stall_frac": stall,
-            "plain_kernel": plain,
-            "speedup_vs_plain": (value / plain["value"]) if plain else None,
-            "cpu_baseline": cpu,
-            "parity_vs_oracle": parity,
-        }
-        print(json.dumps(out))
-        return
-
-    # ---------------- batched ----------------
-    raise SystemExit("batched mode is not built yet")
+    single = run_single(args) if args.mode in ("single", "both") else None
+    batched = run_batched(args) if args.mode in ("batched", "both") else None
+    if single is not None:
+        out = {"metric": "simulated cycles/sec @1M-LUT design; batched stimulus-cycles/sec vs plain",
+               "value": single["value"], "unit": "simulated cycles/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": single["dt"] / args.steps * 1e3, "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+               "config": single["config"], "roofline": single["roofline"],
+               "barrier_stall_frac": single["barrier_stall_frac"],
+               "plain_kernel": single.get("plain"),
+               "speedup_vs_plain": (single["value"] / single["plain"]["value"]) if single.get("plain") else None,
+               "cpu_baseline": single.get("cpu"), "parity_vs_oracle": single.get("parity"), "batched": batched}
+    else:
+        out = {"metric": "batched stimulus-cycles/sec @200K-LUT design x instances", "value": batched["value"],
+               "unit": "stimulus-cycles/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+               "ms_per_step": batched["seconds"] * 1e3, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+               "config": {"workload": batched["workload"], "part_luts": batched["part_luts"]},
+               "roofline": batched["roofline"], "cpu_baseline": batched.get("cpu"),
+               "speedup_vs_plain": batched.get("speedup_vs_plain"), "batched": batched}
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -101,6 +101,10 @@ int tgs_load_files(tgs_ctx *ctx, const char *design_path, const char *symbol_pat
 
 /* Choose the kernel family used by eval / eval_free_running. */
 int tgs_set_kernel(tgs_ctx *ctx, int kernel);
+/* Launch shape of a kernel family: threads per workgroup (multiple of 64,
+ * <= 1024) and workgroups (0 = automatic). The autotuner's knob; the
+ * reference fixes 1024 threads and one block per SM (ToucanSim.cu:56-80). */
+int tgs_set_launch(tgs_ctx *ctx, int kernel, uint32_t block, uint32_t groups);
 /* setEnablePrint (src/SimEval.cu:1095-1097). */
 int tgs_set_print(tgs_ctx *ctx, int enable);
 

@@ -152,7 +152,7 @@ def gen_lib():
 # Exported symbols of libtoucan_rt.so, as declared in include/toucan_rt.h.
 RT_SYMBOLS = [
     "tgs_program_stats_from_bytes", "tgs_initial_state", "tgs_create", "tgs_destroy", "tgs_last_error", "tgs_load",
-    "tgs_load_files", "tgs_set_kernel", "tgs_set_print", "tgs_set_instances", "tgs_set_stimulus",
+    "tgs_load_files", "tgs_set_kernel", "tgs_set_launch", "tgs_set_print", "tgs_set_instances", "tgs_set_stimulus",
     "tgs_eval", "tgs_eval_free_running", "tgs_run_batch", "tgs_read_regs", "tgs_write_regs",
     "tgs_read_pool", "tgs_write_pool", "tgs_pool_size", "tgs_get_stop", "tgs_get_cycles",
     "tgs_get_stop_cycle", "tgs_get_print_log", "tgs_get_profile", "tgs_reset_profile",
@@ -181,6 +181,7 @@ def rt_lib():
         lib.tgs_load_files.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int]
         lib.tgs_set_kernel.argtypes = [vp, ctypes.c_int]
         lib.tgs_set_print.argtypes = [vp, ctypes.c_int]
+        lib.tgs_set_launch.argtypes = [vp, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
         lib.tgs_set_instances.argtypes = [vp, ctypes.c_uint32]
         lib.tgs_set_stimulus.argtypes = [vp, _u32p, ctypes.c_uint32, ctypes.c_uint64]
         lib.tgs_eval.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]

@@ -27,6 +27,34 @@ struct W64Args {
   uint32_t barrierBase;      // barrier index offset (counter is zeroed per launch)
 };
 
+// Batched kernel: pools are instance-minor (nibble k of slot s at k * instPad + s).
+struct BatchArgs {
+  const uint8_t *lut, *consts;
+  uint8_t *reg, *mem, *exg;
+  uint32_t regSize, memSize, exgSize;
+  uint32_t instPad;               // slots, a multiple of 64
+  const uint32_t *words, *partOff;  // wave64 image
+  uint32_t n0, nTotal;
+  uint32_t vpBytes;               // value-pool slots per lane (16-multiple)
+  const uint32_t *runCycles;      // per slot: cycles to run in this launch
+  const uint64_t *cycleBase;      // per slot: cycles completed before this launch
+  const uint32_t *groupsActive;   // per launch cycle: groups with a live slot (a prefix)
+  uint32_t nCycles;               // longest run in this launch
+  uint64_t *xs;                   // per slot xorshift64 state
+  uint64_t *draws;                // [2][nDraws][instPad]
+  const uint32_t *inIds;
+  uint32_t nIn;
+  uint64_t *stop;                 // per slot
+  uint64_t *stopCycle;            // per slot
+  uint32_t *logCount;
+  uint64_t *logCycle;
+  uint64_t *logMsg;               // msg | instance << 32
+  const uint32_t *instOf;         // slot -> instance
+  uint32_t logCap;
+  uint32_t printEnable;
+  uint32_t *barrier;
+};
+
 hipError_t launchPlain(const PlainArgs &args, uint32_t grid, uint32_t block, uint32_t lds, hipStream_t stream);
 hipError_t plainMaxBlocksPerCU(int *blocks, uint32_t block, uint32_t lds);
 hipError_t plainSetLds(uint32_t lds);
@@ -36,5 +64,13 @@ hipError_t launchWave64(const W64Args &args, bool resident, uint32_t grid, uint3
 hipError_t wave64MaxBlocksPerCU(int *blocks, bool resident, uint32_t block, uint32_t lds);
 hipError_t wave64SetLds(uint32_t lds);
 uint32_t wave64LutBytes();
+
+hipError_t launchBatch(const BatchArgs &args, uint32_t grid, uint32_t block, uint32_t lds, hipStream_t stream);
+hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t block, uint32_t lds);
+hipError_t batchSetLds(uint32_t lds);
+hipError_t launchPermute(const uint8_t *in, uint8_t *out, uint64_t rows, uint32_t instPad, const uint32_t *src,
+                         hipStream_t stream);
+hipError_t launchPermute64(const uint64_t *in, uint64_t *out, uint64_t rows, uint32_t instPad, const uint32_t *src,
+                           hipStream_t stream);
 
 }  // namespace tgs

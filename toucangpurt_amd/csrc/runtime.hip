@@ -13,6 +13,7 @@
 #include <fstream>
 #include <iterator>
 #include <memory>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -58,8 +59,33 @@ struct Instance {
   uint64_t cycles = 0;
 };
 
+// Instance-minor state of the batched kernel: pool nibble k of slot s at
+// k * instPad + s. Slots are instances reordered by decreasing cycle count.
+struct BatchState {
+  uint32_t n = 0, instPad = 0;
+  DBuf reg, mem, exg, xs, draws, stop, stopCycle, logCount, logCycle, logMsg, barrier;
+  DBuf runCycles, cycleBase, groupsActive, instOfD, perm, scratch;
+  std::vector<uint32_t> slotOf, instOf;  // instance -> slot, slot -> instance
+  std::vector<uint64_t> cycles;          // per instance
+};
+
+// Host copy of one instance's architectural state (layout conversions).
+struct HostInst {
+  std::vector<uint8_t> reg, mem, exg;
+  uint64_t xs = 0, stopCycle = UINT64_MAX, cycles = 0;
+  uint32_t stop = 0;
+};
+
 uint32_t up4(uint32_t x) { return (x + 3u) & ~3u; }
 uint32_t up16(uint32_t x) { return (x + 15u) & ~15u; }
+
+uint64_t seedFor(uint64_t seed, uint32_t instance) {
+  uint64_t z = seed + instance + 0x9E3779B97F4A7C15ull;  // splitmix64(seed + instance)
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return z ? z : 0x9E3779B97F4A7C15ull;
+}
 
 }  // namespace
 
@@ -80,6 +106,10 @@ struct tgs_ctx {
   StimMap stim;
   uint64_t stimSeed = 0;
   std::vector<std::unique_ptr<Instance>> inst;
+  bool batched = false;  // instance state lives in `bs` (batched kernel)
+  BatchState bs;
+  uint32_t batchBlock = 256, batchGrid = 0, batchLds = 0, batchGridOverride = 0;
+  bool batchReady = false;
 
   DBuf lut, inIds;
   bool plainReady = false, w64Ready = false;
@@ -148,13 +178,7 @@ int resetInstance(tgs_ctx *c, Instance &in, uint32_t index) {
   uint64_t never = UINT64_MAX;
   std::memcpy(&ctl[C_STOPCYC], &never, 8);
   HIPCHK(c, hipMemcpyAsync(in.ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, c->stream));
-  // splitmix64(seed + instance), never zero
-  uint64_t z = c->stimSeed + index + 0x9E3779B97F4A7C15ull;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  z ^= z >> 31;
-  if (z == 0) z = 0x9E3779B97F4A7C15ull;
-  std::vector<uint64_t> xs(1, z);
+  std::vector<uint64_t> xs(1, seedFor(c->stimSeed, index));
   HIPCHK(c, hipMemcpyAsync(in.xs.p, xs.data(), 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   in.cycles = 0;
@@ -317,6 +341,332 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles) {
   return TGS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Instance state in either layout, through host copies (layout conversion,
+// instance count changes).
+
+int initialHost(tgs_ctx *c, uint32_t n, std::vector<HostInst> &h) {
+  const auto &d = c->design;
+  h.assign(n, HostInst());
+  for (uint32_t i = 0; i < n; ++i) {
+    h[i].reg.assign(d.regPoolSize, 0);
+    h[i].mem.assign(d.memPoolSize, 0);
+    h[i].exg.assign(d.exchangePoolSize, 0);
+    std::copy(c->initReg.begin(), c->initReg.end(), h[i].reg.begin());
+    std::copy(c->initMem.begin(), c->initMem.end(), h[i].mem.begin());
+    h[i].xs = seedFor(c->stimSeed, i);
+  }
+  return TGS_OK;
+}
+
+int instUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
+  c->inst.clear();
+  for (uint32_t i = 0; i < h.size(); ++i) {
+    c->inst.emplace_back(new Instance());
+    Instance &in = *c->inst.back();
+    int rc = resetInstance(c, in, i);
+    if (rc) return rc;
+    const auto &x = h[i];
+    if (!x.reg.empty()) HIPCHK(c, hipMemcpy(in.reg.p, x.reg.data(), x.reg.size(), hipMemcpyHostToDevice));
+    if (!x.mem.empty()) HIPCHK(c, hipMemcpy(in.mem.p, x.mem.data(), x.mem.size(), hipMemcpyHostToDevice));
+    if (!x.exg.empty()) HIPCHK(c, hipMemcpy(in.exg.p, x.exg.data(), x.exg.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(in.xs.p, &x.xs, 8, hipMemcpyHostToDevice));
+    uint32_t ctl[C_WORDS] = {0};
+    ctl[C_STOP] = x.stop;
+    std::memcpy(&ctl[C_STOPCYC], &x.stopCycle, 8);
+    HIPCHK(c, hipMemcpy(in.ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice));
+    in.cycles = x.cycles;
+  }
+  return TGS_OK;
+}
+
+int instDownload(tgs_ctx *c, std::vector<HostInst> &h) {
+  const auto &d = c->design;
+  h.assign(c->inst.size(), HostInst());
+  for (uint32_t i = 0; i < c->inst.size(); ++i) {
+    Instance &in = *c->inst[i];
+    auto &x = h[i];
+    x.reg.resize(d.regPoolSize);
+    x.mem.resize(d.memPoolSize);
+    x.exg.resize(d.exchangePoolSize);
+    if (!x.reg.empty()) HIPCHK(c, hipMemcpy(x.reg.data(), in.reg.p, x.reg.size(), hipMemcpyDeviceToHost));
+    if (!x.mem.empty()) HIPCHK(c, hipMemcpy(x.mem.data(), in.mem.p, x.mem.size(), hipMemcpyDeviceToHost));
+    if (!x.exg.empty()) HIPCHK(c, hipMemcpy(x.exg.data(), in.exg.p, x.exg.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&x.xs, in.xs.p, 8, hipMemcpyDeviceToHost));
+    uint32_t ctl[C_WORDS];
+    HIPCHK(c, hipMemcpy(ctl, in.ctl.p, sizeof ctl, hipMemcpyDeviceToHost));
+    x.stop = ctl[C_STOP];
+    std::memcpy(&x.stopCycle, &ctl[C_STOPCYC], 8);
+    x.cycles = in.cycles;
+  }
+  return TGS_OK;
+}
+
+uint64_t regRows(const SimDesignInfo &d) { return up4(d.regPoolSize) + 4; }
+uint64_t memRows(const SimDesignInfo &d) { return uint64_t(d.memPoolSize) + 1; }
+uint64_t exgRows(const SimDesignInfo &d) { return up4(d.exchangePoolSize) + 4; }
+
+int batchUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
+  auto &b = c->bs;
+  const auto &d = c->design;
+  const uint32_t n = h.size();
+  b.n = n;
+  b.instPad = (n + 63) / 64 * 64;
+  const size_t P = b.instPad;
+  HIPCHK(c, b.reg.alloc(regRows(d) * P));
+  HIPCHK(c, b.mem.alloc(memRows(d) * P));
+  HIPCHK(c, b.exg.alloc(exgRows(d) * P));
+  HIPCHK(c, b.xs.alloc(8 * P));
+  HIPCHK(c, b.draws.alloc(8 * P * 2 * kMaxDraws));
+  HIPCHK(c, b.stop.alloc(8 * P));
+  HIPCHK(c, b.stopCycle.alloc(8 * P));
+  HIPCHK(c, b.logCount.alloc(4));
+  HIPCHK(c, b.logCycle.alloc(kLogCap * 8));
+  HIPCHK(c, b.logMsg.alloc(kLogCap * 8));
+  HIPCHK(c, b.barrier.alloc(4));
+  HIPCHK(c, b.runCycles.alloc(4 * P));
+  HIPCHK(c, b.cycleBase.alloc(8 * P));
+  HIPCHK(c, b.instOfD.alloc(4 * P));
+  HIPCHK(c, b.perm.alloc(4 * P));
+  HIPCHK(c, b.scratch.alloc((std::max({regRows(d), memRows(d), exgRows(d)}) + 8) * P));
+  b.slotOf.resize(n);
+  b.instOf.assign(P, 0xFFFFFFFFu);
+  b.cycles.assign(n, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    b.slotOf[i] = i;
+    b.instOf[i] = i;
+    b.cycles[i] = h[i].cycles;
+  }
+  auto put = [&](DBuf &buf, uint64_t rows, auto field) -> int {
+    std::vector<uint8_t> a(rows * P, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+      const auto &v = field(h[i]);
+      for (size_t r = 0; r < v.size(); ++r) a[r * P + i] = v[r];
+    }
+    HIPCHK(c, hipMemcpy(buf.p, a.data(), a.size(), hipMemcpyHostToDevice));
+    return TGS_OK;
+  };
+  int rc;
+  if ((rc = put(b.reg, regRows(d), [](const HostInst &x) -> const std::vector<uint8_t> & { return x.reg; }))) return rc;
+  if ((rc = put(b.mem, memRows(d), [](const HostInst &x) -> const std::vector<uint8_t> & { return x.mem; }))) return rc;
+  if ((rc = put(b.exg, exgRows(d), [](const HostInst &x) -> const std::vector<uint8_t> & { return x.exg; }))) return rc;
+  std::vector<uint64_t> xs(P, 0), st(P, 0), sc(P, UINT64_MAX);
+  for (uint32_t i = 0; i < n; ++i) {
+    xs[i] = h[i].xs;
+    st[i] = h[i].stop;
+    sc[i] = h[i].stopCycle;
+  }
+  HIPCHK(c, hipMemcpy(b.xs.p, xs.data(), 8 * P, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(b.stop.p, st.data(), 8 * P, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(b.stopCycle.p, sc.data(), 8 * P, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemset(b.logCount.p, 0, 4));
+  HIPCHK(c, hipMemcpy(b.instOfD.p, b.instOf.data(), 4 * P, hipMemcpyHostToDevice));
+  return TGS_OK;
+}
+
+int batchDownload(tgs_ctx *c, std::vector<HostInst> &h) {
+  auto &b = c->bs;
+  const auto &d = c->design;
+  const size_t P = b.instPad;
+  h.assign(b.n, HostInst());
+  auto get = [&](DBuf &buf, uint64_t rows, uint32_t keep, auto field) -> int {
+    std::vector<uint8_t> a(rows * P);
+    HIPCHK(c, hipMemcpy(a.data(), buf.p, a.size(), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < b.n; ++i) {
+      auto &v = field(h[i]);
+      v.resize(keep);
+      for (uint32_t r = 0; r < keep; ++r) v[r] = a[r * P + b.slotOf[i]];
+    }
+    return TGS_OK;
+  };
+  int rc;
+  if ((rc = get(b.reg, regRows(d), d.regPoolSize, [](HostInst &x) -> std::vector<uint8_t> & { return x.reg; }))) return rc;
+  if ((rc = get(b.mem, memRows(d), d.memPoolSize, [](HostInst &x) -> std::vector<uint8_t> & { return x.mem; }))) return rc;
+  if ((rc = get(b.exg, exgRows(d), d.exchangePoolSize, [](HostInst &x) -> std::vector<uint8_t> & { return x.exg; }))) return rc;
+  std::vector<uint64_t> xs(P), st(P), sc(P);
+  HIPCHK(c, hipMemcpy(xs.data(), b.xs.p, 8 * P, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(st.data(), b.stop.p, 8 * P, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(sc.data(), b.stopCycle.p, 8 * P, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < b.n; ++i) {
+    h[i].xs = xs[b.slotOf[i]];
+    h[i].stop = uint32_t(st[b.slotOf[i]]);
+    h[i].stopCycle = sc[b.slotOf[i]];
+    h[i].cycles = b.cycles[i];
+  }
+  return TGS_OK;
+}
+
+int downloadAll(tgs_ctx *c, std::vector<HostInst> &h) { return c->batched ? batchDownload(c, h) : instDownload(c, h); }
+int uploadAll(tgs_ctx *c, const std::vector<HostInst> &h) { return c->batched ? batchUpload(c, h) : instUpload(c, h); }
+uint32_t instanceCount(const tgs_ctx *c) { return c->batched ? c->bs.n : uint32_t(c->inst.size()); }
+
+int ensureBatchImage(tgs_ctx *c) {
+  if (c->batchReady) return TGS_OK;
+  const uint32_t nWaves = c->batchBlock / 64;
+  c->w64 = buildWave64Image(c->design, c->stim, nWaves);
+  c->w64Ready = false;  // the wave64 schedule must be rebuilt for this image
+  auto &im = c->w64;
+  int rc;
+  if ((rc = upload(c, c->wWords, im.words.data(), im.words.size() * 4))) return rc;
+  if ((rc = upload(c, c->wPartOff, im.partOff.data(), im.partOff.size() * 4))) return rc;
+  if ((rc = upload(c, c->wConsts, im.consts.data(), im.consts.size()))) return rc;
+  const uint32_t vpBytes = up16(std::max<uint32_t>(im.maxVp, 16));
+  c->batchLds = wave64LutBytes() + vpBytes * 64 + nWaves * 4096;
+  if (c->batchLds > c->prop.sharedMemPerBlock)
+    return c->fail(TGS_E_LIMIT,
+                   "batched execution needs %u B of LDS (value pool %u slots x 64 instances); device allows %zu B",
+                   c->batchLds, im.maxVp, size_t(c->prop.sharedMemPerBlock));
+  HIPCHK(c, batchSetLds(c->batchLds));
+  int perCU = 0;
+  HIPCHK(c, batchMaxBlocksPerCU(&perCU, c->batchBlock, c->batchLds));
+  if (perCU < 1) return c->fail(TGS_E_LIMIT, "batched kernel cannot be resident with %u B LDS", c->batchLds);
+  perCU = std::min(perCU, 4);
+  c->batchGrid = uint32_t(perCU) * c->prop.multiProcessorCount;
+  if (c->batchGridOverride) c->batchGrid = std::min(c->batchGrid, c->batchGridOverride);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->batchReady = true;
+  return TGS_OK;
+}
+
+// Reorder slots so that slot order is decreasing remaining cycle count.
+int batchOrder(tgs_ctx *c, const uint32_t *cycles) {
+  auto &b = c->bs;
+  const auto &d = c->design;
+  const uint32_t n = b.n, P = b.instPad;
+  std::vector<uint32_t> order(n);
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return cycles[x] > cycles[y]; });
+  std::vector<uint32_t> newSlot(n);
+  bool same = true;
+  for (uint32_t k = 0; k < n; ++k) {
+    newSlot[order[k]] = k;
+    same = same && b.slotOf[order[k]] == k;
+  }
+  if (same) return TGS_OK;
+  std::vector<uint32_t> src(P);
+  for (uint32_t k = n; k < P; ++k) src[k] = k;
+  for (uint32_t i = 0; i < n; ++i) src[newSlot[i]] = b.slotOf[i];
+  HIPCHK(c, hipMemcpyAsync(b.perm.p, src.data(), 4 * P, hipMemcpyHostToDevice, c->stream));
+  const uint32_t *sp = b.perm.as<uint32_t>();
+  auto perm8 = [&](DBuf &buf, uint64_t rows) -> int {
+    HIPCHK(c, launchPermute(buf.as<uint8_t>(), b.scratch.as<uint8_t>(), rows, P, sp, c->stream));
+    HIPCHK(c, hipMemcpyAsync(buf.p, b.scratch.p, rows * P, hipMemcpyDeviceToDevice, c->stream));
+    return TGS_OK;
+  };
+  auto perm64 = [&](DBuf &buf) -> int {
+    HIPCHK(c, launchPermute64(buf.as<uint64_t>(), b.scratch.as<uint64_t>(), 1, P, sp, c->stream));
+    HIPCHK(c, hipMemcpyAsync(buf.p, b.scratch.p, 8 * P, hipMemcpyDeviceToDevice, c->stream));
+    return TGS_OK;
+  };
+  int rc;
+  if ((rc = perm8(b.reg, regRows(d))) || (rc = perm8(b.mem, memRows(d))) || (rc = perm8(b.exg, exgRows(d))) ||
+      (rc = perm64(b.xs)) || (rc = perm64(b.stop)) || (rc = perm64(b.stopCycle)))
+    return rc;
+  for (uint32_t i = 0; i < n; ++i) b.slotOf[i] = newSlot[i];
+  std::fill(b.instOf.begin(), b.instOf.end(), 0xFFFFFFFFu);
+  for (uint32_t i = 0; i < n; ++i) b.instOf[newSlot[i]] = i;
+  HIPCHK(c, hipMemcpyAsync(b.instOfD.p, b.instOf.data(), 4 * P, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TGS_OK;
+}
+
+int runBatched(tgs_ctx *c, const uint32_t *cycles) {
+  int rc;
+  if ((rc = ensureBatchImage(c))) return rc;
+  auto &b = c->bs;
+  const uint32_t P = b.instPad;
+  if ((rc = batchOrder(c, cycles))) return rc;
+  std::vector<uint64_t> remaining(P, 0), base(P, 0);
+  uint64_t maxC = 0;
+  for (uint32_t s = 0; s < P; ++s)
+    if (b.instOf[s] != 0xFFFFFFFFu) {
+      remaining[s] = cycles[b.instOf[s]];
+      base[s] = b.cycles[b.instOf[s]];
+      maxC = std::max(maxC, remaining[s]);
+    }
+  const uint64_t perLaunch = std::min<uint64_t>(1u << 20, 0xE0000000ull / (2ull * c->batchGrid + 2));
+  std::vector<uint32_t> run(P), active;
+  while (maxC > 0) {
+    const uint32_t L = uint32_t(std::min<uint64_t>(maxC, perLaunch));
+    for (uint32_t s = 0; s < P; ++s) run[s] = uint32_t(std::min<uint64_t>(remaining[s], L));
+    active.assign(L, 0);
+    for (uint32_t cy = 0; cy < L; ++cy) {
+      uint32_t g = 0;
+      while (g < P / 64 && run[g * 64] > cy) ++g;  // slots sorted by decreasing cycles
+      active[cy] = g;
+    }
+    HIPCHK(c, b.groupsActive.alloc(4ull * L));
+    HIPCHK(c, hipMemcpyAsync(b.runCycles.p, run.data(), 4ull * P, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.cycleBase.p, base.data(), 8ull * P, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.groupsActive.p, active.data(), 4ull * L, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemsetAsync(b.barrier.p, 0, 4, c->stream));
+    BatchArgs a{};
+    a.lut = c->lut.as<uint8_t>();
+    a.consts = c->wConsts.as<uint8_t>();
+    a.reg = b.reg.as<uint8_t>();
+    a.mem = b.mem.as<uint8_t>();
+    a.exg = b.exg.as<uint8_t>();
+    a.regSize = c->design.regPoolSize;
+    a.memSize = c->design.memPoolSize;
+    a.exgSize = c->design.exchangePoolSize;
+    a.instPad = P;
+    a.words = c->wWords.as<uint32_t>();
+    a.partOff = c->wPartOff.as<uint32_t>();
+    a.n0 = c->w64.n0;
+    a.nTotal = c->w64.nTotal;
+    a.vpBytes = up16(std::max<uint32_t>(c->w64.maxVp, 16));
+    a.runCycles = b.runCycles.as<uint32_t>();
+    a.cycleBase = b.cycleBase.as<uint64_t>();
+    a.groupsActive = b.groupsActive.as<uint32_t>();
+    a.nCycles = L;
+    a.xs = b.xs.as<uint64_t>();
+    a.draws = b.draws.as<uint64_t>();
+    a.inIds = c->inIds.as<uint32_t>();
+    a.nIn = c->stim.ids.size();
+    a.stop = b.stop.as<uint64_t>();
+    a.stopCycle = b.stopCycle.as<uint64_t>();
+    a.logCount = b.logCount.as<uint32_t>();
+    a.logCycle = b.logCycle.as<uint64_t>();
+    a.logMsg = b.logMsg.as<uint64_t>();
+    a.instOf = b.instOfD.as<uint32_t>();
+    a.logCap = kLogCap;
+    a.printEnable = c->printEnable;
+    a.barrier = b.barrier.as<uint32_t>();
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, launchBatch(a, c->batchGrid, c->batchBlock, c->batchLds, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->prof.kernel_ms += ms;
+    c->prof.launches += 1;
+    c->prof.kernel = TGS_KERNEL_BATCH;
+    c->prof.grid = c->batchGrid;
+    c->prof.block = c->batchBlock;
+    c->prof.lds_bytes = c->batchLds;
+    c->prof.prog_bytes = c->w64.words.size() * 4;
+    maxC = 0;
+    for (uint32_t s = 0; s < P; ++s) {
+      c->prof.cycles += run[s];
+      remaining[s] -= run[s];
+      base[s] += run[s];
+      maxC = std::max(maxC, remaining[s]);
+    }
+  }
+  for (uint32_t i = 0; i < b.n; ++i) b.cycles[i] += cycles[i];
+  return TGS_OK;
+}
+
+int switchLayout(tgs_ctx *c, bool batched) {
+  if (c->batched == batched) return TGS_OK;
+  std::vector<HostInst> h;
+  int rc = downloadAll(c, h);
+  if (rc) return rc;
+  c->batched = batched;
+  if (batched) c->inst.clear();
+  return uploadAll(c, h);
+}
+
 int checkLoaded(tgs_ctx *c) {
   if (!c) return TGS_E_ARG;
   if (!c->loaded) return c->fail(TGS_E_STATE, "no design loaded");
@@ -448,6 +798,9 @@ int tgs_load(tgs_ctx *c, const uint8_t *design, size_t designSize, const uint8_t
   std::copy(c->design.lut.begin(), c->design.lut.end(), lut.begin());
   if ((rc = upload(c, c->lut, lut.data(), lut.size()))) return rc;
   if ((rc = upload(c, c->inIds, nullptr, 0))) return rc;
+  c->batched = false;
+  c->batchReady = false;
+  c->kernel = c->kernel == TGS_KERNEL_BATCH ? TGS_KERNEL_WAVE64 : c->kernel;
   c->inst.emplace_back(new Instance());
   if ((rc = resetInstance(c, *c->inst[0], 0))) return rc;
   c->loaded = true;
@@ -471,11 +824,38 @@ int tgs_load_files(tgs_ctx *c, const char *designPath, const char *symbolPath, i
 
 int tgs_set_kernel(tgs_ctx *c, int kernel) {
   if (!c) return TGS_E_ARG;
-  if (kernel != TGS_KERNEL_PLAIN && kernel != TGS_KERNEL_WAVE64) {
-    if (kernel == TGS_KERNEL_BATCH) return c->fail(TGS_E_UNSUPPORTED, "batched kernel not built into this library");
+  if (kernel != TGS_KERNEL_PLAIN && kernel != TGS_KERNEL_WAVE64 && kernel != TGS_KERNEL_BATCH)
     return c->fail(TGS_E_ARG, "unknown kernel %d", kernel);
+  if (c->loaded) {
+    int rc = setDevice(c);
+    if (rc) return rc;
+    if ((rc = switchLayout(c, kernel == TGS_KERNEL_BATCH))) return rc;
   }
   c->kernel = kernel;
+  return TGS_OK;
+}
+
+int tgs_set_launch(tgs_ctx *c, int kernel, uint32_t block, uint32_t groups) {
+  if (!c) return TGS_E_ARG;
+  if (block % 64 || block > 1024) return c->fail(TGS_E_ARG, "block must be a multiple of 64 up to 1024");
+  switch (kernel) {
+    case TGS_KERNEL_PLAIN:
+      if (block) c->plainBlock = block;
+      c->plainReady = false;
+      break;
+    case TGS_KERNEL_WAVE64:
+      if (block) c->w64Block = block;
+      c->w64Groups = groups;
+      c->w64Ready = false;
+      break;
+    case TGS_KERNEL_BATCH:
+      if (block) c->batchBlock = block;
+      c->batchGridOverride = groups;
+      c->batchReady = false;
+      break;
+    default:
+      return c->fail(TGS_E_ARG, "unknown kernel %d", kernel);
+  }
   return TGS_OK;
 }
 
@@ -489,12 +869,9 @@ int tgs_set_instances(tgs_ctx *c, uint32_t n) {
   int rc = checkLoaded(c);
   if (rc) return rc;
   if (n == 0) return c->fail(TGS_E_ARG, "at least one instance");
-  c->inst.clear();
-  for (uint32_t i = 0; i < n; ++i) {
-    c->inst.emplace_back(new Instance());
-    if ((rc = resetInstance(c, *c->inst.back(), i))) return rc;
-  }
-  return TGS_OK;
+  std::vector<HostInst> h;
+  if ((rc = initialHost(c, n, h))) return rc;
+  return uploadAll(c, h);
 }
 
 int tgs_set_stimulus(tgs_ctx *c, const uint32_t *ids, uint32_t n, uint64_t seed) {
@@ -507,75 +884,57 @@ int tgs_set_stimulus(tgs_ctx *c, const uint32_t *ids, uint32_t n, uint64_t seed)
   if ((rc = buildStimMap(c->design, ids, n, &m, &e))) return c->fail(rc, "%s", e.c_str());
   c->stim = std::move(m);
   c->stimSeed = seed;
-  c->plainReady = c->w64Ready = false;
+  c->plainReady = c->w64Ready = c->batchReady = false;
   if ((rc = upload(c, c->inIds, c->stim.ids.data(), c->stim.ids.size() * 4))) return rc;
   programStats(c->design, n, &c->stats);
   // every instance restarts its generator from splitmix64(seed + k)
-  for (uint32_t i = 0; i < c->inst.size(); ++i) {
-    uint64_t z = seed + i + 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    z ^= z >> 31;
-    if (z == 0) z = 0x9E3779B97F4A7C15ull;
-    HIPCHK(c, hipMemcpy(c->inst[i]->xs.p, &z, 8, hipMemcpyHostToDevice));
+  if (c->batched) {
+    std::vector<uint64_t> xs(c->bs.instPad, 0);
+    for (uint32_t i = 0; i < c->bs.n; ++i) xs[c->bs.slotOf[i]] = seedFor(seed, i);
+    HIPCHK(c, hipMemcpy(c->bs.xs.p, xs.data(), 8 * xs.size(), hipMemcpyHostToDevice));
+  } else {
+    for (uint32_t i = 0; i < c->inst.size(); ++i) {
+      uint64_t z = seedFor(seed, i);
+      HIPCHK(c, hipMemcpy(c->inst[i]->xs.p, &z, 8, hipMemcpyHostToDevice));
+    }
   }
   return TGS_OK;
 }
 
-int tgs_eval_free_running(tgs_ctx *c, uint32_t cycles, int *stopped) {
-  int rc = checkLoaded(c);
-  if (rc) return rc;
-  for (auto &in : c->inst)
-    if ((rc = runInstance(c, *in, cycles))) return rc;
-  if (stopped) return readStop(c, *c->inst[0], stopped);
-  return TGS_OK;
-}
-
-int tgs_eval(tgs_ctx *c, int *stopped) { return tgs_eval_free_running(c, 1, stopped); }
-
 int tgs_run_batch(tgs_ctx *c, const uint32_t *cycles, uint32_t n, int *anyStopped) {
   int rc = checkLoaded(c);
   if (rc) return rc;
-  if (!cycles || n != c->inst.size()) return c->fail(TGS_E_ARG, "need one cycle count per instance");
+  if (!cycles || n != instanceCount(c)) return c->fail(TGS_E_ARG, "need one cycle count per instance");
   int any = 0;
-  for (uint32_t i = 0; i < n; ++i) {
-    if ((rc = runInstance(c, *c->inst[i], cycles[i]))) return rc;
-    int s = 0;
-    if ((rc = readStop(c, *c->inst[i], &s))) return rc;
-    any |= s;
+  if (c->batched) {
+    if ((rc = runBatched(c, cycles))) return rc;
+    std::vector<uint64_t> st(c->bs.instPad);
+    HIPCHK(c, hipMemcpy(st.data(), c->bs.stop.p, 8 * st.size(), hipMemcpyDeviceToHost));
+    for (auto v : st) any |= v != 0;
+  } else {
+    for (uint32_t i = 0; i < n; ++i) {
+      if ((rc = runInstance(c, *c->inst[i], cycles[i]))) return rc;
+      int s = 0;
+      if ((rc = readStop(c, *c->inst[i], &s))) return rc;
+      any |= s;
+    }
   }
   if (anyStopped) *anyStopped = any;
   return TGS_OK;
 }
 
-int tgs_read_regs(tgs_ctx *c, uint32_t instance, const uint32_t *ids, uint32_t n, uint8_t *out) {
+int tgs_get_stop(tgs_ctx *c, uint32_t instance, int *stopped);
+
+int tgs_eval_free_running(tgs_ctx *c, uint32_t cycles, int *stopped) {
   int rc = checkLoaded(c);
   if (rc) return rc;
-  Instance *in = instanceOf(c, instance);
-  if (!in) return TGS_E_ARG;
-  if (n && (!ids || !out)) return TGS_E_ARG;
-  for (uint32_t i = 0; i < n; ++i)
-    if (ids[i] >= c->design.regPoolSize) return c->fail(TGS_E_ARG, "register id %u out of range", ids[i]);
-  std::vector<uint8_t> pool(c->design.regPoolSize);
-  if (!pool.empty()) HIPCHK(c, hipMemcpy(pool.data(), in->reg.p, pool.size(), hipMemcpyDeviceToHost));
-  for (uint32_t i = 0; i < n; ++i) out[i] = pool[ids[i]];
+  std::vector<uint32_t> all(instanceCount(c), cycles);
+  if ((rc = tgs_run_batch(c, all.data(), uint32_t(all.size()), nullptr))) return rc;
+  if (stopped) return tgs_get_stop(c, 0, stopped);
   return TGS_OK;
 }
 
-int tgs_write_regs(tgs_ctx *c, uint32_t instance, const uint32_t *ids, uint32_t n, const uint8_t *vals) {
-  int rc = checkLoaded(c);
-  if (rc) return rc;
-  Instance *in = instanceOf(c, instance);
-  if (!in) return TGS_E_ARG;
-  if (n && (!ids || !vals)) return TGS_E_ARG;
-  for (uint32_t i = 0; i < n; ++i)
-    if (ids[i] >= c->design.regPoolSize) return c->fail(TGS_E_ARG, "register id %u out of range", ids[i]);
-  std::vector<uint8_t> pool(c->design.regPoolSize);
-  if (!pool.empty()) HIPCHK(c, hipMemcpy(pool.data(), in->reg.p, pool.size(), hipMemcpyDeviceToHost));
-  for (uint32_t i = 0; i < n; ++i) pool[ids[i]] = vals[i];
-  if (!pool.empty()) HIPCHK(c, hipMemcpy(in->reg.p, pool.data(), pool.size(), hipMemcpyHostToDevice));
-  return TGS_OK;
-}
+int tgs_eval(tgs_ctx *c, int *stopped) { return tgs_eval_free_running(c, 1, stopped); }
 
 int tgs_pool_size(const tgs_ctx *c, int pool, uint64_t *size) {
   if (!c || !size || !c->loaded) return TGS_E_ARG;
@@ -587,60 +946,94 @@ int tgs_pool_size(const tgs_ctx *c, int pool, uint64_t *size) {
   }
 }
 
-static int poolPtr(tgs_ctx *c, Instance *in, int pool, uint64_t off, uint64_t n, uint8_t **p) {
+// Copy [off, off + n) of one instance's pool to or from host memory, in either layout.
+static int poolIO(tgs_ctx *c, uint32_t instance, int pool, uint64_t off, uint64_t n, uint8_t *host, bool write) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if (instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "instance %u out of range", instance);
   uint64_t size = 0;
   if (tgs_pool_size(c, pool, &size)) return c->fail(TGS_E_ARG, "unknown pool %d", pool);
   if (off > size || n > size - off) return c->fail(TGS_E_ARG, "pool range out of bounds");
-  DBuf &b = pool == TGS_POOL_REG ? in->reg : pool == TGS_POOL_MEM ? in->mem : in->exg;
-  *p = b.as<uint8_t>() + off;
+  if (n == 0) return TGS_OK;
+  if (!host) return TGS_E_ARG;
+  if (!c->batched) {
+    Instance *in = c->inst[instance].get();
+    DBuf &b = pool == TGS_POOL_REG ? in->reg : pool == TGS_POOL_MEM ? in->mem : in->exg;
+    uint8_t *p = b.as<uint8_t>() + off;
+    HIPCHK(c, write ? hipMemcpy(p, host, n, hipMemcpyHostToDevice) : hipMemcpy(host, p, n, hipMemcpyDeviceToHost));
+    return TGS_OK;
+  }
+  auto &bs = c->bs;
+  DBuf &b = pool == TGS_POOL_REG ? bs.reg : pool == TGS_POOL_MEM ? bs.mem : bs.exg;
+  const size_t P = bs.instPad;
+  uint8_t *p = b.as<uint8_t>() + off * P + bs.slotOf[instance];
+  if (write)
+    HIPCHK(c, hipMemcpy2D(p, P, host, 1, 1, n, hipMemcpyHostToDevice));
+  else
+    HIPCHK(c, hipMemcpy2D(host, 1, p, P, 1, n, hipMemcpyDeviceToHost));
   return TGS_OK;
 }
 
 int tgs_read_pool(tgs_ctx *c, uint32_t instance, int pool, uint64_t off, uint64_t n, uint8_t *out) {
-  int rc = checkLoaded(c);
-  if (rc) return rc;
-  Instance *in = instanceOf(c, instance);
-  if (!in || (n && !out)) return TGS_E_ARG;
-  uint8_t *p;
-  if ((rc = poolPtr(c, in, pool, off, n, &p))) return rc;
-  if (n) HIPCHK(c, hipMemcpy(out, p, n, hipMemcpyDeviceToHost));
-  return TGS_OK;
+  return poolIO(c, instance, pool, off, n, out, false);
 }
 
 int tgs_write_pool(tgs_ctx *c, uint32_t instance, int pool, uint64_t off, uint64_t n, const uint8_t *src) {
+  return poolIO(c, instance, pool, off, n, const_cast<uint8_t *>(src), true);
+}
+
+int tgs_read_regs(tgs_ctx *c, uint32_t instance, const uint32_t *ids, uint32_t n, uint8_t *out) {
   int rc = checkLoaded(c);
   if (rc) return rc;
-  Instance *in = instanceOf(c, instance);
-  if (!in || (n && !src)) return TGS_E_ARG;
-  uint8_t *p;
-  if ((rc = poolPtr(c, in, pool, off, n, &p))) return rc;
-  if (n) HIPCHK(c, hipMemcpy(p, src, n, hipMemcpyHostToDevice));
+  if (n && (!ids || !out)) return TGS_E_ARG;
+  for (uint32_t i = 0; i < n; ++i)
+    if (ids[i] >= c->design.regPoolSize) return c->fail(TGS_E_ARG, "register id %u out of range", ids[i]);
+  std::vector<uint8_t> pool(c->design.regPoolSize);
+  if ((rc = poolIO(c, instance, TGS_POOL_REG, 0, pool.size(), pool.data(), false))) return rc;
+  for (uint32_t i = 0; i < n; ++i) out[i] = pool[ids[i]];
   return TGS_OK;
+}
+
+int tgs_write_regs(tgs_ctx *c, uint32_t instance, const uint32_t *ids, uint32_t n, const uint8_t *vals) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if (n && (!ids || !vals)) return TGS_E_ARG;
+  for (uint32_t i = 0; i < n; ++i)
+    if (ids[i] >= c->design.regPoolSize) return c->fail(TGS_E_ARG, "register id %u out of range", ids[i]);
+  std::vector<uint8_t> pool(c->design.regPoolSize);
+  if ((rc = poolIO(c, instance, TGS_POOL_REG, 0, pool.size(), pool.data(), false))) return rc;
+  for (uint32_t i = 0; i < n; ++i) pool[ids[i]] = vals[i];
+  return poolIO(c, instance, TGS_POOL_REG, 0, pool.size(), pool.data(), true);
 }
 
 int tgs_get_stop(tgs_ctx *c, uint32_t instance, int *stopped) {
   int rc = checkLoaded(c);
   if (rc) return rc;
-  Instance *in = instanceOf(c, instance);
-  if (!in || !stopped) return TGS_E_ARG;
-  return readStop(c, *in, stopped);
+  if (!stopped || instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "bad instance or pointer");
+  if (!c->batched) return readStop(c, *c->inst[instance], stopped);
+  uint64_t v = 0;
+  HIPCHK(c, hipMemcpy(&v, c->bs.stop.as<uint64_t>() + c->bs.slotOf[instance], 8, hipMemcpyDeviceToHost));
+  *stopped = v != 0;
+  return TGS_OK;
 }
 
 int tgs_get_cycles(tgs_ctx *c, uint32_t instance, uint64_t *cycles) {
   int rc = checkLoaded(c);
   if (rc) return rc;
-  Instance *in = instanceOf(c, instance);
-  if (!in || !cycles) return TGS_E_ARG;
-  *cycles = in->cycles;
+  if (!cycles || instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "bad instance or pointer");
+  *cycles = c->batched ? c->bs.cycles[instance] : c->inst[instance]->cycles;
   return TGS_OK;
 }
 
 int tgs_get_stop_cycle(tgs_ctx *c, uint32_t instance, uint64_t *cycle) {
   int rc = checkLoaded(c);
   if (rc) return rc;
-  Instance *in = instanceOf(c, instance);
-  if (!in || !cycle) return TGS_E_ARG;
-  HIPCHK(c, hipMemcpy(cycle, in->ctl.as<uint32_t>() + C_STOPCYC, 8, hipMemcpyDeviceToHost));
+  if (!cycle || instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "bad instance or pointer");
+  if (c->batched) {
+    HIPCHK(c, hipMemcpy(cycle, c->bs.stopCycle.as<uint64_t>() + c->bs.slotOf[instance], 8, hipMemcpyDeviceToHost));
+  } else {
+    HIPCHK(c, hipMemcpy(cycle, c->inst[instance]->ctl.as<uint32_t>() + C_STOPCYC, 8, hipMemcpyDeviceToHost));
+  }
   return TGS_OK;
 }
 
@@ -648,14 +1041,35 @@ int tgs_get_print_log(tgs_ctx *c, uint32_t instance, uint64_t *cycles, uint32_t 
                       uint64_t *count) {
   int rc = checkLoaded(c);
   if (rc) return rc;
-  Instance *in = instanceOf(c, instance);
-  if (!in || !count) return TGS_E_ARG;
+  if (!count || instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "bad instance or pointer");
+  if (!c->batched) {
+    Instance *in = c->inst[instance].get();
+    uint32_t n = 0;
+    HIPCHK(c, hipMemcpy(&n, in->ctl.as<uint32_t>() + C_LOGCOUNT, 4, hipMemcpyDeviceToHost));
+    *count = std::min<uint64_t>(n, kLogCap);
+    uint64_t k = std::min<uint64_t>(cap, *count);
+    if (k && cycles) HIPCHK(c, hipMemcpy(cycles, in->logCycle.p, k * 8, hipMemcpyDeviceToHost));
+    if (k && msgs) HIPCHK(c, hipMemcpy(msgs, in->logMsg.p, k * 4, hipMemcpyDeviceToHost));
+    return TGS_OK;
+  }
   uint32_t n = 0;
-  HIPCHK(c, hipMemcpy(&n, in->ctl.as<uint32_t>() + C_LOGCOUNT, 4, hipMemcpyDeviceToHost));
-  *count = n;
-  uint64_t k = std::min<uint64_t>({cap, n, kLogCap});
-  if (k && cycles) HIPCHK(c, hipMemcpy(cycles, in->logCycle.p, k * 8, hipMemcpyDeviceToHost));
-  if (k && msgs) HIPCHK(c, hipMemcpy(msgs, in->logMsg.p, k * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&n, c->bs.logCount.p, 4, hipMemcpyDeviceToHost));
+  const uint64_t total = std::min<uint64_t>(n, kLogCap);
+  std::vector<uint64_t> cy(total), mg(total);
+  if (total) {
+    HIPCHK(c, hipMemcpy(cy.data(), c->bs.logCycle.p, total * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(mg.data(), c->bs.logMsg.p, total * 8, hipMemcpyDeviceToHost));
+  }
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < total; ++i) {
+    if ((mg[i] >> 32) != instance) continue;
+    if (k < cap) {
+      if (cycles) cycles[k] = cy[i];
+      if (msgs) msgs[k] = uint32_t(mg[i]);
+    }
+    ++k;
+  }
+  *count = k;
   return TGS_OK;
 }
 

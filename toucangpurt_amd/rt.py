@@ -162,6 +162,10 @@ class Simulator:
     def set_kernel(self, kernel):
         self._check(self._lib.tgs_set_kernel(self._h, kernel))
 
+    def set_launch(self, kernel, block=0, groups=0):
+        """Threads per workgroup and workgroup count of a kernel family (0 = automatic)."""
+        self._check(self._lib.tgs_set_launch(self._h, kernel, int(block), int(groups)))
+
     def set_print(self, enable):
         self._check(self._lib.tgs_set_print(self._h, 1 if enable else 0))
 
