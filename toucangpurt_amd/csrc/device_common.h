@@ -89,24 +89,26 @@ __device__ __forceinline__ uint64_t memAddress(const uint8_t *vp, uint32_t addrV
   return base + addr;
 }
 
-// Grid-wide barrier over a monotonically increasing arrival counter. Every
-// wave publishes its global writes (agent-scope release) before the
-// workgroup arrives, and re-acquires after the barrier opens, so register,
-// exchange and memory pools written before the barrier are visible to every
-// XCD after it. `target` = (barrier index + 1) * gridDim.x.
+// Grid-wide barrier over a monotonically increasing arrival counter.
+// The workgroup barrier first completes every wave's global stores into this
+// CU's L2; one wave then publishes them device-wide (agent-scope release,
+// an L2 write-back), arrives, spins on relaxed loads, and re-acquires once
+// (agent-scope acquire, an L1/L2 invalidate that every wave of the CU then
+// sees). Register, exchange and memory pools written before the barrier are
+// visible to every XCD after it. `target` = (barrier index + 1) * gridDim.x.
 __device__ __forceinline__ uint64_t gridBarrier(uint32_t *counter, uint32_t target, bool measure) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
   __syncthreads();
   uint64_t waited = 0;
   if (threadIdx.x == 0) {
     const uint64_t t0 = measure ? __builtin_amdgcn_s_memrealtime() : 0;
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
       __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (measure) waited = __builtin_amdgcn_s_memrealtime() - t0;
   }
   __syncthreads();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return waited;
 }
 

@@ -62,6 +62,7 @@ enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3 
 //   lanes 32-63 = B). Words:
 //   w0  kind | nMid << 8
 //   w1  nTopA | nTopB << 8 | nLastA << 16 | nLastB << 24
+//   w2  word offset (from w0) of the last-level ops
 //   ceil(nMid/2) words of middle sizes, (nA | nB << 8) per 16 bits
 //   top ops   (nTopA + nTopB) x 2 words: lut | op0 << 16, op1 | op2 << 16
 //   middle    per level (nA + nB) words: the packed on-disk op

@@ -356,7 +356,10 @@ PlainImage buildPlainImage(const SimDesignInfo &d, const StimMap &stim) {
           for (size_t l = 0; l < m.middleLevels.size(); ++l) sizes[1 + l] = static_cast<uint8_t>(m.middleLevels[l].size());
           sizes[1 + m.middleLevels.size()] = static_cast<uint8_t>(m.lastLevel.size());
           b.insert(b.end(), sizes, sizes + 160);
-          putVec(b, m.topLevel);
+          for (auto o : m.topLevel) {
+            if (o.lutIndex == 0) o.op0 = o.op1 = 0;  // unused operands of a pass-through op
+            put(b, o);
+          }
           for (const auto &l : m.middleLevels) putVec(b, l);
           putVec(b, m.lastLevel);
         } else {
@@ -472,6 +475,7 @@ Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32
         uint32_t ntA = A->topLevel.size(), ntB = B ? B->topLevel.size() : 0;
         uint32_t nlA = A->lastLevel.size(), nlB = B ? B->lastLevel.size() : 0;
         t.push_back(ntA | ntB << 8 | nlA << 16 | nlB << 24);
+        t.push_back(0);  // word offset of the last-level ops, filled below
         for (uint32_t l = 0; l < nMid; l += 2) {
           uint32_t v = 0;
           for (uint32_t k = 0; k < 2 && l + k < nMid; ++k) {
@@ -483,8 +487,10 @@ Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32
         auto tops = [&](const CGMicroPartInfo *m) {
           if (!m) return;
           for (const auto &o : m->topLevel) {
-            t.push_back(uint32_t(o.lutIndex) | uint32_t(o.op0) << 16);
-            t.push_back(uint32_t(o.op1) | uint32_t(o.op2) << 16);
+            // a pass-through op ignores op0/op1: store immediates so no kernel reads an unchecked slot
+            const uint32_t o0 = o.lutIndex ? o.op0 : 0u, o1 = o.lutIndex ? o.op1 : 0u;
+            t.push_back(uint32_t(o.lutIndex) | o0 << 16);
+            t.push_back(o1 | uint32_t(o.op2) << 16);
           }
         };
         tops(A);
@@ -497,6 +503,7 @@ Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32
           if (!m) return;
           for (const auto &o : m->lastLevel) t.push_back(uint32_t(o.result) | uint32_t(o.shuffleId) << 16);
         };
+        t[2] = uint32_t(t.size());
         lasts(A);
         lasts(B);
         st.push_back(std::move(t));

@@ -33,7 +33,7 @@ struct BatchArgs {
   uint8_t *reg, *mem, *exg;
   uint32_t regSize, memSize, exgSize;
   uint32_t instPad;               // slots, a multiple of 64
-  const uint32_t *words, *partOff;  // wave64 image
+  const uint32_t *words, *partOff, *partWords;  // wave64 image
   uint32_t n0, nTotal;
   uint32_t vpBytes;               // value-pool slots per lane (16-multiple)
   const uint32_t *runCycles;      // per slot: cycles to run in this launch
@@ -59,9 +59,13 @@ hipError_t launchPlain(const PlainArgs &args, uint32_t grid, uint32_t block, uin
 hipError_t plainMaxBlocksPerCU(int *blocks, uint32_t block, uint32_t lds);
 hipError_t plainSetLds(uint32_t lds);
 
-hipError_t launchWave64(const W64Args &args, bool resident, uint32_t grid, uint32_t block, uint32_t lds,
+// Where the wave64 kernel reads program words from: global memory, an LDS
+// copy staged per partition visit, or LDS-resident for the whole launch.
+enum ProgMode : int { PM_GLOBAL = 0, PM_STAGED = 1, PM_RESIDENT = 2 };
+
+hipError_t launchWave64(const W64Args &args, int mode, uint32_t grid, uint32_t block, uint32_t lds,
                         hipStream_t stream);
-hipError_t wave64MaxBlocksPerCU(int *blocks, bool resident, uint32_t block, uint32_t lds);
+hipError_t wave64MaxBlocksPerCU(int *blocks, int mode, uint32_t block, uint32_t lds);
 hipError_t wave64SetLds(uint32_t lds);
 uint32_t wave64LutBytes();
 

@@ -2,18 +2,21 @@
 // one persistent launch, one instance per lane.
 //
 // Semantics per instance are the reference evaluator's (src/SimEval.cu:304-761,
-// :765-864, :889-965). The execution is laid out for 64-wide waves: every
-// pool is instance-minor (nibble k of instance i at k * instPad + i), so a
-// wave of 64 instances moves 64 contiguous bytes per op, program words are
-// uniform (scalar loads shared by all 64 instances), and the value pool of a
-// (partition, 64-instance group) work item sits in LDS as [slot][lane].
-// A LUT micropartition's level values live in a per-wave LDS buffer
-// [2][32][64]; a lane index past the previous level's size reads 0, as the
-// reference's idle lanes do (SimEval.cu:372-375,406-408).
-//
+// :765-864, :889-965). The execution is laid out for 64-wide waves:
+//   * every pool is instance-minor (nibble k of slot s at k * instPad + s), so
+//     a 64-instance group's nibble k is 64 contiguous bytes; L0 reads and the
+//     last-level register / exchange copies move them 16 bytes per lane,
+//     16 ops per wave instruction;
+//   * a work item is (partition, up to W groups): the partition's program is
+//     staged into LDS once and each of the W waves runs the whole partition
+//     for its own group with its own value pool [slot][lane] and level buffer
+//     [2][32][64] in LDS, so waves never wait on each other inside a region;
+//   * ops of one micropartition level are issued four at a time so their LDS
+//     latencies overlap; a lane index past the previous level's size reads 0,
+//     as the reference's idle lanes do (SimEval.cu:372-375,406-408).
 // Instances run uneven cycle counts: the host orders instance slots by
-// decreasing cycle count, so the groups still active in a cycle are a prefix
-// and finished groups cost nothing; lanes past their own count never write.
+// decreasing cycle count, so the live groups of a cycle are a prefix and the
+// live lanes of a group are a prefix; lanes past their own count never write.
 // Stimulus draws for cycle c+1 are produced during region 1 of cycle c into
 // the other half of a double buffer, so no extra grid barrier is needed.
 #include <hip/hip_runtime.h>
@@ -27,57 +30,88 @@ namespace {
 constexpr uint32_t kLutBytesB = ((kLutSize + 15) / 16) * 16;
 
 struct Ctx {
-  const BatchArgs &a;
-  uint8_t *vp;       // [slot][64]
-  uint8_t *lvl;      // this wave's [2][32][64]
+  const uint8_t *mem;
+  uint64_t memSize;
+  size_t instPad;
+  uint8_t *vp;    // this wave's value pool [slot][64]
+  uint8_t *lvl;   // this wave's level buffer [2][32][64]
   const uint8_t *lut;
-  uint32_t lane, g, slot0;  // group g, first instance slot of the group
-  uint64_t cycle;
+  uint32_t lane, slot0;
 };
 
 __device__ __forceinline__ uint32_t vpAt(const Ctx &c, uint32_t s) { return c.vp[s * 64u + c.lane]; }
 __device__ __forceinline__ uint32_t opnd(const Ctx &c, uint32_t s) { return s < 16 ? s : vpAt(c, s); }
 __device__ __forceinline__ void vpPut(const Ctx &c, uint32_t s, uint32_t v) { c.vp[s * 64u + c.lane] = uint8_t(v); }
-__device__ __forceinline__ uint32_t lutB(const Ctx &c, uint32_t pos) { return pos < kLutSize ? c.lut[pos] : 0u; }
 
-// One on-disk micropartition (half of a T_LUT2 task), ops sequential, lanes = instances.
-__device__ __forceinline__ void lutHalf(const Ctx &c, const uint32_t *S, const uint32_t *top, const uint32_t *ops, uint32_t nTop,
-                        uint32_t nMid, uint32_t half, const uint32_t *last, uint32_t nLast) {
-  uint8_t *L = c.lvl;
-  for (uint32_t i = 0; i < nTop; ++i) {
+// Middle-level operand: immediate below 32, else lane (o mod 32) of the
+// previous level, 0 past that level's size (SimEval.cu:372-404).
+__device__ __forceinline__ uint32_t midOp(const uint8_t *__restrict__ P, uint32_t o, uint32_t nPrev, uint32_t lane) {
+  return o < 32 ? o : ((o & 31u) < nPrev ? P[(o & 31u) * 64u + lane] : 0u);
+}
+
+// One on-disk micropartition, lanes = instances.
+__device__ __forceinline__ void lutHalf(const Ctx &c, const uint32_t *S, const uint32_t *top, const uint32_t *ops,
+                                        uint32_t nTop, uint32_t nMid, uint32_t half, const uint32_t *last,
+                                        uint32_t nLast) {
+  uint8_t *__restrict__ L = c.lvl;
+  const uint8_t *__restrict__ vp = c.vp;
+  const uint8_t *__restrict__ lut = c.lut;
+  const uint32_t lane = c.lane;
+  auto topOp = [&](uint32_t i) -> uint32_t {
     const uint32_t x = top[2 * i], y = top[2 * i + 1];
     const uint32_t lt = x & 0xFFFFu, o0 = x >> 16, o1 = y & 0xFFFFu, o2 = y >> 16;
-    const uint32_t cc = opnd(c, o2);
-    uint32_t v = cc;
-    if (lt != 0) v = lutB(c, lutIndex(lt, opnd(c, o0), opnd(c, o1), cc));
-    L[i * 64u + c.lane] = uint8_t(v);
+    const uint32_t cc = o2 < 16 ? o2 : vp[o2 * 64u + lane];
+    const uint32_t aa = o0 < 16 ? o0 : vp[o0 * 64u + lane];
+    const uint32_t bb = o1 < 16 ? o1 : vp[o1 * 64u + lane];
+    const uint32_t pos = lutIndex(lt, aa, bb, cc);
+    const uint32_t lv = pos < kLutSize ? lut[pos] : 0u;
+    return lt == 0 ? cc : lv;
+  };
+  uint32_t i = 0;
+  for (; i + 4 <= nTop; i += 4) {
+    const uint32_t v0 = topOp(i), v1 = topOp(i + 1), v2 = topOp(i + 2), v3 = topOp(i + 3);
+    L[i * 64u + lane] = uint8_t(v0);
+    L[(i + 1) * 64u + lane] = uint8_t(v1);
+    L[(i + 2) * 64u + lane] = uint8_t(v2);
+    L[(i + 3) * 64u + lane] = uint8_t(v3);
   }
+  for (; i < nTop; ++i) L[i * 64u + lane] = uint8_t(topOp(i));
   uint32_t nPrev = nTop, cur = 0;
   for (uint32_t m = 0; m < nMid; ++m) {
     const uint32_t sz = (S[m >> 1] >> ((m & 1u) << 4)) & 0xFFFFu;
     const uint32_t na = sz & 0xFFu, nb = sz >> 8;
     const uint32_t n = half ? nb : na;
     const uint32_t *w = ops + (half ? na : 0u);
-    const uint8_t *P = L + cur * 2048u;
-    uint8_t *Q = L + (cur ^ 1u) * 2048u;
-    for (uint32_t i = 0; i < n; ++i) {
-      const uint32_t op = w[i];
+    const uint8_t *__restrict__ P = L + cur * 2048u;
+    uint8_t *__restrict__ Q = L + (cur ^ 1u) * 2048u;
+    auto midEval = [&](uint32_t op) -> uint32_t {
       const uint32_t o2 = op & 63u, o1 = (op >> 6) & 63u, o0 = (op >> 12) & 63u, lt = op >> 18;
-      const uint32_t a = o0 < 32 ? o0 : ((o0 & 31u) < nPrev ? P[(o0 & 31u) * 64u + c.lane] : 0u);
-      const uint32_t b = o1 < 32 ? o1 : ((o1 & 31u) < nPrev ? P[(o1 & 31u) * 64u + c.lane] : 0u);
-      const uint32_t d = o2 < 32 ? o2 : ((o2 & 31u) < nPrev ? P[(o2 & 31u) * 64u + c.lane] : 0u);
-      Q[i * 64u + c.lane] = uint8_t(lt == 0 ? d : lutB(c, lutIndex(lt, a, b, d)));
+      const uint32_t aa = midOp(P, o0, nPrev, lane), bb = midOp(P, o1, nPrev, lane), dd = midOp(P, o2, nPrev, lane);
+      const uint32_t pos = lutIndex(lt, aa, bb, dd);
+      const uint32_t lv = pos < kLutSize ? lut[pos] : 0u;
+      return lt == 0 ? dd : lv;
+    };
+    uint32_t k = 0;
+    for (; k + 4 <= n; k += 4) {
+      const uint32_t w0 = w[k], w1 = w[k + 1], w2 = w[k + 2], w3 = w[k + 3];
+      const uint32_t v0 = midEval(w0), v1 = midEval(w1), v2 = midEval(w2), v3 = midEval(w3);
+      Q[k * 64u + lane] = uint8_t(v0);
+      Q[(k + 1) * 64u + lane] = uint8_t(v1);
+      Q[(k + 2) * 64u + lane] = uint8_t(v2);
+      Q[(k + 3) * 64u + lane] = uint8_t(v3);
     }
+    for (; k < n; ++k) Q[k * 64u + lane] = uint8_t(midEval(w[k]));
     ops += na + nb;
     nPrev = n;
     cur ^= 1u;
   }
-  const uint8_t *P = L + cur * 2048u;
-  for (uint32_t i = 0; i < nLast; ++i) {
-    const uint32_t w = last[i];
+  const uint8_t *__restrict__ P = L + cur * 2048u;
+  uint8_t *__restrict__ vpw = c.vp;
+  for (uint32_t k = 0; k < nLast; ++k) {
+    const uint32_t w = last[k];
     const uint32_t sid = (w >> 16) & 0xFFu;
-    const uint32_t v = sid < 32 ? sid : ((sid & 31u) < nPrev ? P[(sid & 31u) * 64u + c.lane] : 0u);
-    vpPut(c, w & 0xFFFFu, v);
+    const uint32_t v = sid < 32 ? sid : ((sid & 31u) < nPrev ? P[(sid & 31u) * 64u + lane] : 0u);
+    vpw[(w & 0xFFFFu) * 64u + lane] = uint8_t(v);
   }
 }
 
@@ -85,23 +119,18 @@ __device__ __forceinline__ void lutTask(const Ctx &c, const uint32_t *T) {
   const uint32_t w0 = T[0], w1 = T[1];
   const uint32_t nMid = (w0 >> 8) & 0xFFFFu;
   const uint32_t ntA = w1 & 0xFFu, ntB = (w1 >> 8) & 0xFFu, nlA = (w1 >> 16) & 0xFFu, nlB = w1 >> 24;
-  const uint32_t *S = T + 2;
+  const uint32_t *S = T + 3;
   const uint32_t *top = S + ((nMid + 1) >> 1);
   const uint32_t *mids = top + 2 * (ntA + ntB);
-  uint32_t midWords = 0;
-  for (uint32_t m = 0; m < nMid; ++m) {
-    const uint32_t sz = (S[m >> 1] >> ((m & 1u) << 4)) & 0xFFFFu;
-    midWords += (sz & 0xFFu) + (sz >> 8);
-  }
-  const uint32_t *last = mids + midWords;
+  const uint32_t *last = T + T[2];
   lutHalf(c, S, top, mids, ntA, nMid, 0, last, nlA);
   if (ntB | nlB) lutHalf(c, S, top + 2 * ntA, mids, ntB, nMid, 1, last + nlA, nlB);
 }
 
+// VecRead (SimEval.cu:449-481). Ops are evaluated one after another; the
+// loader rejects programs in which one op of an exec level reads a slot
+// another op of that level writes, so the order is unobservable.
 __device__ __forceinline__ void vecReadB(const Ctx &c, const uint32_t *T, const uint8_t *consts) {
-  // Ops are evaluated one after another; the loader rejects programs in which
-  // one op of an exec level reads a slot another op of that level writes, so
-  // the order is unobservable.
   const uint32_t n = T[0] >> 8;
   for (uint32_t k = 0; k < n; ++k) {
     const uint32_t *o = T + 1 + 5 * k;
@@ -113,6 +142,7 @@ __device__ __forceinline__ void vecReadB(const Ctx &c, const uint32_t *T, const 
   }
 }
 
+// VecArithOrLogic (SimEval.cu:482-548).
 __device__ __forceinline__ void vecOpB(const Ctx &c, const uint32_t *T, const uint8_t *consts) {
   const uint32_t n = T[0] >> 8;
   for (uint32_t k = 0; k < n; ++k) {
@@ -145,9 +175,9 @@ __device__ __forceinline__ void vecOpB(const Ctx &c, const uint32_t *T, const ui
   }
 }
 
+// MemRead (SimEval.cu:550-575).
 __device__ __forceinline__ void memReadB(const Ctx &c, const uint32_t *T) {
   const uint32_t n = T[0] >> 8;
-  const BatchArgs &a = c.a;
   for (uint32_t k = 0; k < n; ++k) {
     const uint32_t *o = T + 1 + 4 * k;
     const uint32_t en = opnd(c, o[2] & 0xFFFFu);
@@ -157,37 +187,83 @@ __device__ __forceinline__ void memReadB(const Ctx &c, const uint32_t *T) {
     for (uint32_t j = 0; j < 8; ++j) addr |= vpAt(c, (o[2] >> 16) + j) << (4 * j);
     if (o[3] >> 16) addr <<= 2;
     const uint64_t at = (uint64_t(o[0]) | uint64_t(o[1]) << 32) + addr;
-    vpPut(c, o[3] & 0xFFFFu, at < a.memSize ? a.mem[at * a.instPad + c.slot0 + c.lane] : 0u);
+    vpPut(c, o[3] & 0xFFFFu, at < c.memSize ? c.mem[at * c.instPad + c.slot0 + c.lane] : 0u);
   }
 }
 
-__device__ __forceinline__ void batchItem(const BatchArgs &a, uint32_t pid, uint32_t g, uint8_t *vp, uint8_t *lvlBase,
-                          const uint8_t *lut, const uint64_t *draws, uint32_t c) {
-  const uint32_t *P = a.words + a.partOff[pid];
-  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6, nWaves = blockDim.x >> 6;
+// L0 reads of one group: op k copies pool[src_k][slot0 .. slot0+64) into
+// vp[slot_k][0 .. 64); lane l moves 16 bytes of op (k0 + l/4).
+__device__ __forceinline__ void wideRead(uint8_t *__restrict__ vp, const uint8_t *__restrict__ pool, size_t instPad,
+                                         uint32_t slot0, const uint32_t *__restrict__ L, uint32_t n, uint32_t lane) {
+  const uint32_t q = lane >> 2, j = (lane & 3u) * 16u;
+  const uint8_t *base = pool + slot0 + j;
+  uint32_t k = q;
+  for (; k + 48 < n; k += 64) {
+    const uint32_t s0 = L[2 * k], d0 = L[2 * k + 1], s1 = L[2 * k + 32], d1 = L[2 * k + 33];
+    const uint32_t s2 = L[2 * k + 64], d2 = L[2 * k + 65], s3 = L[2 * k + 96], d3 = L[2 * k + 97];
+    const uint4 v0 = *reinterpret_cast<const uint4 *>(base + size_t(s0) * instPad);
+    const uint4 v1 = *reinterpret_cast<const uint4 *>(base + size_t(s1) * instPad);
+    const uint4 v2 = *reinterpret_cast<const uint4 *>(base + size_t(s2) * instPad);
+    const uint4 v3 = *reinterpret_cast<const uint4 *>(base + size_t(s3) * instPad);
+    *reinterpret_cast<uint4 *>(vp + d0 * 64u + j) = v0;
+    *reinterpret_cast<uint4 *>(vp + d1 * 64u + j) = v1;
+    *reinterpret_cast<uint4 *>(vp + d2 * 64u + j) = v2;
+    *reinterpret_cast<uint4 *>(vp + d3 * 64u + j) = v3;
+  }
+  for (; k < n; k += 16) {
+    const uint32_t s0 = L[2 * k], d0 = L[2 * k + 1];
+    *reinterpret_cast<uint4 *>(vp + d0 * 64u + j) = *reinterpret_cast<const uint4 *>(base + size_t(s0) * instPad);
+  }
+}
+
+// Word copy of the last level (SimEval.cu:658-665,751-758) for one group:
+// rows [0, n) of vp from `dat` to `pool` from `row0`, live lanes only
+// (the live lanes of a group are a prefix of length `live`).
+__device__ __forceinline__ void wideWrite(uint8_t *pool, size_t instPad, uint32_t slot0, uint32_t row0,
+                                          const uint8_t *vp, uint32_t dat, uint32_t n, uint32_t lane, uint32_t live) {
+  const uint32_t q = lane >> 2, j = (lane & 3u) * 16u;
+  for (uint32_t k = q; k < n; k += 16) {
+    const uint4 v = *reinterpret_cast<const uint4 *>(vp + (dat + k) * 64u + j);
+    uint8_t *dst = pool + size_t(row0 + k) * instPad + slot0 + j;
+    if (j + 16 <= live) {
+      *reinterpret_cast<uint4 *>(dst) = v;
+    } else if (j < live) {
+      for (uint32_t t = 0; t < live - j; ++t) {
+        const uint32_t w = t < 4 ? v.x : t < 8 ? v.y : t < 12 ? v.z : v.w;
+        dst[t] = uint8_t(w >> ((t & 3u) * 8u));
+      }
+    }
+  }
+}
+
+// One (partition, group) evaluated by one wave, no workgroup barriers.
+__device__ __forceinline__ void waveItem(const BatchArgs &a, const uint32_t *P, uint32_t g, uint8_t *vp, uint8_t *lvl,
+                                         const uint8_t *lut, const uint64_t *draws, uint32_t c) {
+  const uint32_t lane = threadIdx.x & 63u;
   const uint32_t slot0 = g * 64u, inst = slot0 + lane;
-  const bool live = c < a.runCycles[inst];
+  const bool liveLane = c < a.runCycles[inst];
+  const uint32_t live = __popcll(__ballot(liveLane));
   const uint64_t cycle = a.cycleBase[inst] + c;
-  Ctx cx{a, vp, lvlBase + wave * 4096u, lut, lane, g, slot0, cycle};
+  const size_t IP = a.instPad;
+  Ctx cx{a.mem, a.memSize, IP, vp, lvl, lut, lane, slot0};
   {
     const uint32_t nrr = P[H_NRR], ner = P[H_NER], nsr = P[H_NSR];
     const uint32_t *L = P + P[H_OFF_L0];
-    for (uint32_t k = wave; k < nrr; k += nWaves) vpPut(cx, L[2 * k + 1], a.reg[size_t(L[2 * k]) * a.instPad + inst]);
+    wideRead(vp, a.reg, IP, slot0, L, nrr, lane);
     L += 2 * nrr;
-    for (uint32_t k = wave; k < ner; k += nWaves) vpPut(cx, L[2 * k + 1], a.exg[size_t(L[2 * k]) * a.instPad + inst]);
+    wideRead(vp, a.exg, IP, slot0, L, ner, lane);
     L += 2 * ner;
-    for (uint32_t k = wave; k < nsr; k += nWaves) {
+    for (uint32_t k = 0; k < nsr; ++k) {
       const uint32_t in = L[2 * k];
-      vpPut(cx, L[2 * k + 1], uint32_t((draws[size_t(in >> 6) * a.instPad + inst] >> (in & 63)) & 1u));
+      vpPut(cx, L[2 * k + 1], uint32_t((draws[size_t(in >> 6) * IP + inst] >> (in & 63)) & 1u));
     }
   }
-  __syncthreads();
   const uint8_t *consts = a.consts + P[H_CONST_BASE];
   const uint32_t nLev = P[H_NLEV];
   const uint32_t *lev = P + P[H_OFF_LEV];
   const uint32_t *tab = P + P[H_OFF_TASK];
   for (uint32_t e = 0; e < nLev; ++e) {
-    for (uint32_t t = lev[2 * e] + wave; t < lev[2 * e + 1]; t += nWaves) {
+    for (uint32_t t = lev[2 * e]; t < lev[2 * e + 1]; ++t) {
       const uint32_t *T = P + tab[t];
       const uint32_t kind = T[0] & 0xFFu;
       if (kind == T_LUT2) lutTask(cx, T);
@@ -195,57 +271,53 @@ __device__ __forceinline__ void batchItem(const BatchArgs &a, uint32_t pid, uint
       else if (kind == T_VECOP) vecOpB(cx, T, consts);
       else memReadB(cx, T);
     }
-    __syncthreads();
   }
   // last level (SimEval.cu:625-761); lanes past their cycle count write nothing
   const uint32_t ewc = P[H_EW_DC] >> 16;
   if (ewc) {
-    const uint32_t exg = (P[H_EW_EXG] >> 2) * 4, dat = ((P[H_EW_DC] & 0xFFFFu) >> 2) * 4, n = (ewc + 3) / 4 * 4;
-    if (live)
-      for (uint32_t k = wave; k < n; k += nWaves) a.exg[size_t(exg + k) * a.instPad + inst] = uint8_t(vpAt(cx, dat + k));
-  } else {
-    const uint32_t rwc = P[H_RW_DC] >> 16;
-    if (rwc && live) {
-      const uint32_t reg = (P[H_RW_REG] >> 2) * 4, dat = ((P[H_RW_DC] & 0xFFFFu) >> 2) * 4, n = (rwc + 3) / 4 * 4;
-      for (uint32_t k = wave; k < n; k += nWaves) a.reg[size_t(reg + k) * a.instPad + inst] = uint8_t(vpAt(cx, dat + k));
-    }
-    const uint32_t nmw = P[H_NMW], npr = P[H_NPR], nst = P[H_NST];
-    const uint32_t *L = P + P[H_OFF_LAST];
-    if (live)
-      for (uint32_t k = wave; k < nmw; k += nWaves) {
-        const uint32_t *o = L + 4 * k;
-        if (opnd(cx, o[3] & 0xFFFFu) == 0) continue;
-        const uint32_t dat = opnd(cx, o[2] >> 16);
-        uint32_t addr = 0;
-#pragma unroll
-        for (uint32_t j = 0; j < 8; ++j) addr |= vpAt(cx, (o[2] & 0xFFFFu) + j) << (4 * j);
-        if (o[3] >> 16) addr <<= 2;
-        const uint64_t at = (uint64_t(o[0]) | uint64_t(o[1]) << 32) + addr;
-        if (at < a.memSize) a.mem[at * a.instPad + inst] = uint8_t(dat);
-      }
-    L += 4 * nmw;
-    if (live && a.printEnable)
-      for (uint32_t k = wave; k < npr; k += nWaves)
-        if (opnd(cx, L[k] & 0xFFFFu) != 0) {
-          uint32_t q = atomicAdd(a.logCount, 1u);
-          if (q < a.logCap) {
-            a.logCycle[q] = cycle;
-            a.logMsg[q] = uint64_t(L[k] >> 16) | (uint64_t(a.instOf[inst]) << 32);
-          }
-        }
-    L += npr;
-    if (live)
-      for (uint32_t k = wave; k < nst; k += nWaves)
-        if (opnd(cx, L[k] & 0xFFFFu) != 0) {
-          atomicOr(reinterpret_cast<unsigned long long *>(&a.stop[inst]), 1ull);
-          atomicMin(reinterpret_cast<unsigned long long *>(&a.stopCycle[inst]), (unsigned long long)cycle);
-        }
+    wideWrite(a.exg, IP, slot0, (P[H_EW_EXG] >> 2) * 4, vp, ((P[H_EW_DC] & 0xFFFFu) >> 2) * 4, (ewc + 3) / 4 * 4, lane,
+              live);
+    return;
   }
-  __syncthreads();
+  const uint32_t rwc = P[H_RW_DC] >> 16;
+  if (rwc)
+    wideWrite(a.reg, IP, slot0, (P[H_RW_REG] >> 2) * 4, vp, ((P[H_RW_DC] & 0xFFFFu) >> 2) * 4, (rwc + 3) / 4 * 4, lane,
+              live);
+  const uint32_t nmw = P[H_NMW], npr = P[H_NPR], nst = P[H_NST];
+  const uint32_t *L = P + P[H_OFF_LAST];
+  if (liveLane)
+    for (uint32_t k = 0; k < nmw; ++k) {
+      const uint32_t *o = L + 4 * k;
+      if (opnd(cx, o[3] & 0xFFFFu) == 0) continue;
+      const uint32_t dat = opnd(cx, o[2] >> 16);
+      uint32_t addr = 0;
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) addr |= vpAt(cx, (o[2] & 0xFFFFu) + j) << (4 * j);
+      if (o[3] >> 16) addr <<= 2;
+      const uint64_t at = (uint64_t(o[0]) | uint64_t(o[1]) << 32) + addr;
+      if (at < a.memSize) a.mem[at * IP + inst] = uint8_t(dat);
+    }
+  L += 4 * nmw;
+  if (liveLane && a.printEnable)
+    for (uint32_t k = 0; k < npr; ++k)
+      if (opnd(cx, L[k] & 0xFFFFu) != 0) {
+        uint32_t q = atomicAdd(a.logCount, 1u);
+        if (q < a.logCap) {
+          a.logCycle[q] = cycle;
+          a.logMsg[q] = uint64_t(L[k] >> 16) | (uint64_t(a.instOf[inst]) << 32);
+        }
+      }
+  L += npr;
+  if (liveLane)
+    for (uint32_t k = 0; k < nst; ++k)
+      if (opnd(cx, L[k] & 0xFFFFu) != 0) {
+        atomicOr(reinterpret_cast<unsigned long long *>(&a.stop[inst]), 1ull);
+        atomicMin(reinterpret_cast<unsigned long long *>(&a.stopCycle[inst]), (unsigned long long)cycle);
+      }
 }
 
-// Advance the stimulus of every live instance of the groups this workgroup
-// owns and store the draws of cycle `cycle` in buffer `buf`.
+// Advance the stimulus of every live instance of the active groups and store
+// the draws of launch cycle `c` in buffer `buf`.
 __device__ __forceinline__ void stimulusFor(const BatchArgs &a, uint32_t buf, uint32_t c, uint32_t activeGroups) {
   if (a.nIn == 0) return;
   const uint32_t nDraws = (a.nIn + 63) / 64;
@@ -261,13 +333,40 @@ __device__ __forceinline__ void stimulusFor(const BatchArgs &a, uint32_t buf, ui
   }
 }
 
+// Copy a partition's program words (16-byte aligned, padded to 4 words) into LDS.
+__device__ __forceinline__ void stageProgram(uint32_t *dst, const uint32_t *src, uint32_t words) {
+  const uint32_t n4 = (words + 3) >> 2;
+  for (uint32_t k = threadIdx.x; k < n4; k += blockDim.x)
+    reinterpret_cast<uint4 *>(dst)[k] = reinterpret_cast<const uint4 *>(src)[k];
+}
+
+// Work items of one region: (partition, W consecutive groups), one group per wave.
+__device__ __forceinline__ void batchRegion(const BatchArgs &a, uint32_t pb, uint32_t pe, uint32_t active,
+                                            uint8_t *vpAll, uint8_t *lvlAll, uint32_t *prog, const uint8_t *lut,
+                                            const uint64_t *draws, uint32_t c) {
+  const uint32_t np = pe - pb, W = blockDim.x >> 6, wave = threadIdx.x >> 6;
+  const uint32_t nChunks = (active + W - 1) / W;
+  const uint32_t items = np * nChunks;
+  uint8_t *vp = vpAll + wave * a.vpBytes * 64u;
+  uint8_t *lvl = lvlAll + wave * 4096u;
+  for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
+    const uint32_t p = pb + k / nChunks, g = (k % nChunks) * W + wave;
+    stageProgram(prog, a.words + a.partOff[p], a.partWords[p]);
+    __syncthreads();
+    if (g < active) waveItem(a, prog, g, vp, lvl, lut, draws, c);
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 __global__ __launch_bounds__(1024) void batchCycles(BatchArgs a) {
   extern __shared__ __align__(16) uint8_t bLds[];
+  const uint32_t W = blockDim.x >> 6;
   uint8_t *lut = bLds;
   uint8_t *vp = bLds + kLutBytesB;
-  uint8_t *lvl = vp + a.vpBytes * 64u;
+  uint8_t *lvl = vp + W * a.vpBytes * 64u;
+  uint32_t *prog = reinterpret_cast<uint32_t *>(lvl + W * 4096u);
   for (uint32_t k = threadIdx.x; k < kLutBytesB / 4; k += blockDim.x)
     reinterpret_cast<uint32_t *>(lut)[k] = reinterpret_cast<const uint32_t *>(a.lut)[k];
   const uint32_t nDraws = (a.nIn + 63) / 64;
@@ -285,15 +384,11 @@ __global__ __launch_bounds__(1024) void batchCycles(BatchArgs a) {
       if (c < a.runCycles[s])
         for (uint32_t i = 0; i < a.nIn; ++i)
           a.reg[size_t(a.inIds[i]) * a.instPad + s] = uint8_t((draws[size_t(i >> 6) * a.instPad + s] >> (i & 63)) & 1u);
-    const uint32_t items0 = a.n0 * active;
-    for (uint32_t k = blockIdx.x; k < items0; k += gridDim.x)
-      batchItem(a, k / active, k % active, vp, lvl, lut, draws, c);
+    batchRegion(a, 0, a.n0, active, vp, lvl, prog, lut, draws, c);
     bar++;
     gridBarrier(a.barrier, bar * gridDim.x, false);
     if (c + 1 < a.nCycles) stimulusFor(a, (c + 1) & 1u, c + 1, a.groupsActive[c + 1]);
-    const uint32_t items1 = (a.nTotal - a.n0) * active;
-    for (uint32_t k = blockIdx.x; k < items1; k += gridDim.x)
-      batchItem(a, a.n0 + k / active, k % active, vp, lvl, lut, draws, c);
+    batchRegion(a, a.n0, a.nTotal, active, vp, lvl, prog, lut, draws, c);
     bar++;
     gridBarrier(a.barrier, bar * gridDim.x, false);
   }
@@ -314,10 +409,6 @@ hipError_t batchSetLds(uint32_t lds) {
   return hipFuncSetAttribute(reinterpret_cast<const void *>(batchCycles), hipFuncAttributeMaxDynamicSharedMemorySize,
                              lds);
 }
-
-}  // namespace tgs
-
-namespace tgs {
 
 // Reorder instance slots of an instance-minor pool: out[r][s] = in[r][src[s]].
 __global__ void permuteSlots(const uint8_t *in, uint8_t *out, uint64_t rows, uint32_t instPad, const uint32_t *src) {

@@ -5,14 +5,17 @@
 // is re-planned for CDNA4:
 //   * one persistent workgroup per CU, partitions statically assigned per
 //     region by a longest-processing-time schedule (no atomic work queues);
-//   * each workgroup's program words, the 4642-entry LUT and the value pool
-//     live in LDS for the whole launch when they fit (RESIDENT), so a cycle
-//     reads HBM only for register, exchange and memory state;
+//   * the 4642-entry LUT and the value pool live in LDS; the program words
+//     are evaluated from LDS too: either resident for the whole launch (when
+//     the workgroup's partitions fit) or staged per partition visit with
+//     16-byte coalesced copies, so no op word is a dependent HBM load;
 //   * two on-disk 32-lane micropartitions run in the two halves of one
-//     64-lane wave; operand shuffles are ds_bpermute within a half;
+//     64-lane wave; operand shuffles are ds_bpermute within a half, and the
+//     next level's op word is loaded while the current level evaluates;
 //   * vector/memory ops of an exec level are regrouped 64 per wave;
-//   * one atomic-counter grid barrier per region with agent-scope
-//     release/acquire, launched cooperatively so all workgroups are resident.
+//   * one atomic-counter grid barrier per region (one agent-scope
+//     release/acquire per workgroup), launched cooperatively so every
+//     workgroup is resident.
 #include <hip/hip_runtime.h>
 
 #include "kernel_args.h"
@@ -27,38 +30,65 @@ __device__ __forceinline__ uint32_t rdOp(const uint8_t *vp, uint32_t s) { return
 
 __device__ __forceinline__ uint32_t lutL(const uint8_t *lut, uint32_t pos) { return pos < kLutSize ? lut[pos] : 0u; }
 
+// Copy `words` u32 words (16-byte aligned source and destination, the source
+// padded to a 4-word multiple) with the whole workgroup.
+__device__ __forceinline__ void copyWords(uint32_t *dst, const uint32_t *src, uint32_t words) {
+  const uint32_t n4 = (words + 3) >> 2;
+  const uint4 *s = reinterpret_cast<const uint4 *>(src);
+  uint4 *d = reinterpret_cast<uint4 *>(dst);
+  for (uint32_t k = threadIdx.x; k < n4; k += blockDim.x) d[k] = s[k];
+}
+
 // Two 32-lane LUT micropartitions in one wave (SimEval.cu:319-448).
 __device__ __forceinline__ void lutPair(const uint32_t *T, uint8_t *vp, const uint8_t *lut) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t h = lane >> 5, i = lane & 31u, hb = h << 5;
-  const uint32_t w0 = T[0], w1 = T[1];
+  const uint32_t w0 = T[0], w1 = T[1], lastOff = T[2];
   const uint32_t nMid = (w0 >> 8) & 0xFFFFu;
   const uint32_t ntA = w1 & 0xFFu, ntB = (w1 >> 8) & 0xFFu, nlA = (w1 >> 16) & 0xFFu, nlB = w1 >> 24;
-  const uint32_t *S = T + 2;
+  const uint32_t *S = T + 3;
   const uint32_t *ops = S + ((nMid + 1) >> 1);
+  // independent loads first: this lane's top op and last-level op
+  const uint32_t nt = h ? ntB : ntA, tb = h ? ntA : 0u;
+  uint32_t x = 0, y = 0;
+  if (i < nt) {
+    x = ops[2 * (tb + i)];
+    y = ops[2 * (tb + i) + 1];
+  }
+  const uint32_t nl = h ? nlB : nlA, lb = h ? nlA : 0u;
+  const uint32_t wl = i < nl ? T[lastOff + lb + i] : 0u;
+  ops += 2 * (ntA + ntB);
   uint32_t val = 0;
-  {
-    const uint32_t n = h ? ntB : ntA, base = h ? ntA : 0u;
-    if (i < n) {
-      const uint32_t x = ops[2 * (base + i)], y = ops[2 * (base + i) + 1];
-      const uint32_t lt = x & 0xFFFFu, o0 = x >> 16, o1 = y & 0xFFFFu, o2 = y >> 16;
-      const uint32_t c = rdOp(vp, o2);
-      if (lt == 0) {
-        val = c;
-      } else {
-        const uint32_t a = rdOp(vp, o0), b = rdOp(vp, o1);
-        val = lutL(lut, lutIndex(lt, a, b, c));
-      }
+  if (i < nt) {
+    const uint32_t lt = x & 0xFFFFu, o0 = x >> 16, o1 = y & 0xFFFFu, o2 = y >> 16;
+    const uint32_t c = rdOp(vp, o2);
+    if (lt == 0) {
+      val = c;
+    } else {
+      const uint32_t a = rdOp(vp, o0), b = rdOp(vp, o1);
+      val = lutL(lut, lutIndex(lt, a, b, c));
     }
-    ops += 2 * (ntA + ntB);
+  }
+  // middle levels, the next level's op word loaded one level ahead
+  uint32_t w = 0, na = 0, nb = 0;
+  if (nMid) {
+    const uint32_t sz = S[0] & 0xFFFFu;
+    na = sz & 0xFFu;
+    nb = sz >> 8;
+    const uint32_t n = h ? nb : na;
+    w = i < n ? ops[(h ? na : 0u) + i] : 0u;
   }
   for (uint32_t m = 0; m < nMid; ++m) {
-    const uint32_t sz = (S[m >> 1] >> ((m & 1u) << 4)) & 0xFFFFu;
-    const uint32_t na = sz & 0xFFu, nb = sz >> 8;
-    const uint32_t n = h ? nb : na, base = h ? na : 0u;
-    const uint32_t w = i < n ? ops[base + i] : 0u;
+    const uint32_t cur = w;
     ops += na + nb;
-    const uint32_t o2 = w & 63u, o1 = (w >> 6) & 63u, o0 = (w >> 12) & 63u, lt = w >> 18;
+    if (m + 1 < nMid) {
+      const uint32_t sz = (S[(m + 1) >> 1] >> (((m + 1) & 1u) << 4)) & 0xFFFFu;
+      na = sz & 0xFFu;
+      nb = sz >> 8;
+      const uint32_t n = h ? nb : na;
+      w = i < n ? ops[(h ? na : 0u) + i] : 0u;
+    }
+    const uint32_t o2 = cur & 63u, o1 = (cur >> 6) & 63u, o0 = (cur >> 12) & 63u, lt = cur >> 18;
     const uint32_t s0 = __shfl(val, int((o0 & 31u) | hb));
     const uint32_t s1 = __shfl(val, int((o1 & 31u) | hb));
     const uint32_t s2 = __shfl(val, int((o2 & 31u) | hb));
@@ -66,12 +96,10 @@ __device__ __forceinline__ void lutPair(const uint32_t *T, uint8_t *vp, const ui
     const uint32_t lv = lutL(lut, lutIndex(lt, a, b, c));
     val = lt == 0 ? c : lv;
   }
-  const uint32_t n = h ? nlB : nlA, base = h ? nlA : 0u;
-  const uint32_t w = i < n ? ops[base + i] : 0u;
-  const uint32_t sid = (w >> 16) & 0xFFu;
+  const uint32_t sid = (wl >> 16) & 0xFFu;
   uint32_t r = __shfl(val, int((sid & 31u) | hb));
   if (sid < 32) r = sid;
-  if (i < n) vp[w & 0xFFFFu] = uint8_t(r);
+  if (i < nl) vp[wl & 0xFFFFu] = uint8_t(r);
 }
 
 // VecRead (SimEval.cu:449-481), up to 64 ops per wave.
@@ -136,9 +164,8 @@ __device__ __forceinline__ void memReadTask(const uint32_t *T, uint8_t *vp, cons
   vp[o[3] & 0xFFFFu] = at < c.memSize ? c.mem[at] : uint8_t(0);
 }
 
-template <bool RES>
-__device__ void w64Partition(const W64Args &a, const uint32_t *P, uint8_t *vp, const uint8_t *lut,
-                             const uint64_t *draws, uint64_t cycle) {
+__device__ __forceinline__ void w64Partition(const W64Args &a, const uint32_t *P, uint8_t *vp, const uint8_t *lut,
+                                             const uint64_t *draws, uint64_t cycle) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t wave = tid >> 6, nWaves = nt >> 6;
   // L0 (SimEval.cu:582-622) plus stimulus reads
@@ -203,9 +230,19 @@ __device__ void w64Partition(const W64Args &a, const uint32_t *P, uint8_t *vp, c
   __syncthreads();
 }
 
+template <int MODE>
+__device__ __forceinline__ const uint32_t *progOf(const W64Args &a, uint32_t *prog, uint32_t s) {
+  if (MODE == PM_RESIDENT) return prog + a.schedLds[s];
+  const uint32_t *src = a.words + a.partOff[a.schedPart[s]];
+  if (MODE == PM_GLOBAL) return src;
+  copyWords(prog, src, a.partWords[a.schedPart[s]]);
+  __syncthreads();
+  return prog;
+}
+
 }  // namespace
 
-template <bool RES>
+template <int MODE>
 __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
   extern __shared__ __align__(16) uint8_t w64Lds[];
   uint8_t *lut = w64Lds;
@@ -216,14 +253,9 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
   for (uint32_t k = tid; k < kLutBytes / 4; k += nt)
     reinterpret_cast<uint32_t *>(lut)[k] = reinterpret_cast<const uint32_t *>(a.c.lut)[k];
   const uint32_t s0 = a.schedOff[2 * g], s1 = a.schedOff[2 * g + 1], s2 = a.schedOff[2 * g + 2];
-  if (RES) {
-    for (uint32_t s = s0; s < s2; ++s) {
-      const uint32_t *src = a.words + a.partOff[a.schedPart[s]];
-      uint32_t *dst = prog + a.schedLds[s];
-      const uint32_t n = a.partWords[a.schedPart[s]];
-      for (uint32_t k = tid; k < n; k += nt) dst[k] = src[k];
-    }
-  }
+  if (MODE == PM_RESIDENT)
+    for (uint32_t s = s0; s < s2; ++s)
+      copyWords(prog + a.schedLds[s], a.words + a.partOff[a.schedPart[s]], a.partWords[a.schedPart[s]]);
   __syncthreads();
   uint64_t xs = *a.c.xs;
   uint64_t barTicks = 0;
@@ -236,15 +268,9 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
     __syncthreads();
     if (g == 0)
       for (uint32_t k = tid; k < a.c.nIn; k += nt) a.c.reg[a.c.inIds[k]] = uint8_t(stimBit(draws, k));
-    for (uint32_t s = s0; s < s1; ++s) {
-      const uint32_t *P = RES ? prog + a.schedLds[s] : a.words + a.partOff[a.schedPart[s]];
-      w64Partition<RES>(a, P, vp, lut, draws, cycle);
-    }
+    for (uint32_t s = s0; s < s1; ++s) w64Partition(a, progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
     barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, prof);
-    for (uint32_t s = s1; s < s2; ++s) {
-      const uint32_t *P = RES ? prog + a.schedLds[s] : a.words + a.partOff[a.schedPart[s]];
-      w64Partition<RES>(a, P, vp, lut, draws, cycle);
-    }
+    for (uint32_t s = s1; s < s2; ++s) w64Partition(a, progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
     barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, prof);
   }
   if (g == 0 && tid == 0) {
@@ -254,27 +280,31 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
   }
 }
 
-hipError_t launchWave64(const W64Args &args, bool resident, uint32_t grid, uint32_t block, uint32_t lds,
+namespace {
+const void *w64Fn(int mode) {
+  return mode == PM_RESIDENT ? reinterpret_cast<const void *>(w64Cycles<PM_RESIDENT>)
+         : mode == PM_STAGED ? reinterpret_cast<const void *>(w64Cycles<PM_STAGED>)
+                             : reinterpret_cast<const void *>(w64Cycles<PM_GLOBAL>);
+}
+}  // namespace
+
+hipError_t launchWave64(const W64Args &args, int mode, uint32_t grid, uint32_t block, uint32_t lds,
                         hipStream_t stream) {
   W64Args a = args;
   void *params[] = {&a};
-  const void *fn = resident ? reinterpret_cast<const void *>(w64Cycles<true>)
-                            : reinterpret_cast<const void *>(w64Cycles<false>);
-  return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), params, lds, stream);
+  return hipLaunchCooperativeKernel(w64Fn(mode), dim3(grid), dim3(block), params, lds, stream);
 }
 
-hipError_t wave64MaxBlocksPerCU(int *blocks, bool resident, uint32_t block, uint32_t lds) {
-  const void *fn = resident ? reinterpret_cast<const void *>(w64Cycles<true>)
-                            : reinterpret_cast<const void *>(w64Cycles<false>);
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, fn, block, lds);
+hipError_t wave64MaxBlocksPerCU(int *blocks, int mode, uint32_t block, uint32_t lds) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, w64Fn(mode), block, lds);
 }
 
 hipError_t wave64SetLds(uint32_t lds) {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(w64Cycles<true>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(w64Cycles<false>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+  for (int m = 0; m < 3; ++m) {
+    hipError_t e = hipFuncSetAttribute(w64Fn(m), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 uint32_t wave64LutBytes() { return kLutBytes; }

@@ -108,7 +108,7 @@ struct tgs_ctx {
   std::vector<std::unique_ptr<Instance>> inst;
   bool batched = false;  // instance state lives in `bs` (batched kernel)
   BatchState bs;
-  uint32_t batchBlock = 256, batchGrid = 0, batchLds = 0, batchGridOverride = 0;
+  uint32_t batchBlock = 256, batchBlockEff = 256, batchGrid = 0, batchLds = 0, batchGridOverride = 0;
   bool batchReady = false;
 
   DBuf lut, inIds;
@@ -119,7 +119,8 @@ struct tgs_ctx {
   Wave64Schedule sched;
   DBuf wWords, wPartOff, wPartWords, wSchedOff, wSchedPart, wSchedLds, wConsts;
   uint32_t w64Block = 512, w64Groups = 0, w64Lds = 0;
-  bool w64Resident = false;
+  int w64Mode = PM_GLOBAL;
+  int w64ModeForce = -1;
   uint32_t plainGrid = 0, plainBlock = 1024, plainLds = 0;
 
   tgs_profile prof{};
@@ -222,8 +223,12 @@ int buildW64(tgs_ctx *c) {
   const uint32_t base = wave64LutBytes() + up16(std::max<uint32_t>(im.maxVp, 16)) + kMaxDraws * 8;
   const uint32_t resLds = base + c->sched.maxGroupWords * 4;
   const size_t limit = c->prop.sharedMemPerBlock;
-  c->w64Resident = resLds <= limit;
-  c->w64Lds = c->w64Resident ? resLds : base;
+  uint32_t maxPart = 0;
+  for (auto w : im.partWords) maxPart = std::max(maxPart, (w + 3) & ~3u);
+  const uint32_t stagedLds = base + maxPart * 4;
+  c->w64Mode = resLds <= limit ? PM_RESIDENT : stagedLds <= limit ? PM_STAGED : PM_GLOBAL;
+  if (c->w64ModeForce >= 0 && c->w64ModeForce < c->w64Mode) c->w64Mode = c->w64ModeForce;
+  c->w64Lds = c->w64Mode == PM_RESIDENT ? resLds : c->w64Mode == PM_STAGED ? stagedLds : base;
   if (base > limit) return c->fail(TGS_E_LIMIT, "value pool needs %u B of LDS, device allows %zu B", base, limit);
   int rc;
   if ((rc = upload(c, c->wWords, im.words.data(), im.words.size() * 4))) return rc;
@@ -235,7 +240,7 @@ int buildW64(tgs_ctx *c) {
   if ((rc = upload(c, c->wConsts, im.consts.data(), im.consts.size()))) return rc;
   HIPCHK(c, wave64SetLds(c->w64Lds));
   int perCU = 0;
-  HIPCHK(c, wave64MaxBlocksPerCU(&perCU, c->w64Resident, c->w64Block, c->w64Lds));
+  HIPCHK(c, wave64MaxBlocksPerCU(&perCU, c->w64Mode, c->w64Block, c->w64Lds));
   if (perCU < 1) return c->fail(TGS_E_LIMIT, "wave64 kernel cannot be resident with %u B LDS", c->w64Lds);
   if (uint64_t(perCU) * c->prop.multiProcessorCount < groups)
     return c->fail(TGS_E_LIMIT, "%u workgroups exceed the co-resident limit", groups);
@@ -314,12 +319,12 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles) {
       a.barrierBase = 0;
       HIPCHK(c, hipMemsetAsync(in.ctl.as<uint32_t>() + C_BARRIER, 0, 4, c->stream));
       HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-      HIPCHK(c, launchWave64(a, c->w64Resident, c->sched.groups, c->w64Block, c->w64Lds, c->stream));
+      HIPCHK(c, launchWave64(a, c->w64Mode, c->sched.groups, c->w64Block, c->w64Lds, c->stream));
       HIPCHK(c, hipEventRecord(c->ev1, c->stream));
       c->prof.grid = c->sched.groups;
       c->prof.block = c->w64Block;
       c->prof.lds_bytes = c->w64Lds;
-      c->prof.resident = c->w64Resident;
+      c->prof.resident = uint32_t(c->w64Mode);
       c->prof.prog_bytes = c->w64.words.size() * 4;
       cycles -= n;
       in.cycles += n;
@@ -509,18 +514,27 @@ int ensureBatchImage(tgs_ctx *c) {
   int rc;
   if ((rc = upload(c, c->wWords, im.words.data(), im.words.size() * 4))) return rc;
   if ((rc = upload(c, c->wPartOff, im.partOff.data(), im.partOff.size() * 4))) return rc;
+  if ((rc = upload(c, c->wPartWords, im.partWords.data(), im.partWords.size() * 4))) return rc;
   if ((rc = upload(c, c->wConsts, im.consts.data(), im.consts.size()))) return rc;
   const uint32_t vpBytes = up16(std::max<uint32_t>(im.maxVp, 16));
-  c->batchLds = wave64LutBytes() + vpBytes * 64 + nWaves * 4096;
-  if (c->batchLds > c->prop.sharedMemPerBlock)
+  uint32_t maxPart = 0;
+  for (auto w : im.partWords) maxPart = std::max(maxPart, (w + 3) & ~3u);
+  // Each wave holds one 64-instance group's value pool and level buffer; the
+  // LUT and the staged partition program are shared by the workgroup.
+  const uint32_t perWave = vpBytes * 64 + 4096, shared = wave64LutBytes() + maxPart * 4;
+  const uint32_t ldsMax = uint32_t(c->prop.sharedMemPerBlock);
+  if (shared + perWave > ldsMax)
     return c->fail(TGS_E_LIMIT,
-                   "batched execution needs %u B of LDS (value pool %u slots x 64 instances); device allows %zu B",
-                   c->batchLds, im.maxVp, size_t(c->prop.sharedMemPerBlock));
+                   "batched execution needs %u B of LDS (value pool %u slots x 64 instances); device allows %u B",
+                   shared + perWave, im.maxVp, ldsMax);
+  const uint32_t waves = std::min(nWaves, (ldsMax - shared) / perWave);
+  c->batchBlockEff = waves * 64;
+  c->batchLds = shared + waves * perWave;
   HIPCHK(c, batchSetLds(c->batchLds));
   int perCU = 0;
-  HIPCHK(c, batchMaxBlocksPerCU(&perCU, c->batchBlock, c->batchLds));
+  HIPCHK(c, batchMaxBlocksPerCU(&perCU, c->batchBlockEff, c->batchLds));
   if (perCU < 1) return c->fail(TGS_E_LIMIT, "batched kernel cannot be resident with %u B LDS", c->batchLds);
-  perCU = std::min(perCU, 4);
+  perCU = std::min<int>(perCU, std::max<uint32_t>(1, 32 / waves));
   c->batchGrid = uint32_t(perCU) * c->prop.multiProcessorCount;
   if (c->batchGridOverride) c->batchGrid = std::min(c->batchGrid, c->batchGridOverride);
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -612,6 +626,7 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
     a.instPad = P;
     a.words = c->wWords.as<uint32_t>();
     a.partOff = c->wPartOff.as<uint32_t>();
+    a.partWords = c->wPartWords.as<uint32_t>();
     a.n0 = c->w64.n0;
     a.nTotal = c->w64.nTotal;
     a.vpBytes = up16(std::max<uint32_t>(c->w64.maxVp, 16));
@@ -633,7 +648,7 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
     a.printEnable = c->printEnable;
     a.barrier = b.barrier.as<uint32_t>();
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    HIPCHK(c, launchBatch(a, c->batchGrid, c->batchBlock, c->batchLds, c->stream));
+    HIPCHK(c, launchBatch(a, c->batchGrid, c->batchBlockEff, c->batchLds, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0;
@@ -642,7 +657,7 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
     c->prof.launches += 1;
     c->prof.kernel = TGS_KERNEL_BATCH;
     c->prof.grid = c->batchGrid;
-    c->prof.block = c->batchBlock;
+    c->prof.block = c->batchBlockEff;
     c->prof.lds_bytes = c->batchLds;
     c->prof.prog_bytes = c->w64.words.size() * 4;
     maxC = 0;
@@ -680,13 +695,6 @@ int readStop(tgs_ctx *c, Instance &in, int *stopped) {
   return TGS_OK;
 }
 
-Instance *instanceOf(tgs_ctx *c, uint32_t i) {
-  if (i >= c->inst.size()) {
-    c->fail(TGS_E_ARG, "instance %u out of range (%zu instances)", i, c->inst.size());
-    return nullptr;
-  }
-  return c->inst[i].get();
-}
 
 }  // namespace
 
@@ -747,6 +755,8 @@ int tgs_create(int device, tgs_ctx **out) {
   if (blk) c->w64Block = std::max(64u, std::min(1024u, uint32_t(std::atoi(blk)) / 64 * 64));
   const char *grp = std::getenv("TGS_W64_GROUPS");
   if (grp) c->w64Groups = uint32_t(std::atoi(grp));
+  const char *mode = std::getenv("TGS_W64_MODE");
+  if (mode) c->w64ModeForce = std::atoi(mode);
   *out = c.release();
   return TGS_OK;
 }
