@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--min-cycles", type=int, default=1_000)
     ap.add_argument("--max-cycles", type=int, default=50_000)
     ap.add_argument("--tune-cycles", type=int, default=100)
+    ap.add_argument("--slice", type=int, default=5_000, help="cycles per instance per batched launch")
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
@@ -119,7 +120,7 @@ def run_single(args):
                                "16 SRAM 4096x64b, 128 inputs, 100,000 cycles",
                    "cycles": cycles, "cycles_per_step": args.chunk, "kernel": "wave64", "grid": prof["grid"],
                    "block": prof["block"], "lds_bytes": prof["lds_bytes"],
-                   "program_resident_in_lds": bool(prof["resident"])},
+                   "program_source": ["global", "lds-staged", "lds-resident"][prof["resident"]]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cycle": bytes_per_cycle,
                      "b_prog": stats["b_prog"], "b_state": stats["b_state"], "launch_ms": launch_ms,
@@ -158,7 +159,7 @@ def run_batched(args):
     for part in (256, 512, 1024, 2048, 4096):
         d = gen.generate("lut200k", seed=args.seed, part_luts=part)
         designs[part] = d
-        for block in (256, 512):
+        for block in (256, 512, 1024):
             sim = rt.Simulator(0)
             assert sim.init(0, d.design, d.symbols, False, False) == 0
             try:
@@ -188,8 +189,18 @@ def run_batched(args):
     sim.set_instances(n)
     sim.set_stimulus(d.inputs, args.seed)
     sim.reset_profile()
+    # the run is issued in slices of --slice cycles per instance so that
+    # progress is visible; each slice is one persistent launch
+    left = cycles.astype(np.int64).copy()
+    launch_cycles = 0
     t0 = time.perf_counter()
-    sim.run_batch(cycles)
+    while left.max() > 0:
+        step = np.minimum(left, args.slice).astype(np.uint32)
+        sim.run_batch(step)
+        launch_cycles += int(step.max())
+        left -= step
+        log(f"batched: {int((cycles - left).sum())} of {int(cycles.sum())} instance-cycles done, "
+            f"{int((left > 0).sum())} instances live, {time.perf_counter() - t0:.1f} s")
     dt = time.perf_counter() - t0
     prof = sim.profile()
     total = int(cycles.sum())
@@ -203,7 +214,6 @@ def run_batched(args):
                        f"cycles uniform in [{args.min_cycles}, {args.max_cycles}]",
            "bytes_per_cycle_shared": stats["b_prog"], "bytes_per_cycle_per_instance": stats["b_state"]}
     # roofline: program bytes once per launch-cycle, state bytes per live instance-cycle
-    launch_cycles = int(cycles.max())
     algo = stats["b_prog"] * launch_cycles + stats["b_state"] * total
     achieved = algo / (prof["kernel_ms"] / 1e3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -37,6 +37,7 @@ extern "C" {
 #define TGS_KERNEL_PLAIN 0   /* reference execution model: block per partition, 32-lane micropart per wave */
 #define TGS_KERNEL_WAVE64 1  /* persistent wave64 kernel, program evaluated from LDS */
 #define TGS_KERNEL_BATCH 2   /* instance-per-lane batched kernel */
+#define TGS_KERNEL_STREAM 3  /* persistent kernel, program rows streamed into LDS by a loader wave */
 
 /* pools */
 #define TGS_POOL_REG 0
@@ -64,7 +65,7 @@ typedef struct tgs_profile {
   uint32_t grid, block;   /* launch shape of the last launch */
   uint32_t lds_bytes;     /* dynamic LDS of the last launch */
   uint32_t kernel;        /* kernel family of the last launch */
-  uint32_t resident;      /* wave64 program source: 0 global, 1 staged in LDS per visit, 2 LDS-resident */
+  uint32_t resident;      /* program source: 0 global, 1 staged in LDS per visit, 2 LDS-resident, 3 streamed */
   uint64_t prog_bytes;    /* device program image bytes */
   uint64_t barrier_ticks; /* wave64: summed ticks spent waiting at grid barriers (rank 0 block) */
   uint64_t total_ticks;   /* wave64: summed ticks of the cycle loop (rank 0 block) */

@@ -5,12 +5,12 @@ import pytest
 
 import edge_programs as E
 import oracle as O
-from toucangpurt_amd import KERNEL_PLAIN, KERNEL_WAVE64, POOL_EXCHANGE, POOL_MEM, POOL_REG, rt
+from toucangpurt_amd import KERNEL_PLAIN, KERNEL_STREAM, KERNEL_WAVE64, POOL_EXCHANGE, POOL_MEM, POOL_REG, rt
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("kernel", [KERNEL_PLAIN, KERNEL_WAVE64], ids=["plain", "wave64"])
+@pytest.mark.parametrize("kernel", [KERNEL_PLAIN, KERNEL_WAVE64, KERNEL_STREAM], ids=["plain", "wave64", "stream"])
 @pytest.mark.parametrize("case", E.all_cases(), ids=lambda c: c.name)
 def test_edge_case_on_gpu(gpu_available, case, kernel):
     print(f"case {case.name} kernel={kernel}", flush=True)

@@ -7,11 +7,11 @@ cycle for a short trace (eval), then after a longer free run
 import pytest
 
 from simharness import OracleRun, assert_pools_equal, gpu_pools, gpu_sim
-from toucangpurt_amd import KERNEL_PLAIN, KERNEL_WAVE64, gen
+from toucangpurt_amd import KERNEL_PLAIN, KERNEL_STREAM, KERNEL_WAVE64, gen
 
 pytestmark = pytest.mark.gpu
 
-KERNELS = [("plain", KERNEL_PLAIN), ("wave64", KERNEL_WAVE64)]
+KERNELS = [("plain", KERNEL_PLAIN), ("wave64", KERNEL_WAVE64), ("stream", KERNEL_STREAM)]
 
 DESIGNS = {
     "lut20k": dict(preset="lut20k", seed=3),

@@ -25,6 +25,7 @@ TGS_E_UNSUPPORTED = -10
 KERNEL_PLAIN = 0
 KERNEL_WAVE64 = 1
 KERNEL_BATCH = 2
+KERNEL_STREAM = 3
 
 POOL_REG = 0
 POOL_MEM = 1

@@ -90,13 +90,16 @@ __device__ __forceinline__ uint64_t memAddress(const uint8_t *vp, uint32_t addrV
 }
 
 // Grid-wide barrier over a monotonically increasing arrival counter.
-// The workgroup barrier first completes every wave's global stores into this
-// CU's L2; one wave then publishes them device-wide (agent-scope release,
+// Every wave first waits for its own outstanding global stores (`drain`; a
+// wave that has none, such as a pure loader, may skip it); after the
+// workgroup barrier one wave publishes them device-wide (agent-scope release,
 // an L2 write-back), arrives, spins on relaxed loads, and re-acquires once
 // (agent-scope acquire, an L1/L2 invalidate that every wave of the CU then
 // sees). Register, exchange and memory pools written before the barrier are
 // visible to every XCD after it. `target` = (barrier index + 1) * gridDim.x.
-__device__ __forceinline__ uint64_t gridBarrier(uint32_t *counter, uint32_t target, bool measure) {
+__device__ __forceinline__ uint64_t gridBarrier(uint32_t *counter, uint32_t target, bool measure, bool drain = true) {
+  // every wave completes its own global stores first (s_waitcnt vmcnt(0))
+  if (drain) __builtin_amdgcn_s_waitcnt(0x0F70);
   __syncthreads();
   uint64_t waited = 0;
   if (threadIdx.x == 0) {

@@ -81,4 +81,44 @@ enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3 
 // region, the partitions it evaluates (indices into the partition table).
 // sched[g * 2 + r] .. sched[g * 2 + r + 1] index `parts`.
 
+// ---------------------------------------------------------------------------
+// Stream image (streamed kernel). The program is a u32 word array of 64-word
+// rows. Each workgroup has nC compute waves and one loader wave and walks a
+// flat step stream for one cycle; every step ends with a workgroup barrier.
+// The loader wave copies the rows of upcoming steps from HBM into an LDS ring
+// (LDS-DMA loads) so the compute waves never wait on a global load for
+// program words.
+// Word 0..63 of the image is a zero row (the rows of "no task" entries).
+//
+// Step record: 1 + 2 * nC words: type, then per compute wave an entry
+// {x = word offset (a multiple of 64) of the task or partition block,
+//  y = kind << 4 | n << 8 | rows << 24}.
+enum StreamStep : uint32_t {
+  S_LEVEL = 0,   // exec level (or a slice of one): one task per compute wave
+  S_PART = 1,    // partition begin: L0 reads by the compute waves
+  S_PEND = 2,    // partition end: last level by the compute waves
+  S_REGION = 3   // grid barrier (end of a region)
+};
+constexpr uint32_t kKindNone = 15;  // entry kind of a wave with no task in the step
+constexpr uint32_t kRow = 64;
+// Task rows, lane l of a row belongs to lane l of the wave:
+//   T_LUT2   (n = nMid, rows 3 + nMid): row 0 top word {lut | op0 << 16},
+//            row 1 top word {op1 | op2 << 16}, row 2 last {result | shuffleId
+//            << 16} (0xFFFFFFFF = no op), row 3 + m middle level m (packed
+//            on-disk op, 0 = no op); lanes 0-31 micropart A, 32-63 B.
+//   T_VECREAD (n ops, 5 rows), T_VECOP (3 rows), T_MEMREAD (4 rows): row k
+//            lane i = word k of op i in the wave64 layout above.
+// Partition block: row 0 = header (word index below), then 2 * nJ rows of
+// nC * 64 words: L0 entry j of compute thread t has its source at row 2j and
+// its value-pool slot at row 2j + 1 (source = index | kind << 30, kind 0
+// register, 1 exchange, 2 stimulus input, 3 none), then the last-level lists.
+enum StreamHdr : uint32_t {
+  S_VP = 0, S_NJ, S_RW_REG, S_RW_DC, S_EW_EXG, S_EW_DC, S_NMW, S_NPR, S_NST,
+  S_OFF_LAST,       // word offset (from the block) of memwrite/print/stop lists
+  S_CONST_BASE,     // byte offset of the constant-vector pool in the const image
+  S_WORDS
+};
+constexpr uint32_t kL0None = 3u << 30;
+constexpr uint32_t kStreamRows = 6;   // rows staged per wave and step
+
 }  // namespace tgs

@@ -651,4 +651,257 @@ Wave64Schedule scheduleWave64(const Wave64Image &img, uint32_t groups) {
   return s;
 }
 
+
+// ---------------------------------------------------------------------------
+// Stream image
+
+namespace {
+
+struct StreamTask {
+  uint32_t off, y;  // entry words (type filled in per wave)
+  uint64_t cost;
+};
+
+// Append `rows` zeroed 64-word rows; returns the word offset of the first.
+uint32_t addRows(std::vector<uint32_t> &w, uint32_t rows) {
+  const uint32_t off = uint32_t(w.size());
+  w.resize(w.size() + size_t(rows) * kRow, 0);
+  return off;
+}
+
+// Longest-processing-time assignment of weighted items to `bins`; returns
+// the bin of each item and the largest bin load.
+uint64_t lpt(const std::vector<uint64_t> &cost, uint32_t bins, std::vector<uint32_t> &binOf) {
+  std::vector<uint32_t> ids(cost.size());
+  std::iota(ids.begin(), ids.end(), 0u);
+  std::stable_sort(ids.begin(), ids.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  using E = std::pair<uint64_t, uint32_t>;
+  std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
+  for (uint32_t b = 0; b < bins; ++b) pq.emplace(0, b);
+  binOf.assign(cost.size(), 0);
+  uint64_t mx = 0;
+  for (uint32_t id : ids) {
+    auto [l, b] = pq.top();
+    pq.pop();
+    binOf[id] = b;
+    l += cost[id];
+    mx = std::max(mx, l);
+    pq.emplace(l, b);
+  }
+  return mx;
+}
+
+}  // namespace
+
+StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32_t nWaves, uint32_t groups) {
+  StreamImage img;
+  img.nC = nWaves = std::max<uint32_t>(nWaves, 1);
+  img.groups = groups = std::max<uint32_t>(groups, 1);
+  img.n0 = d.regionPartitionIds[0].size();
+  img.nTotal = d.parts.size();
+  const uint32_t nT = nWaves * 64;
+  auto &W = img.words;
+  addRows(W, 1);  // zero row
+  std::vector<uint32_t> blockOff(d.parts.size()), blockRows(d.parts.size());
+  std::vector<uint64_t> partCost(d.parts.size());
+  // per partition, per level, per wave: tasks
+  std::vector<std::vector<std::vector<std::vector<StreamTask>>>> waveTasks(d.parts.size());
+  for (size_t pi = 0; pi < d.parts.size(); ++pi) {
+    const auto &p = d.parts[pi];
+    img.maxVp = std::max(img.maxVp, p.valuePoolSize);
+    auto &levels = waveTasks[pi];
+    levels.resize(p.exec_mParts.size());
+    uint64_t cost = 0;
+    for (size_t li = 0; li < p.exec_mParts.size(); ++li) {
+      std::vector<StreamTask> tasks;
+      std::vector<const CGMicroPartInfo *> luts;
+      std::vector<const CGMicroPartVecRead *> vr;
+      std::vector<const CGMicroPartVecArithOrLogic *> vo;
+      std::vector<const CGMicroPartMemRead *> mr;
+      for (const auto &m : p.exec_mParts[li]) {
+        if (m.isLUTPart) {
+          luts.push_back(&m);
+          continue;
+        }
+        int k = otherKind(m);
+        if (k == 1) for (const auto &o : m.vecRead) vr.push_back(&o);
+        if (k == 2) for (const auto &o : m.vecArithAndLogic) vo.push_back(&o);
+        if (k == 3) for (const auto &o : m.memRead) mr.push_back(&o);
+      }
+      std::stable_sort(luts.begin(), luts.end(), [](const CGMicroPartInfo *a, const CGMicroPartInfo *b) {
+        return a->middleLevels.size() > b->middleLevels.size();
+      });
+      for (size_t i = 0; i < luts.size();) {
+        const CGMicroPartInfo *A = luts[i];
+        const CGMicroPartInfo *B = nullptr;
+        if (i + 1 < luts.size() && luts[i + 1]->middleLevels.size() == A->middleLevels.size()) B = luts[i + 1];
+        i += B ? 2 : 1;
+        const uint32_t nMid = A->middleLevels.size(), rows = 3 + nMid;
+        const uint32_t off = addRows(W, rows);
+        std::fill(W.begin() + off + 2 * kRow, W.begin() + off + 3 * kRow, 0xFFFFFFFFu);
+        auto put = [&](const CGMicroPartInfo *m, uint32_t base) {
+          for (size_t k = 0; k < m->topLevel.size(); ++k) {
+            const auto &o = m->topLevel[k];
+            // a pass-through op ignores op0/op1: store immediates so no lane reads an unchecked slot
+            const uint32_t o0 = o.lutIndex ? o.op0 : 0u, o1 = o.lutIndex ? o.op1 : 0u;
+            W[off + base + k] = uint32_t(o.lutIndex) | o0 << 16;
+            W[off + kRow + base + k] = o1 | uint32_t(o.op2) << 16;
+          }
+          for (size_t k = 0; k < m->lastLevel.size(); ++k)
+            W[off + 2 * kRow + base + k] = uint32_t(m->lastLevel[k].result) | uint32_t(m->lastLevel[k].shuffleId) << 16;
+          for (uint32_t l = 0; l < nMid; ++l)
+            for (size_t k = 0; k < m->middleLevels[l].size(); ++k)
+              W[off + (3 + l) * kRow + base + k] = m->middleLevels[l][k].getPacked();
+        };
+        put(A, 0);
+        if (B) put(B, 32);
+        tasks.push_back({off, T_LUT2 << 4 | nMid << 8 | rows << 24, 4 + 2ull * nMid});
+      }
+      auto vecTasks = [&](auto &ops, uint32_t kind, uint32_t rows, auto fill, auto costOf) {
+        for (size_t i = 0; i < ops.size(); i += 64) {
+          const uint32_t n = std::min<size_t>(64, ops.size() - i);
+          const uint32_t off = addRows(W, rows);
+          uint64_t c = 0;
+          for (uint32_t k = 0; k < n; ++k) {
+            uint32_t v[5] = {0, 0, 0, 0, 0};
+            fill(*ops[i + k], v);
+            for (uint32_t r = 0; r < rows; ++r) W[off + r * kRow + k] = v[r];
+            c = std::max<uint64_t>(c, costOf(*ops[i + k]));
+          }
+          tasks.push_back({off, kind << 4 | n << 8 | rows << 24, c});
+        }
+      };
+      vecTasks(vr, T_VECREAD, 5,
+               [](const CGMicroPartVecRead &o, uint32_t *v) {
+                 v[0] = uint32_t(o.vecBase) | uint32_t(o.vecLength) << 16;
+                 v[1] = uint32_t(o.offset) | uint32_t(o.isConstVec ? 1 : 0) << 16;
+                 v[2] = uint32_t(o.index0) | uint32_t(o.index1) << 16;
+                 v[3] = uint32_t(o.index2) | uint32_t(o.index3) << 16;
+                 v[4] = uint32_t(o.outRangeValue) | uint32_t(o.result) << 16;
+               },
+               [](const CGMicroPartVecRead &) { return uint64_t(6); });
+      vecTasks(vo, T_VECOP, 3,
+               [](const CGMicroPartVecArithOrLogic &o, uint32_t *v) {
+                 v[0] = uint32_t(o.vec1Base) | uint32_t(o.vec2Base) << 16;
+                 v[1] = uint32_t(o.vecLength) | uint32_t(o.opName) << 16 | uint32_t(o.isV1V2Const) << 24;
+                 v[2] = o.result;
+               },
+               [](const CGMicroPartVecArithOrLogic &o) { return uint64_t(6 + o.vecLength); });
+      vecTasks(mr, T_MEMREAD, 4,
+               [](const CGMicroPartMemRead &o, uint32_t *v) {
+                 v[0] = uint32_t(o.memBase);
+                 v[1] = uint32_t(o.memBase >> 32);
+                 v[2] = uint32_t(o.en) | uint32_t(o.addrVec) << 16;
+                 v[3] = uint32_t(o.result) | uint32_t(o.hasMultipleWriter ? 1 : 0) << 16;
+               },
+               [](const CGMicroPartMemRead &) { return uint64_t(16); });
+      img.tasks += tasks.size();
+      std::vector<uint64_t> tc(tasks.size());
+      for (size_t t = 0; t < tasks.size(); ++t) tc[t] = tasks[t].cost;
+      std::vector<uint32_t> binOf;
+      cost += 2 + lpt(tc, nWaves, binOf);
+      levels[li].resize(nWaves);
+      for (size_t t = 0; t < tasks.size(); ++t) levels[li][binOf[t]].push_back(tasks[t]);
+    }
+    // partition block: header row, L0 rows, last-level lists
+    std::vector<uint32_t> src, slot;
+    for (const auto &o : p.ops_l0_regRead) {
+      const int32_t in = stim.lookup(o.reg);
+      src.push_back(in >= 0 ? uint32_t(in) | 2u << 30 : o.reg);
+      slot.push_back(o.result);
+    }
+    for (const auto &o : p.ops_l0_exchangeRead) {
+      src.push_back(o.exchange | 1u << 30);
+      slot.push_back(o.result);
+    }
+    const uint32_t nL0 = src.size(), nJ = (nL0 + nT - 1) / nT;
+    const uint32_t lastWords = 4 * p.ops_last_memWrite.size() + p.ops_last_print.size() + p.ops_last_stop.size();
+    const uint32_t l0Rows = (2 * nJ * nT + kRow - 1) / kRow, lastRows = (lastWords + kRow - 1) / kRow;
+    const uint32_t off = addRows(W, 1 + l0Rows + lastRows);
+    blockOff[pi] = off;
+    blockRows[pi] = std::min<uint32_t>(1 + 2 * nJ, 255);
+    for (uint32_t j = 0; j < nJ; ++j)
+      for (uint32_t t = 0; t < nT; ++t) {
+        const uint32_t k = j * nT + t;
+        W[off + kRow + (2 * j) * nT + t] = k < nL0 ? src[k] : kL0None;
+        W[off + kRow + (2 * j + 1) * nT + t] = k < nL0 ? slot[k] : 0u;
+      }
+    const auto &ew = p.op_last_exchangeWrite;
+    const auto &rw = p.op_last_regWrite;
+    const bool exg = ew.count != 0;
+    uint32_t *h = &W[off];
+    h[S_VP] = p.valuePoolSize;
+    h[S_NJ] = nJ;
+    h[S_RW_REG] = rw.reg;
+    h[S_RW_DC] = exg ? 0 : (uint32_t(rw.dat) | uint32_t(rw.count) << 16);
+    h[S_EW_EXG] = ew.exchange;
+    h[S_EW_DC] = uint32_t(ew.dat) | uint32_t(ew.count) << 16;
+    h[S_NMW] = exg ? 0 : p.ops_last_memWrite.size();
+    h[S_NPR] = exg ? 0 : p.ops_last_print.size();
+    h[S_NST] = exg ? 0 : p.ops_last_stop.size();
+    h[S_OFF_LAST] = (1 + l0Rows) * kRow;
+    h[S_CONST_BASE] = img.consts.size();
+    img.consts.insert(img.consts.end(), p.constVecPool.begin(), p.constVecPool.end());
+    uint32_t q = off + h[S_OFF_LAST];
+    if (!exg) {
+      for (const auto &o : p.ops_last_memWrite) {
+        W[q++] = uint32_t(o.memBase);
+        W[q++] = uint32_t(o.memBase >> 32);
+        W[q++] = uint32_t(o.addrVec) | uint32_t(o.dat) << 16;
+        W[q++] = uint32_t(o.en) | uint32_t(o.hasMultipleWriter ? 1 : 0) << 16;
+      }
+      for (const auto &o : p.ops_last_print) W[q++] = uint32_t(o.en) | uint32_t(o.msg) << 16;
+      for (const auto &o : p.ops_last_stop) W[q++] = o.en;
+    }
+    partCost[pi] = cost + 8 + 4 * nJ + (p.ops_last_memWrite.empty() ? 0 : 8);
+  }
+  if (img.consts.empty()) img.consts.push_back(0);
+  // partitions -> workgroups per region, then the per-wave entry streams
+  std::vector<std::vector<uint32_t>> per(size_t(groups) * 2);
+  for (int r = 0; r < 2; ++r) {
+    const auto &ids = d.regionPartitionIds[r];
+    std::vector<uint64_t> c(ids.size());
+    for (size_t k = 0; k < ids.size(); ++k) c[k] = partCost[ids[k]];
+    std::vector<uint32_t> binOf;
+    img.maxGroupCost += lpt(c, groups, binOf);
+    for (size_t k = 0; k < ids.size(); ++k) per[size_t(binOf[k]) * 2 + r].push_back(ids[k]);
+  }
+  const uint32_t rec = 1 + 2 * nWaves;
+  auto step = [&](uint32_t type, uint32_t x, uint32_t y) {
+    img.steps.push_back(type);
+    for (uint32_t w = 0; w < nWaves; ++w) {
+      img.steps.push_back(x);
+      img.steps.push_back(y);
+    }
+  };
+  const uint32_t none = kKindNone << 4 | 1u << 24;
+  for (uint32_t g = 0; g < groups; ++g) {
+    const uint32_t base = img.steps.size() / rec;
+    for (int r = 0; r < 2; ++r) {
+      for (uint32_t pi : per[size_t(g) * 2 + r]) {
+        step(S_PART, blockOff[pi], blockRows[pi] << 24);
+        for (const auto &lv : waveTasks[pi]) {
+          size_t nSub = 1;
+          for (const auto &mine : lv) nSub = std::max(nSub, mine.size());
+          for (size_t k = 0; k < nSub; ++k) {
+            img.steps.push_back(S_LEVEL);
+            for (uint32_t w = 0; w < nWaves; ++w) {
+              const bool has = k < lv[w].size();
+              img.steps.push_back(has ? lv[w][k].off : 0u);
+              img.steps.push_back(has ? lv[w][k].y : none);
+            }
+          }
+        }
+        step(S_PEND, blockOff[pi], 1u << 24);
+      }
+      step(S_REGION, 0, none);
+    }
+    img.stepBase.push_back(base);
+    img.stepCount.push_back(img.steps.size() / rec - base);
+    img.maxSteps = std::max(img.maxSteps, img.stepCount.back());
+  }
+  addRows(W, kStreamRows);  // margin past the last task
+  return img;
+}
+
 }  // namespace tgs

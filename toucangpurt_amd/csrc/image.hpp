@@ -61,4 +61,18 @@ struct Wave64Schedule {
 // Longest-processing-time assignment of partitions to `groups` workgroups per region.
 Wave64Schedule scheduleWave64(const Wave64Image &img, uint32_t groups);
 
+// Stream image: 64-word rows plus, per workgroup, the step stream of one
+// cycle (partitions assigned per region by longest processing time, the
+// tasks of every exec level assigned to the nC compute waves the same way).
+struct StreamImage {
+  std::vector<uint32_t> words;
+  std::vector<uint32_t> steps;                // step records, 1 + 2 * nC words each
+  std::vector<uint32_t> stepBase, stepCount;  // per group: first record, records per cycle
+  std::vector<uint8_t> consts;
+  uint32_t maxVp = 0, n0 = 0, nTotal = 0, groups = 0, nC = 0, maxSteps = 0;
+  uint64_t maxGroupCost = 0, tasks = 0;
+};
+StreamImage buildStreamImage(const toucanGPUSim::SimDesignInfo &d, const StimMap &stim, uint32_t nC,
+                             uint32_t groups);
+
 }  // namespace tgs

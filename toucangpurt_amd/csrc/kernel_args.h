@@ -27,6 +27,16 @@ struct W64Args {
   uint32_t barrierBase;      // barrier index offset (counter is zeroed per launch)
 };
 
+struct StreamArgs {
+  CommonArgs c;
+  const uint32_t *words;      // stream image rows
+  const uint32_t *steps;      // step records, 1 + 2 * NC words each
+  const uint32_t *stepBase;   // per workgroup: first record
+  const uint32_t *stepCount;  // per workgroup: records per cycle
+  uint32_t vpBytes;           // value-pool bytes in LDS (16-byte multiple)
+  uint32_t barrierBase;
+};
+
 // Batched kernel: pools are instance-minor (nibble k of slot s at k * instPad + s).
 struct BatchArgs {
   const uint8_t *lut, *consts;
@@ -69,9 +79,19 @@ hipError_t wave64MaxBlocksPerCU(int *blocks, int mode, uint32_t block, uint32_t 
 hipError_t wave64SetLds(uint32_t lds);
 uint32_t wave64LutBytes();
 
+// Streamed kernel with `nc` compute waves (2 or 4) plus one loader wave.
+bool streamSupportsWaves(uint32_t nc);
+uint32_t streamLdsBytes(uint32_t nc, uint32_t vpBytes, uint32_t maxSteps);  // dynamic part
+uint32_t streamStaticLdsBytes(uint32_t nc);
+hipError_t launchStream(const StreamArgs &args, uint32_t nc, uint32_t grid, uint32_t lds, hipStream_t stream);
+hipError_t streamMaxBlocksPerCU(int *blocks, uint32_t nc, uint32_t lds);
+hipError_t streamSetLds(uint32_t lds);
+
 hipError_t launchBatch(const BatchArgs &args, uint32_t grid, uint32_t block, uint32_t lds, hipStream_t stream);
 hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t block, uint32_t lds);
 hipError_t batchSetLds(uint32_t lds);
+hipError_t launchSlotColumn(uint8_t *pool, uint32_t instPad, uint32_t slot, uint64_t rows, uint8_t *dense, bool toPool,
+                            hipStream_t stream);
 hipError_t launchPermute(const uint8_t *in, uint8_t *out, uint64_t rows, uint32_t instPad, const uint32_t *src,
                          hipStream_t stream);
 hipError_t launchPermute64(const uint64_t *in, uint64_t *out, uint64_t rows, uint32_t instPad, const uint32_t *src,

@@ -21,6 +21,8 @@
 // the other half of a double buffer, so no extra grid barrier is needed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernel_args.h"
 
 namespace tgs {
@@ -427,6 +429,25 @@ __global__ void permuteSlots64(const uint64_t *in, uint64_t *out, uint64_t rows,
     const uint32_t s = uint32_t(i - r * instPad);
     out[i] = in[r * instPad + src[s]];
   }
+}
+
+// One instance's column of an instance-minor pool to or from a dense buffer
+// (host register and pool I/O of the batched layout).
+__global__ void slotColumn(uint8_t *pool, uint32_t instPad, uint32_t slot, uint64_t rows, uint8_t *dense, int toPool) {
+  for (uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < rows; r += uint64_t(gridDim.x) * blockDim.x) {
+    if (toPool)
+      pool[r * instPad + slot] = dense[r];
+    else
+      dense[r] = pool[r * instPad + slot];
+  }
+}
+
+hipError_t launchSlotColumn(uint8_t *pool, uint32_t instPad, uint32_t slot, uint64_t rows, uint8_t *dense, bool toPool,
+                            hipStream_t stream) {
+  if (rows == 0) return hipSuccess;
+  const uint32_t grid = uint32_t(std::min<uint64_t>((rows + 255) / 256, 1024));
+  slotColumn<<<dim3(grid), dim3(256), 0, stream>>>(pool, instPad, slot, rows, dense, toPool ? 1 : 0);
+  return hipGetLastError();
 }
 
 hipError_t launchPermute(const uint8_t *in, uint8_t *out, uint64_t rows, uint32_t instPad, const uint32_t *src,

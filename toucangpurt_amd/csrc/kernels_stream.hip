@@ -1,0 +1,399 @@
+// Streamed cycle kernel for gfx950.
+//
+// Semantics are those of the reference evaluator (src/SimEval.cu:304-761 per
+// micropartition, :765-864 per partition, :889-965 per cycle). Execution:
+//   * persistent workgroups, one per partition slot: the host assigns
+//     partitions per region (longest processing time) and the tasks of every
+//     exec level to the compute waves the same way, and writes for each
+//     workgroup a flat step stream of one cycle (exec levels, partition
+//     begin / end, region barriers), every step ending in a workgroup barrier;
+//   * each workgroup has NC compute waves and one loader wave. The loader
+//     streams the 64-word program rows of the step three steps after the one
+//     it is writing from HBM (or the Infinity Cache) into registers, and two
+//     steps before they are needed writes them into a two-slot LDS ring; the
+//     compute waves read the next step's rows from the ring while they
+//     evaluate the current step. No compute wave ever waits on a global load
+//     for program words, and each wave's memory counters see only its own
+//     simple traffic;
+//   * the LUT, the value pool and the stimulus draws live in LDS, program
+//     rows are never resident, so partitions of any size fit and several
+//     workgroups share a CU;
+//   * one atomic-counter grid barrier per region.
+#include <hip/hip_runtime.h>
+
+#include "kernel_args.h"
+
+namespace tgs {
+
+namespace {
+
+constexpr uint32_t kLutBytesS = ((kLutSize + 15) / 16) * 16;
+
+// Operand read without a branch: slots below 16 are immediates, but reading
+// them is in bounds (the value pool is at least 16 bytes), so the load is
+// unconditional and the immediate is selected afterwards.
+__device__ __forceinline__ uint32_t rdOpS(const uint8_t *vp, uint32_t s) {
+  const uint32_t v = vp[s];
+  return s < 16 ? s : v;
+}
+// LUT read without a branch: positions past the table read the zero padding
+// byte at kLutSize.
+__device__ __forceinline__ uint32_t lutS(const uint8_t *lut, uint32_t pos) {
+  return lut[pos < kLutSize ? pos : kLutSize];
+}
+
+// The rows and entry of one step for one compute wave.
+struct Pf {
+  uint32_t ex, ey;  // the entry (wave-uniform)
+  uint32_t r0, r1, r2, r3, r4, r5;
+};
+
+// Two 32-lane LUT micropartitions in one wave (SimEval.cu:319-448).
+__device__ __forceinline__ void lutPairS(const Pf &p, const uint32_t *__restrict__ W, uint8_t *vp,
+                                         const uint8_t *lut) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t hb = lane & 32u;
+  const uint32_t nMid = (p.ey >> 8) & 0xFFFFu;
+  const uint32_t x = p.r0, y = p.r1, wl = p.r2;
+  uint32_t val;
+  {
+    const uint32_t lt = x & 0xFFFFu, o0 = x >> 16, o1 = y & 0xFFFFu, o2 = y >> 16;
+    const uint32_t c = rdOpS(vp, o2);
+    const uint32_t a = rdOpS(vp, o0), b = rdOpS(vp, o1);
+    const uint32_t lv = lutS(lut, lutIndex(lt, a, b, c));
+    val = lt == 0 ? c : lv;
+  }
+  auto mid = [&](uint32_t w) {
+    const uint32_t o2 = w & 63u, o1 = (w >> 6) & 63u, o0 = (w >> 12) & 63u, lt = w >> 18;
+    const uint32_t s0 = __shfl(val, int((o0 & 31u) | hb));
+    const uint32_t s1 = __shfl(val, int((o1 & 31u) | hb));
+    const uint32_t s2 = __shfl(val, int((o2 & 31u) | hb));
+    const uint32_t a = o0 < 32 ? o0 : s0, b = o1 < 32 ? o1 : s1, c = o2 < 32 ? o2 : s2;
+    const uint32_t lv = lutS(lut, lutIndex(lt, a, b, c));
+    val = lt == 0 ? c : lv;
+  };
+  if (nMid > 0) mid(p.r3);
+  if (nMid > 1) mid(p.r4);
+  if (nMid > 2) mid(p.r5);
+  for (uint32_t m = 3; m < nMid; ++m) mid(W[p.ex + (3 + m) * kRow + lane]);
+  const uint32_t sid = (wl >> 16) & 0xFFu;
+  uint32_t r = __shfl(val, int((sid & 31u) | hb));
+  if (sid < 32) r = sid;
+  if (wl != 0xFFFFFFFFu) vp[wl & 0xFFFFu] = uint8_t(r);
+}
+
+// VecRead (SimEval.cu:449-481), one op per lane.
+__device__ __forceinline__ void vecReadS(const Pf &p, uint8_t *vp, const uint8_t *consts) {
+  const uint32_t lane = threadIdx.x & 63u;
+  if (lane >= ((p.ey >> 8) & 0xFFFFu)) return;
+  const uint32_t base = p.r0 & 0xFFFFu, len = p.r0 >> 16, offset = p.r1 & 0xFFFFu, isConst = p.r1 >> 16;
+  const uint32_t i0 = rdOpS(vp, p.r2 & 0xFFFFu), i1 = rdOpS(vp, p.r2 >> 16);
+  const uint32_t i2 = rdOpS(vp, p.r3 & 0xFFFFu), i3 = rdOpS(vp, p.r3 >> 16);
+  const uint32_t off = ((i0 << 12) | (i1 << 8) | (i2 << 4) | i3) + offset;
+  uint32_t v;
+  if (off < len) v = isConst ? consts[base + off] : vp[base + off];
+  else v = rdOpS(vp, p.r4 & 0xFFFFu);
+  vp[p.r4 >> 16] = uint8_t(v);
+}
+
+// VecArithOrLogic (SimEval.cu:482-548), one op per lane.
+__device__ __forceinline__ void vecOpS(const Pf &p, uint8_t *vp, const uint8_t *consts) {
+  const uint32_t lane = threadIdx.x & 63u;
+  if (lane >= ((p.ey >> 8) & 0xFFFFu)) return;
+  const uint32_t b1 = p.r0 & 0xFFFFu, b2 = p.r0 >> 16;
+  const uint32_t len = p.r1 & 0xFFFFu, name = (p.r1 >> 16) & 0xFFu, cf = p.r1 >> 24, res = p.r2;
+  unsigned __int128 v1 = 0, v2 = 0;
+  for (uint32_t k = 0; k < len; ++k) {
+    unsigned __int128 s1 = (cf & 2) ? consts[b1 + k] : vp[b1 + k];
+    unsigned __int128 s2 = (cf & 1) ? consts[b2 + k] : vp[b2 + k];
+    v1 |= s1 << (4 * k);
+    v2 |= s2 << (4 * k);
+  }
+  if (len > 0 && len < 32) {  // sign_extend_128 (SimEval.cu:222-228)
+    const uint32_t bit = 4 * len - 1;
+    const unsigned __int128 ext = ~((((unsigned __int128)1) << (bit + 1)) - 1);
+    if ((v1 >> bit) & 1) v1 |= ext;
+    if ((v2 >> bit) & 1) v2 |= ext;
+  }
+  if (name <= 2) {
+    unsigned __int128 r = name == 0 ? v1 + v2 : name == 1 ? v1 - v2 : v1 * v2;
+    for (uint32_t k = 0; k < len; ++k) {
+      vp[uint16_t(res + k)] = uint8_t(r & 15u);
+      r >>= 4;
+    }
+  } else {
+    const __int128 s1 = (__int128)v1, s2 = (__int128)v2;
+    vp[res] = uint8_t(name == 3 ? (s1 == s2) : name == 4 ? (s1 < s2) : name == 5 ? (s1 <= s2) : 0);
+  }
+}
+
+// MemRead (SimEval.cu:550-575), one op per lane.
+__device__ __forceinline__ void memReadS(const Pf &p, uint8_t *vp, const CommonArgs &c) {
+  const uint32_t lane = threadIdx.x & 63u;
+  if (lane >= ((p.ey >> 8) & 0xFFFFu)) return;
+  const uint64_t base = uint64_t(p.r0) | uint64_t(p.r1) << 32;
+  const uint32_t en = rdOpS(vp, p.r2 & 0xFFFFu);
+  if (en == 0) return;
+  const uint64_t at = memAddress(vp, p.r2 >> 16, p.r3 >> 16, base);
+  vp[p.r3 & 0xFFFFu] = at < c.memSize ? c.mem[at] : uint8_t(0);
+}
+
+// L0 source (register, exchange or stimulus input).
+__device__ __forceinline__ uint32_t l0Value(const CommonArgs &c, const uint64_t *draws, uint32_t src) {
+  const uint32_t kind = src >> 30, idx = src & 0x3FFFFFFFu;
+  if (kind == 0) return c.reg[idx];
+  if (kind == 1) return c.exg[idx];
+  if (kind == 2) return stimBit(draws, idx);
+  return 0;
+}
+
+// Partition begin: L0 reads (SimEval.cu:582-622) by the compute threads.
+__device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, const uint32_t *__restrict__ W,
+                                          uint8_t *vp, const uint64_t *draws, uint32_t nTc, uint32_t &constBase) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nJ = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(p.r0, S_NJ));
+  constBase = __builtin_amdgcn_readlane(p.r0, S_CONST_BASE);
+  if (nJ > 0) {
+    const bool a0 = (p.r1 >> 30) != 3, a1 = nJ > 1 && (p.r3 >> 30) != 3;
+    const uint32_t v0 = a0 ? l0Value(c, draws, p.r1) : 0u;
+    const uint32_t v1 = a1 ? l0Value(c, draws, p.r3) : 0u;
+    if (a0) vp[p.r2] = uint8_t(v0);
+    if (a1) vp[p.r4] = uint8_t(v1);
+    for (uint32_t j = 2; j < nJ; ++j) {
+      const uint32_t src = W[p.ex + kRow + (2 * j) * nTc + tid];
+      const uint32_t slot = W[p.ex + kRow + (2 * j + 1) * nTc + tid];
+      if ((src >> 30) != 3) vp[slot] = uint8_t(l0Value(c, draws, src));
+    }
+  }
+}
+
+// Partition end: last level (SimEval.cu:625-761) by the compute threads.
+__device__ __forceinline__ void partEnd(const Pf &p, const CommonArgs &c, const uint32_t *__restrict__ W,
+                                        const uint8_t *vp, uint64_t cycle, uint32_t nTc) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t ewdc = __builtin_amdgcn_readlane(p.r0, S_EW_DC);
+  if (ewdc >> 16) {
+    const uint32_t exg = __builtin_amdgcn_readlane(p.r0, S_EW_EXG), dat = ewdc & 0xFFFFu;
+    const uint32_t words = ((ewdc >> 16) + 3) / 4;
+    for (uint32_t k = tid; k < words; k += nTc)
+      reinterpret_cast<uint32_t *>(c.exg)[(exg >> 2) + k] = reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k];
+    return;
+  }
+  const uint32_t rwdc = __builtin_amdgcn_readlane(p.r0, S_RW_DC);
+  if (rwdc >> 16) {
+    const uint32_t reg = __builtin_amdgcn_readlane(p.r0, S_RW_REG), dat = rwdc & 0xFFFFu;
+    const uint32_t words = ((rwdc >> 16) + 3) / 4;
+    for (uint32_t k = tid; k < words; k += nTc)
+      reinterpret_cast<uint32_t *>(c.reg)[(reg >> 2) + k] = reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k];
+  }
+  const uint32_t nmw = __builtin_amdgcn_readlane(p.r0, S_NMW), npr = __builtin_amdgcn_readlane(p.r0, S_NPR);
+  const uint32_t nst = __builtin_amdgcn_readlane(p.r0, S_NST);
+  if (nmw | npr | nst) {
+    const uint32_t *L = W + p.ex + __builtin_amdgcn_readlane(p.r0, S_OFF_LAST);
+    for (uint32_t k = tid; k < nmw; k += nTc) {
+      const uint32_t *o = L + 4 * k;
+      const uint32_t en = rdOpS(vp, o[3] & 0xFFFFu);
+      if (en != 0) {
+        const uint32_t dat = rdOpS(vp, o[2] >> 16);
+        const uint64_t at = memAddress(vp, o[2] & 0xFFFFu, o[3] >> 16, uint64_t(o[0]) | uint64_t(o[1]) << 32);
+        if (at < c.memSize) c.mem[at] = uint8_t(dat);
+      }
+    }
+    L += 4 * nmw;
+    if (c.printEnable)
+      for (uint32_t k = tid; k < npr; k += nTc)
+        if (rdOpS(vp, L[k] & 0xFFFFu) != 0) logPrint(c, cycle, L[k] >> 16);
+    L += npr;
+    for (uint32_t k = tid; k < nst; k += nTc)
+      if (rdOpS(vp, L[k] & 0xFFFFu) != 0) raiseStop(c, cycle);
+  }
+}
+
+// Loader: global word index of row r of compute wave w in step record `rec`.
+__device__ __forceinline__ uint32_t rowAddr(const uint32_t *rec, uint32_t w, uint32_t r, uint32_t lane, uint32_t nTc) {
+  const uint32_t type = rec[0], ex = rec[1 + 2 * w], last = (rec[2 + 2 * w] >> 24) - 1;
+  const uint32_t rr = r < last ? r : last;
+  if (type == S_PART && rr > 0) return ex + kRow + (rr - 1) * nTc + w * 64 + lane;
+  return ex + rr * kRow + lane;
+}
+
+// s_waitcnt with vmcnt(n) only (expcnt and lgkmcnt left at their maxima).
+constexpr uint32_t vmcntOnly(uint32_t n) { return (n & 15u) | (7u << 4) | (15u << 8) | ((n >> 4) & 3u) << 14; }
+
+// Loader: start the LDS-DMA copies of the rows of step record `rec` into ring
+// slot `slot` (kStreamRows dword loads per compute wave, one row each).
+template <int NC>
+__device__ __forceinline__ void dmaStep(const uint32_t *rec, const uint32_t *W, uint32_t *slot, uint32_t lane) {
+#pragma unroll
+  for (int w = 0; w < NC; ++w)
+#pragma unroll
+    for (int r = 0; r < int(kStreamRows); ++r)
+      __builtin_amdgcn_global_load_lds(W + rowAddr(rec, w, r, lane, NC * 64), slot + (w * kStreamRows + r) * 64, 4, 0,
+                                       0);
+}
+
+// Loader barrier: no memory fence (a fence would make the compiler wait for
+// every outstanding copy); the explicit waits around it order the copies.
+__device__ __forceinline__ void loaderBarrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+}  // namespace
+
+// Ring of kSlots step slots; the loader copies step k + 5 while the compute
+// waves evaluate step k and read step k + 1.
+constexpr uint32_t kSlots = 5;
+
+template <int NC>
+__global__ __launch_bounds__((NC + 1) * 64) void streamCycles(StreamArgs a) {
+  extern __shared__ __align__(16) uint8_t sLds[];
+  constexpr uint32_t nTc = NC * 64;
+  constexpr uint32_t rec = 1 + 2 * NC;
+  constexpr uint32_t slotWords = NC * kStreamRows * 64;
+  constexpr uint32_t perStep = NC * kStreamRows;  // copies per step
+  __shared__ __align__(16) uint32_t ring[kSlots * slotWords];
+  uint8_t *lut = sLds;
+  uint8_t *vp = sLds + kLutBytesS;
+  uint64_t *draws = reinterpret_cast<uint64_t *>(vp + a.vpBytes);
+  uint32_t *steps = reinterpret_cast<uint32_t *>(draws + kMaxDraws);
+  const uint32_t g = blockIdx.x, tid = threadIdx.x, lane = tid & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
+  const uint32_t *__restrict__ W = a.words;
+  const CommonArgs &c = a.c;
+  const uint32_t nSteps = a.stepCount[g];
+  {
+    const uint32_t *src = a.steps + size_t(a.stepBase[g]) * rec;
+    for (uint32_t k = tid; k < nSteps * rec; k += blockDim.x) steps[k] = src[k];
+    for (uint32_t k = tid; k < kLutBytesS / 4; k += blockDim.x)
+      reinterpret_cast<uint32_t *>(lut)[k] = reinterpret_cast<const uint32_t *>(c.lut)[k];
+  }
+  __syncthreads();
+  auto recOf = [&](uint32_t j) { return steps + (j % nSteps) * rec; };
+  if (wave == NC) {
+    // loader: steps 0..4 in flight, 0 and 1 landed before the start barrier
+    for (uint32_t k = 0; k < 5; ++k) dmaStep<NC>(recOf(k), W, ring + (k % kSlots) * slotWords, lane);
+    __builtin_amdgcn_s_waitcnt(vmcntOnly(3 * perStep < 63 ? 3 * perStep : 2 * perStep));
+    loaderBarrier();
+    uint32_t j = 0, si = 0, cyc = 0;
+    for (;;) {
+      if (si == 0) {
+        if (cyc == c.nCycles) break;
+        loaderBarrier();  // cycle start (stimulus draws)
+      }
+      // steps j + 1 and j + 2 must have landed by the end of step j
+      __builtin_amdgcn_s_waitcnt(vmcntOnly(2 * perStep));
+      if (recOf(j)[0] == S_REGION) {
+        loaderBarrier();  // the two workgroup barriers of the grid barrier
+        loaderBarrier();
+      } else {
+        loaderBarrier();
+      }
+      // slot of step j + 5 held step j, which every compute wave has read
+      dmaStep<NC>(recOf(j + 5), W, ring + ((j + 5) % kSlots) * slotWords, lane);
+      ++j;
+      if (++si == nSteps) {
+        si = 0;
+        ++cyc;
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(vmcntOnly(0));
+    return;
+  }
+  // compute waves
+  uint64_t xs = *c.xs;
+  uint64_t barTicks = 0;
+  const bool prof = g == 0;
+  const uint64_t tStart = __builtin_amdgcn_s_memrealtime();
+  uint32_t bar = a.barrierBase;
+  __syncthreads();  // start barrier: steps 0 and 1 are in the ring
+  const uint32_t *mySlot = ring + wave * kStreamRows * 64 + lane;
+  auto readStep = [&](Pf &p, uint32_t j) {
+    const uint32_t *r = recOf(j);
+    const uint32_t *sl = mySlot + (j % kSlots) * slotWords;
+    p.ex = r[1 + 2 * wave];
+    p.ey = r[2 + 2 * wave] | r[0];  // the step type rides in the entry's unused low nibble
+    p.r0 = sl[0];
+    p.r1 = sl[64];
+    p.r2 = sl[128];
+    p.r3 = sl[192];
+    p.r4 = sl[256];
+    p.r5 = sl[320];
+  };
+  Pf cur, nxt;
+  readStep(cur, 0);
+  uint32_t j = 0, si = 0, cyc = 0, constBase = 0;
+  for (;;) {
+    if (si == 0) {
+      if (cyc == c.nCycles) break;
+      if (tid == 0) stimulusDraws(xs, c.nIn, draws);
+      __syncthreads();
+      if (g == 0)
+        for (uint32_t k = tid; k < c.nIn; k += nTc) c.reg[c.inIds[k]] = uint8_t(stimBit(draws, k));
+    }
+    readStep(nxt, j + 1);
+    const uint32_t type = cur.ey & 15u, kind = (cur.ey >> 4) & 15u;
+    if (type == S_LEVEL) {
+      if (kind == T_LUT2) lutPairS(cur, W, vp, lut);
+      else if (kind == T_VECREAD) vecReadS(cur, vp, c.consts + constBase);
+      else if (kind == T_VECOP) vecOpS(cur, vp, c.consts + constBase);
+      else if (kind == T_MEMREAD) memReadS(cur, vp, c);
+      __syncthreads();
+    } else if (type == S_PART) {
+      partBegin(cur, c, W, vp, draws, nTc, constBase);
+      __syncthreads();
+    } else if (type == S_PEND) {
+      partEnd(cur, c, W, vp, c.cycleBase + cyc, nTc);
+      __syncthreads();
+    } else {
+      barTicks += gridBarrier(&c.ctl[C_BARRIER], (++bar) * gridDim.x, prof);
+    }
+    cur = nxt;
+    ++j;
+    if (++si == nSteps) {
+      si = 0;
+      ++cyc;
+    }
+  }
+  if (g == 0 && tid == 0) {
+    *c.xs = xs;
+    *reinterpret_cast<uint64_t *>(&c.ctl[C_BARTICKS]) = barTicks;
+    *reinterpret_cast<uint64_t *>(&c.ctl[C_TOTTICKS]) = __builtin_amdgcn_s_memrealtime() - tStart;
+  }
+}
+
+namespace {
+const void *streamFn(uint32_t nc) {
+  return nc == 2 ? reinterpret_cast<const void *>(streamCycles<2>) : reinterpret_cast<const void *>(streamCycles<4>);
+}
+}  // namespace
+
+bool streamSupportsWaves(uint32_t nc) { return nc == 2 || nc == 4; }
+
+// Dynamic LDS (the ring is static): LUT, value pool, draws, step records.
+uint32_t streamLdsBytes(uint32_t nc, uint32_t vpBytes, uint32_t maxSteps) {
+  return kLutBytesS + vpBytes + kMaxDraws * 8 + maxSteps * (1 + 2 * nc) * 4;
+}
+
+uint32_t streamStaticLdsBytes(uint32_t nc) { return kSlots * nc * kStreamRows * 64 * 4; }
+
+hipError_t launchStream(const StreamArgs &args, uint32_t nc, uint32_t grid, uint32_t lds, hipStream_t stream) {
+  StreamArgs a = args;
+  void *params[] = {&a};
+  return hipLaunchCooperativeKernel(streamFn(nc), dim3(grid), dim3((nc + 1) * 64), params, lds, stream);
+}
+
+hipError_t streamMaxBlocksPerCU(int *blocks, uint32_t nc, uint32_t lds) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, streamFn(nc), (nc + 1) * 64, lds);
+}
+
+hipError_t streamSetLds(uint32_t lds) {
+  for (uint32_t nc : {2u, 4u}) {
+    hipError_t e = hipFuncSetAttribute(streamFn(nc), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace tgs

@@ -122,6 +122,10 @@ struct tgs_ctx {
   int w64Mode = PM_GLOBAL;
   int w64ModeForce = -1;
   uint32_t plainGrid = 0, plainBlock = 1024, plainLds = 0;
+  bool streamReady = false;
+  StreamImage simg;
+  DBuf sWords, sSteps, sStepBase, sStepCount, sConsts;
+  uint32_t streamNC = 4, streamGroups = 0, streamGrid = 0, streamLds = 0;
 
   tgs_profile prof{};
 
@@ -249,6 +253,48 @@ int buildW64(tgs_ctx *c) {
   return TGS_OK;
 }
 
+// Streamed kernel: one workgroup per partition slot (up to the co-resident
+// limit), NC compute waves plus a loader wave each.
+int buildStream(tgs_ctx *c) {
+  if (c->streamReady) return TGS_OK;
+  const uint32_t nc = c->streamNC;
+  const uint32_t maxRegion = std::max<uint32_t>(c->design.regionPartitionIds[0].size(),
+                                                c->design.parts.size() - c->design.regionPartitionIds[0].size());
+  const uint32_t cus = c->prop.multiProcessorCount;
+  const size_t limit = c->prop.sharedMemPerBlock;
+  uint32_t groups = c->streamGroups ? c->streamGroups : std::max<uint32_t>(std::min<uint32_t>(maxRegion, 4 * cus), 1);
+  uint32_t lds = 0;
+  for (int attempt = 0;; ++attempt) {
+    c->simg = buildStreamImage(c->design, c->stim, nc, groups);
+    const uint32_t vpBytes = up16(std::max<uint32_t>(c->simg.maxVp, 16));
+    lds = streamLdsBytes(nc, vpBytes, c->simg.maxSteps);
+    if (lds + streamStaticLdsBytes(nc) > limit)
+      return c->fail(TGS_E_LIMIT, "streamed kernel needs %u B of LDS (value pool %u, %u steps per cycle); device allows %zu B",
+                     lds + streamStaticLdsBytes(nc), c->simg.maxVp, c->simg.maxSteps, limit);
+    HIPCHK(c, streamSetLds(lds));
+    int perCU = 0;
+    HIPCHK(c, streamMaxBlocksPerCU(&perCU, nc, lds));
+    if (perCU < 1) return c->fail(TGS_E_LIMIT, "streamed kernel cannot be resident with %u B LDS", lds);
+    const uint64_t cap = uint64_t(perCU) * cus;
+    if (groups <= cap) break;
+    if (c->streamGroups || attempt > 4)
+      return c->fail(TGS_E_LIMIT, "%u workgroups exceed the co-resident limit %llu", groups, (unsigned long long)cap);
+    groups = uint32_t(cap);
+  }
+  const auto &im = c->simg;
+  int rc;
+  if ((rc = upload(c, c->sWords, im.words.data(), im.words.size() * 4))) return rc;
+  if ((rc = upload(c, c->sSteps, im.steps.data(), im.steps.size() * 4))) return rc;
+  if ((rc = upload(c, c->sStepBase, im.stepBase.data(), im.stepBase.size() * 4))) return rc;
+  if ((rc = upload(c, c->sStepCount, im.stepCount.data(), im.stepCount.size() * 4))) return rc;
+  if ((rc = upload(c, c->sConsts, im.consts.data(), im.consts.size()))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->streamGrid = groups;
+  c->streamLds = lds;
+  c->streamReady = true;
+  return TGS_OK;
+}
+
 CommonArgs commonArgs(tgs_ctx *c, Instance &in, uint32_t cycles) {
   CommonArgs a{};
   a.lut = c->lut.as<uint8_t>();
@@ -303,6 +349,30 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles) {
       cycles -= n;
       in.cycles += n;
       c->prof.cycles += n;
+    } else if (c->kernel == TGS_KERNEL_STREAM) {
+      if ((rc = buildStream(c))) return rc;
+      uint32_t n = std::min(cycles, maxCyclesPerLaunch(c->streamGrid));
+      StreamArgs a{};
+      a.c = commonArgs(c, in, n);
+      a.c.consts = c->sConsts.as<uint8_t>();
+      a.words = c->sWords.as<uint32_t>();
+      a.steps = c->sSteps.as<uint32_t>();
+      a.stepBase = c->sStepBase.as<uint32_t>();
+      a.stepCount = c->sStepCount.as<uint32_t>();
+      a.vpBytes = up16(std::max<uint32_t>(c->simg.maxVp, 16));
+      a.barrierBase = 0;
+      HIPCHK(c, hipMemsetAsync(in.ctl.as<uint32_t>() + C_BARRIER, 0, 4, c->stream));
+      HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+      HIPCHK(c, launchStream(a, c->streamNC, c->streamGrid, c->streamLds, c->stream));
+      HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+      c->prof.grid = c->streamGrid;
+      c->prof.block = (c->streamNC + 1) * 64;
+      c->prof.lds_bytes = c->streamLds + streamStaticLdsBytes(c->streamNC);
+      c->prof.resident = 3;
+      c->prof.prog_bytes = c->simg.words.size() * 4;
+      cycles -= n;
+      in.cycles += n;
+      c->prof.cycles += n;
     } else {
       if ((rc = buildW64(c))) return rc;
       uint32_t n = std::min(cycles, maxCyclesPerLaunch(c->sched.groups));
@@ -336,7 +406,7 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles) {
     c->prof.kernel_ms += ms;
     c->prof.launches += 1;
     c->prof.kernel = c->kernel;
-    if (c->kernel == TGS_KERNEL_WAVE64) {
+    if (c->kernel == TGS_KERNEL_WAVE64 || c->kernel == TGS_KERNEL_STREAM) {
       uint64_t t[2];
       HIPCHK(c, hipMemcpy(t, in.ctl.as<uint32_t>() + C_BARTICKS, 16, hipMemcpyDeviceToHost));
       c->prof.barrier_ticks += t[0];
@@ -784,7 +854,7 @@ int tgs_load(tgs_ctx *c, const uint8_t *design, size_t designSize, const uint8_t
   int rc = setDevice(c);
   if (rc) return rc;
   c->loaded = false;
-  c->plainReady = c->w64Ready = false;
+  c->plainReady = c->w64Ready = c->streamReady = false;
   c->inst.clear();
   c->design = SimDesignInfo();
   c->symbols = SimDebugInfo();
@@ -834,7 +904,8 @@ int tgs_load_files(tgs_ctx *c, const char *designPath, const char *symbolPath, i
 
 int tgs_set_kernel(tgs_ctx *c, int kernel) {
   if (!c) return TGS_E_ARG;
-  if (kernel != TGS_KERNEL_PLAIN && kernel != TGS_KERNEL_WAVE64 && kernel != TGS_KERNEL_BATCH)
+  if (kernel != TGS_KERNEL_PLAIN && kernel != TGS_KERNEL_WAVE64 && kernel != TGS_KERNEL_BATCH &&
+      kernel != TGS_KERNEL_STREAM)
     return c->fail(TGS_E_ARG, "unknown kernel %d", kernel);
   if (c->loaded) {
     int rc = setDevice(c);
@@ -862,6 +933,15 @@ int tgs_set_launch(tgs_ctx *c, int kernel, uint32_t block, uint32_t groups) {
       if (block) c->batchBlock = block;
       c->batchGridOverride = groups;
       c->batchReady = false;
+      break;
+    case TGS_KERNEL_STREAM:
+      if (block) {
+        if (!streamSupportsWaves(block / 64 - 1))
+          return c->fail(TGS_E_ARG, "streamed kernel block must be 192 or 320 (2 or 4 compute waves + loader)");
+        c->streamNC = block / 64 - 1;
+      }
+      c->streamGroups = groups;
+      c->streamReady = false;
       break;
     default:
       return c->fail(TGS_E_ARG, "unknown kernel %d", kernel);
@@ -894,7 +974,7 @@ int tgs_set_stimulus(tgs_ctx *c, const uint32_t *ids, uint32_t n, uint64_t seed)
   if ((rc = buildStimMap(c->design, ids, n, &m, &e))) return c->fail(rc, "%s", e.c_str());
   c->stim = std::move(m);
   c->stimSeed = seed;
-  c->plainReady = c->w64Ready = c->batchReady = false;
+  c->plainReady = c->w64Ready = c->batchReady = c->streamReady = false;
   if ((rc = upload(c, c->inIds, c->stim.ids.data(), c->stim.ids.size() * 4))) return rc;
   programStats(c->design, n, &c->stats);
   // every instance restarts its generator from splitmix64(seed + k)
@@ -976,11 +1056,17 @@ static int poolIO(tgs_ctx *c, uint32_t instance, int pool, uint64_t off, uint64_
   auto &bs = c->bs;
   DBuf &b = pool == TGS_POOL_REG ? bs.reg : pool == TGS_POOL_MEM ? bs.mem : bs.exg;
   const size_t P = bs.instPad;
-  uint8_t *p = b.as<uint8_t>() + off * P + bs.slotOf[instance];
-  if (write)
-    HIPCHK(c, hipMemcpy2D(p, P, host, 1, 1, n, hipMemcpyHostToDevice));
-  else
-    HIPCHK(c, hipMemcpy2D(host, 1, p, P, 1, n, hipMemcpyDeviceToHost));
+  // gather / scatter the instance's column through the dense staging buffer
+  uint8_t *p = b.as<uint8_t>() + off * P;
+  uint8_t *dense = bs.scratch.as<uint8_t>();
+  if (write) {
+    HIPCHK(c, hipMemcpyAsync(dense, host, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launchSlotColumn(p, uint32_t(P), bs.slotOf[instance], n, dense, true, c->stream));
+  } else {
+    HIPCHK(c, launchSlotColumn(p, uint32_t(P), bs.slotOf[instance], n, dense, false, c->stream));
+    HIPCHK(c, hipMemcpyAsync(host, dense, n, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return TGS_OK;
 }
 

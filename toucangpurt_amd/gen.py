@@ -27,7 +27,7 @@ PRESETS = {
     "lut10m": dict(n_luts=10_000_000, depth=128, n_regs=500_000, n_inputs=256, n_srams=64,
                    sram_depth=8192, sram_width=64),
     "lut200k": dict(n_luts=200_000, depth=48, n_regs=4_096, n_inputs=64, n_vecops=2_000,
-                    n_srams=8, sram_depth=1024, sram_width=32, word_part_ops=32),
+                    n_srams=8, sram_depth=1024, sram_width=32, word_part_ops=16),
 }
 
 
