@@ -108,17 +108,20 @@ constexpr uint32_t kRow = 64;
 //            on-disk op, 0 = no op); lanes 0-31 micropart A, 32-63 B.
 //   T_VECREAD (n ops, 5 rows), T_VECOP (3 rows), T_MEMREAD (4 rows): row k
 //            lane i = word k of op i in the wave64 layout above.
-// Partition block: row 0 = header (word index below), then 2 * nJ rows of
-// nC * 64 words: L0 entry j of compute thread t has its source at row 2j and
-// its value-pool slot at row 2j + 1 (source = index | kind << 30, kind 0
-// register, 1 exchange, 2 stimulus input, 3 none), then the last-level lists.
+// Partition block: one section of 1 + 2 * nJ rows per compute wave (row 0 =
+// the header, word index below; then for L0 entry j of the wave's lane l,
+// thread t = wave * 64 + l handling entry j * nC * 64 + t, its source at row
+// 1 + 2j and its value-pool slot at row 2 + 2j; source = index | kind << 30,
+// kind 0 register, 1 exchange, 2 stimulus input, 3 none), then the
+// last-level lists. A partition-begin entry points at the wave's section, so
+// for every entry the rows a wave needs are contiguous from x.
 enum StreamHdr : uint32_t {
   S_VP = 0, S_NJ, S_RW_REG, S_RW_DC, S_EW_EXG, S_EW_DC, S_NMW, S_NPR, S_NST,
-  S_OFF_LAST,       // word offset (from the block) of memwrite/print/stop lists
+  S_OFF_LAST,       // word offset (from the first section) of memwrite/print/stop lists
   S_CONST_BASE,     // byte offset of the constant-vector pool in the const image
   S_WORDS
 };
 constexpr uint32_t kL0None = 3u << 30;
-constexpr uint32_t kStreamRows = 6;   // rows staged per wave and step
+constexpr uint32_t kStreamRows = 8;   // rows staged per wave and step (the loader copies rows 0..7 from x)
 
 }  // namespace tgs

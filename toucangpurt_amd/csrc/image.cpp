@@ -816,20 +816,21 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
     }
     const uint32_t nL0 = src.size(), nJ = (nL0 + nT - 1) / nT;
     const uint32_t lastWords = 4 * p.ops_last_memWrite.size() + p.ops_last_print.size() + p.ops_last_stop.size();
-    const uint32_t l0Rows = (2 * nJ * nT + kRow - 1) / kRow, lastRows = (lastWords + kRow - 1) / kRow;
-    const uint32_t off = addRows(W, 1 + l0Rows + lastRows);
+    const uint32_t section = 1 + 2 * nJ, lastRows = (lastWords + kRow - 1) / kRow;
+    const uint32_t off = addRows(W, nWaves * section + lastRows);
     blockOff[pi] = off;
-    blockRows[pi] = std::min<uint32_t>(1 + 2 * nJ, 255);
-    for (uint32_t j = 0; j < nJ; ++j)
-      for (uint32_t t = 0; t < nT; ++t) {
-        const uint32_t k = j * nT + t;
-        W[off + kRow + (2 * j) * nT + t] = k < nL0 ? src[k] : kL0None;
-        W[off + kRow + (2 * j + 1) * nT + t] = k < nL0 ? slot[k] : 0u;
-      }
+    blockRows[pi] = section;
+    for (uint32_t w = 0; w < nWaves; ++w)
+      for (uint32_t j = 0; j < nJ; ++j)
+        for (uint32_t l = 0; l < 64; ++l) {
+          const uint32_t k = j * nT + w * 64 + l;
+          W[off + (w * section + 1 + 2 * j) * kRow + l] = k < nL0 ? src[k] : kL0None;
+          W[off + (w * section + 2 + 2 * j) * kRow + l] = k < nL0 ? slot[k] : 0u;
+        }
     const auto &ew = p.op_last_exchangeWrite;
     const auto &rw = p.op_last_regWrite;
     const bool exg = ew.count != 0;
-    uint32_t *h = &W[off];
+    uint32_t h[kRow] = {};
     h[S_VP] = p.valuePoolSize;
     h[S_NJ] = nJ;
     h[S_RW_REG] = rw.reg;
@@ -839,8 +840,9 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
     h[S_NMW] = exg ? 0 : p.ops_last_memWrite.size();
     h[S_NPR] = exg ? 0 : p.ops_last_print.size();
     h[S_NST] = exg ? 0 : p.ops_last_stop.size();
-    h[S_OFF_LAST] = (1 + l0Rows) * kRow;
+    h[S_OFF_LAST] = nWaves * section * kRow;
     h[S_CONST_BASE] = img.consts.size();
+    for (uint32_t w = 0; w < nWaves; ++w) std::copy(h, h + kRow, W.begin() + off + w * section * kRow);
     img.consts.insert(img.consts.end(), p.constVecPool.begin(), p.constVecPool.end());
     uint32_t q = off + h[S_OFF_LAST];
     if (!exg) {
@@ -867,6 +869,7 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
     for (size_t k = 0; k < ids.size(); ++k) per[size_t(binOf[k]) * 2 + r].push_back(ids[k]);
   }
   const uint32_t rec = 1 + 2 * nWaves;
+  // partition end / region steps: the same entry for every wave
   auto step = [&](uint32_t type, uint32_t x, uint32_t y) {
     img.steps.push_back(type);
     for (uint32_t w = 0; w < nWaves; ++w) {
@@ -879,7 +882,11 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
     const uint32_t base = img.steps.size() / rec;
     for (int r = 0; r < 2; ++r) {
       for (uint32_t pi : per[size_t(g) * 2 + r]) {
-        step(S_PART, blockOff[pi], blockRows[pi] << 24);
+        img.steps.push_back(S_PART);
+        for (uint32_t w = 0; w < nWaves; ++w) {
+          img.steps.push_back(blockOff[pi] + w * blockRows[pi] * kRow);
+          img.steps.push_back(std::min<uint32_t>(blockRows[pi], 255) << 24);
+        }
         for (const auto &lv : waveTasks[pi]) {
           size_t nSub = 1;
           for (const auto &mine : lv) nSub = std::max(nSub, mine.size());

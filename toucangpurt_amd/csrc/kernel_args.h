@@ -79,8 +79,9 @@ hipError_t wave64MaxBlocksPerCU(int *blocks, int mode, uint32_t block, uint32_t 
 hipError_t wave64SetLds(uint32_t lds);
 uint32_t wave64LutBytes();
 
-// Streamed kernel with `nc` compute waves (2 or 4) plus one loader wave.
+// Streamed kernel with `nc` compute waves (2, 4 or 8) plus three loader waves.
 bool streamSupportsWaves(uint32_t nc);
+uint32_t streamBlock(uint32_t nc);
 uint32_t streamLdsBytes(uint32_t nc, uint32_t vpBytes, uint32_t maxSteps);  // dynamic part
 uint32_t streamStaticLdsBytes(uint32_t nc);
 hipError_t launchStream(const StreamArgs &args, uint32_t nc, uint32_t grid, uint32_t lds, hipStream_t stream);

@@ -7,14 +7,15 @@
 //     exec level to the compute waves the same way, and writes for each
 //     workgroup a flat step stream of one cycle (exec levels, partition
 //     begin / end, region barriers), every step ending in a workgroup barrier;
-//   * each workgroup has NC compute waves and one loader wave. The loader
-//     streams the 64-word program rows of the step three steps after the one
-//     it is writing from HBM (or the Infinity Cache) into registers, and two
-//     steps before they are needed writes them into a two-slot LDS ring; the
+//   * each workgroup has NC compute waves and three loader waves. Loader k
+//     owns every third step: it loads that step's 64-word program rows from
+//     HBM (or the Infinity Cache) into registers three steps ahead (16 bytes
+//     per lane, a wave's six rows are contiguous) and writes them into a
+//     two-slot LDS ring two steps before they are evaluated; the
 //     compute waves read the next step's rows from the ring while they
 //     evaluate the current step. No compute wave ever waits on a global load
-//     for program words, and each wave's memory counters see only its own
-//     simple traffic;
+//     for program words, and a loader only ever waits for its own single set
+//     of loads;
 //   * the LUT, the value pool and the stimulus draws live in LDS, program
 //     rows are never resident, so partitions of any size fit and several
 //     workgroups share a CU;
@@ -153,6 +154,7 @@ __device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, cons
   const uint32_t tid = threadIdx.x;
   const uint32_t nJ = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(p.r0, S_NJ));
   constBase = __builtin_amdgcn_readlane(p.r0, S_CONST_BASE);
+  const uint32_t lane = tid & 63u;
   if (nJ > 0) {
     const bool a0 = (p.r1 >> 30) != 3, a1 = nJ > 1 && (p.r3 >> 30) != 3;
     const uint32_t v0 = a0 ? l0Value(c, draws, p.r1) : 0u;
@@ -160,8 +162,8 @@ __device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, cons
     if (a0) vp[p.r2] = uint8_t(v0);
     if (a1) vp[p.r4] = uint8_t(v1);
     for (uint32_t j = 2; j < nJ; ++j) {
-      const uint32_t src = W[p.ex + kRow + (2 * j) * nTc + tid];
-      const uint32_t slot = W[p.ex + kRow + (2 * j + 1) * nTc + tid];
+      const uint32_t src = W[p.ex + (1 + 2 * j) * kRow + lane];
+      const uint32_t slot = W[p.ex + (2 + 2 * j) * kRow + lane];
       if ((src >> 30) != 3) vp[slot] = uint8_t(l0Value(c, draws, src));
     }
   }
@@ -209,50 +211,62 @@ __device__ __forceinline__ void partEnd(const Pf &p, const CommonArgs &c, const 
   }
 }
 
-// Loader: global word index of row r of compute wave w in step record `rec`.
-__device__ __forceinline__ uint32_t rowAddr(const uint32_t *rec, uint32_t w, uint32_t r, uint32_t lane, uint32_t nTc) {
-  const uint32_t type = rec[0], ex = rec[1 + 2 * w], last = (rec[2 + 2 * w] >> 24) - 1;
-  const uint32_t rr = r < last ? r : last;
-  if (type == S_PART && rr > 0) return ex + kRow + (rr - 1) * nTc + w * 64 + lane;
-  return ex + rr * kRow + lane;
-}
-
-// s_waitcnt with vmcnt(n) only (expcnt and lgkmcnt left at their maxima).
-constexpr uint32_t vmcntOnly(uint32_t n) { return (n & 15u) | (7u << 4) | (15u << 8) | ((n >> 4) & 3u) << 14; }
-
-// Loader: start the LDS-DMA copies of the rows of step record `rec` into ring
-// slot `slot` (kStreamRows dword loads per compute wave, one row each).
+// Rows 0..7 of each compute wave: rows 0-3 and rows 4-7, one 16-byte load
+// per lane each (a native vector type: HIP's uint4 wrapper keeps the set out
+// of registers).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 template <int NC>
-__device__ __forceinline__ void dmaStep(const uint32_t *rec, const uint32_t *W, uint32_t *slot, uint32_t lane) {
+struct RowSet {
+  u32x4 lo[NC], hi[NC];
+};
+
+// Loader: load the rows of step record `rec` for every compute wave. The
+// record is read with one LDS load (lane i holds word i) and broadcast with
+// readlane; a wave's rows are contiguous from its entry's x, so each load is
+// a uniform base plus the lane offset.
+template <int NC>
+__device__ __forceinline__ void loadRows(RowSet<NC> &s, const uint32_t *rec, const uint32_t *__restrict__ W,
+                                         uint32_t lane) {
+  constexpr uint32_t words = 1 + 2 * NC;
+  const uint32_t rv = rec[lane < words ? lane : 0];
 #pragma unroll
-  for (int w = 0; w < NC; ++w)
-#pragma unroll
-    for (int r = 0; r < int(kStreamRows); ++r)
-      __builtin_amdgcn_global_load_lds(W + rowAddr(rec, w, r, lane, NC * 64), slot + (w * kStreamRows + r) * 64, 4, 0,
-                                       0);
+  for (int w = 0; w < NC; ++w) {
+    const u32x4 *rows = reinterpret_cast<const u32x4 *>(W + __builtin_amdgcn_readlane(rv, 1 + 2 * w));
+    s.lo[w] = rows[lane];
+    s.hi[w] = rows[64 + lane];
+  }
 }
 
-// Loader barrier: no memory fence (a fence would make the compiler wait for
-// every outstanding copy); the explicit waits around it order the copies.
-__device__ __forceinline__ void loaderBarrier() {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_barrier();
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+template <int NC>
+__device__ __forceinline__ void storeRows(const RowSet<NC> &s, uint32_t *slot, uint32_t lane) {
+#pragma unroll
+  for (int w = 0; w < NC; ++w) {
+    u32x4 *rows = reinterpret_cast<u32x4 *>(slot + w * kStreamRows * 64);
+    rows[lane] = s.lo[w];
+    rows[64 + lane] = s.hi[w];
+  }
 }
 
 }  // namespace
 
-// Ring of kSlots step slots; the loader copies step k + 5 while the compute
-// waves evaluate step k and read step k + 1.
-constexpr uint32_t kSlots = 5;
+// Two ring slots: during step i the compute waves read step i + 1 while a
+// loader wave writes step i + 2.
+constexpr uint32_t kSlots = 2;
+// Loader waves: loader k owns the steps i with i % kLoaders == k; it writes
+// step i + 2 during step i and then loads step i + 2 + kLoaders, so each of
+// its loads has kLoaders steps to arrive and only one load set is ever in
+// flight per wave (the wait before the stores is a plain wait for all).
+constexpr uint32_t kLoaders = 3;
 
 template <int NC>
-__global__ __launch_bounds__((NC + 1) * 64) void streamCycles(StreamArgs a) {
+// Two (NC = 8) or three workgroups per CU: the register budget that keeps
+// every partition slot of a 1M-LUT4 design co-resident.
+template <int NC>
+__global__ __launch_bounds__((NC + kLoaders) * 64, NC == 8 ? 2 : 3) void streamCycles(StreamArgs a) {
   extern __shared__ __align__(16) uint8_t sLds[];
   constexpr uint32_t nTc = NC * 64;
   constexpr uint32_t rec = 1 + 2 * NC;
   constexpr uint32_t slotWords = NC * kStreamRows * 64;
-  constexpr uint32_t perStep = NC * kStreamRows;  // copies per step
   __shared__ __align__(16) uint32_t ring[kSlots * slotWords];
   uint8_t *lut = sLds;
   uint8_t *vp = sLds + kLutBytesS;
@@ -271,34 +285,40 @@ __global__ __launch_bounds__((NC + 1) * 64) void streamCycles(StreamArgs a) {
   }
   __syncthreads();
   auto recOf = [&](uint32_t j) { return steps + (j % nSteps) * rec; };
-  if (wave == NC) {
-    // loader: steps 0..4 in flight, 0 and 1 landed before the start barrier
-    for (uint32_t k = 0; k < 5; ++k) dmaStep<NC>(recOf(k), W, ring + (k % kSlots) * slotWords, lane);
-    __builtin_amdgcn_s_waitcnt(vmcntOnly(3 * perStep < 63 ? 3 * perStep : 2 * perStep));
-    loaderBarrier();
+  if (wave >= NC) {
+    const uint32_t k = wave - NC;
+    RowSet<NC> rs;
+    // steps 0 and 1 go straight into the ring; step 2 + k is this loader's first load
+    if (k < kSlots) {
+      loadRows<NC>(rs, recOf(k), W, lane);
+      storeRows<NC>(rs, ring + k * slotWords, lane);
+    }
+    loadRows<NC>(rs, recOf(2 + k), W, lane);
+    __syncthreads();  // start barrier
     uint32_t j = 0, si = 0, cyc = 0;
     for (;;) {
       if (si == 0) {
         if (cyc == c.nCycles) break;
-        loaderBarrier();  // cycle start (stimulus draws)
+        __syncthreads();  // cycle start (stimulus draws)
       }
-      // steps j + 1 and j + 2 must have landed by the end of step j
-      __builtin_amdgcn_s_waitcnt(vmcntOnly(2 * perStep));
+      if (j % kLoaders == k) {
+        storeRows<NC>(rs, ring + ((j + 2) % kSlots) * slotWords, lane);
+        // the set's registers are reused by the next loads: keep the stores first
+        __builtin_amdgcn_sched_barrier(0);
+        loadRows<NC>(rs, recOf(j + 2 + kLoaders), W, lane);
+      }
       if (recOf(j)[0] == S_REGION) {
-        loaderBarrier();  // the two workgroup barriers of the grid barrier
-        loaderBarrier();
+        __syncthreads();  // the two workgroup barriers of the grid barrier
+        __syncthreads();
       } else {
-        loaderBarrier();
+        __syncthreads();
       }
-      // slot of step j + 5 held step j, which every compute wave has read
-      dmaStep<NC>(recOf(j + 5), W, ring + ((j + 5) % kSlots) * slotWords, lane);
       ++j;
       if (++si == nSteps) {
         si = 0;
         ++cyc;
       }
     }
-    __builtin_amdgcn_s_waitcnt(vmcntOnly(0));
     return;
   }
   // compute waves
@@ -365,11 +385,13 @@ __global__ __launch_bounds__((NC + 1) * 64) void streamCycles(StreamArgs a) {
 
 namespace {
 const void *streamFn(uint32_t nc) {
-  return nc == 2 ? reinterpret_cast<const void *>(streamCycles<2>) : reinterpret_cast<const void *>(streamCycles<4>);
+  return nc == 2 ? reinterpret_cast<const void *>(streamCycles<2>)
+         : nc == 8 ? reinterpret_cast<const void *>(streamCycles<8>)
+                   : reinterpret_cast<const void *>(streamCycles<4>);
 }
 }  // namespace
 
-bool streamSupportsWaves(uint32_t nc) { return nc == 2 || nc == 4; }
+bool streamSupportsWaves(uint32_t nc) { return nc == 2 || nc == 4 || nc == 8; }
 
 // Dynamic LDS (the ring is static): LUT, value pool, draws, step records.
 uint32_t streamLdsBytes(uint32_t nc, uint32_t vpBytes, uint32_t maxSteps) {
@@ -378,18 +400,20 @@ uint32_t streamLdsBytes(uint32_t nc, uint32_t vpBytes, uint32_t maxSteps) {
 
 uint32_t streamStaticLdsBytes(uint32_t nc) { return kSlots * nc * kStreamRows * 64 * 4; }
 
+uint32_t streamBlock(uint32_t nc) { return (nc + kLoaders) * 64; }
+
 hipError_t launchStream(const StreamArgs &args, uint32_t nc, uint32_t grid, uint32_t lds, hipStream_t stream) {
   StreamArgs a = args;
   void *params[] = {&a};
-  return hipLaunchCooperativeKernel(streamFn(nc), dim3(grid), dim3((nc + 1) * 64), params, lds, stream);
+  return hipLaunchCooperativeKernel(streamFn(nc), dim3(grid), dim3((nc + kLoaders) * 64), params, lds, stream);
 }
 
 hipError_t streamMaxBlocksPerCU(int *blocks, uint32_t nc, uint32_t lds) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, streamFn(nc), (nc + 1) * 64, lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, streamFn(nc), (nc + kLoaders) * 64, lds);
 }
 
 hipError_t streamSetLds(uint32_t lds) {
-  for (uint32_t nc : {2u, 4u}) {
+  for (uint32_t nc : {2u, 4u, 8u}) {
     hipError_t e = hipFuncSetAttribute(streamFn(nc), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
   }

@@ -125,7 +125,7 @@ struct tgs_ctx {
   bool streamReady = false;
   StreamImage simg;
   DBuf sWords, sSteps, sStepBase, sStepCount, sConsts;
-  uint32_t streamNC = 4, streamGroups = 0, streamGrid = 0, streamLds = 0;
+  uint32_t streamNC = 0, streamNCUsed = 4, streamGroups = 0, streamGrid = 0, streamLds = 0;
 
   tgs_profile prof{};
 
@@ -257,7 +257,23 @@ int buildW64(tgs_ctx *c) {
 // limit), NC compute waves plus a loader wave each.
 int buildStream(tgs_ctx *c) {
   if (c->streamReady) return TGS_OK;
-  const uint32_t nc = c->streamNC;
+  uint32_t nc = c->streamNC;
+  if (!nc) {
+    // compute waves per workgroup (LUT micropartitions pair up in a wave)
+    std::vector<uint32_t> per;
+    for (const auto &p : c->design.parts)
+      for (const auto &lv : p.exec_mParts) {
+        uint32_t luts = 0, other = 0;
+        for (const auto &m : lv) (m.isLUTPart ? luts : other)++;
+        per.push_back((luts + 1) / 2 + other);
+      }
+    // the widest exec level decides: levels wider than the compute waves split
+    // into extra barrier-separated steps
+    uint32_t widest = 1;
+    for (uint32_t v : per) widest = std::max(widest, v);
+    nc = widest > 4 ? 8 : widest > 2 ? 4 : 2;
+  }
+  c->streamNCUsed = nc;
   const uint32_t maxRegion = std::max<uint32_t>(c->design.regionPartitionIds[0].size(),
                                                 c->design.parts.size() - c->design.regionPartitionIds[0].size());
   const uint32_t cus = c->prop.multiProcessorCount;
@@ -363,11 +379,11 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles) {
       a.barrierBase = 0;
       HIPCHK(c, hipMemsetAsync(in.ctl.as<uint32_t>() + C_BARRIER, 0, 4, c->stream));
       HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-      HIPCHK(c, launchStream(a, c->streamNC, c->streamGrid, c->streamLds, c->stream));
+      HIPCHK(c, launchStream(a, c->streamNCUsed, c->streamGrid, c->streamLds, c->stream));
       HIPCHK(c, hipEventRecord(c->ev1, c->stream));
       c->prof.grid = c->streamGrid;
-      c->prof.block = (c->streamNC + 1) * 64;
-      c->prof.lds_bytes = c->streamLds + streamStaticLdsBytes(c->streamNC);
+      c->prof.block = streamBlock(c->streamNCUsed);
+      c->prof.lds_bytes = c->streamLds + streamStaticLdsBytes(c->streamNCUsed);
       c->prof.resident = 3;
       c->prof.prog_bytes = c->simg.words.size() * 4;
       cycles -= n;
@@ -935,11 +951,12 @@ int tgs_set_launch(tgs_ctx *c, int kernel, uint32_t block, uint32_t groups) {
       c->batchReady = false;
       break;
     case TGS_KERNEL_STREAM:
-      if (block) {
-        if (!streamSupportsWaves(block / 64 - 1))
-          return c->fail(TGS_E_ARG, "streamed kernel block must be 192 or 320 (2 or 4 compute waves + loader)");
-        c->streamNC = block / 64 - 1;
-      }
+      c->streamNC = 0;  // automatic
+      for (uint32_t nc : {2u, 4u, 8u})
+        if (block && streamBlock(nc) == block) c->streamNC = nc;
+      if (block && !c->streamNC)
+        return c->fail(TGS_E_ARG, "streamed kernel block must be %u, %u or %u (2, 4 or 8 compute waves + loaders)",
+                       streamBlock(2), streamBlock(4), streamBlock(8));
       c->streamGroups = groups;
       c->streamReady = false;
       break;

@@ -56,21 +56,23 @@ struct Interp {
   static uint32_t lutIndex(uint32_t lt, uint32_t a, uint32_t b, uint32_t c) {
     return (lt + (((a & 0xFFFFu) << 8) | (b << 4) | c)) & 0xFFFFu;
   }
-  // row r of compute wave w in a step record, as the loader computes it
+  // row r of compute wave w in a step record, as the loader copies it: the
+  // loader always copies rows 0..5 from x, so they must lie in the image
   uint32_t rowWord(const uint32_t *rec, uint32_t w, uint32_t r, uint32_t lane) const {
-    const uint32_t type = rec[0], ex = rec[1 + 2 * w], rows = rec[2 + 2 * w] >> 24;
-    if (rows == 0) die("entry with zero rows");
-    const uint32_t last = rows - 1, rr = r < last ? r : last;
-    if (type == S_PART && rr > 0) return word(uint64_t(ex) + kRow + (rr - 1) * nTc + w * 64 + lane);
-    return word(uint64_t(ex) + rr * kRow + lane);
+    const uint32_t ex = rec[1 + 2 * w];
+    if (ex % kRow) die("entry not row aligned");
+    if (uint64_t(ex) + kStreamRows * kRow > im.words.size()) die("loader rows past the image");
+    if (r >= kStreamRows) die("row past the staged rows");
+    return word(uint64_t(ex) + r * kRow + lane);
   }
 
   void lutPair(const uint32_t *rec, uint32_t w) {
     const uint32_t ey = rec[2 + 2 * w], ex = rec[1 + 2 * w];
     const uint32_t nMid = (ey >> 8) & 0xFFFFu;
     uint32_t val[64], rows[64][kStreamRows];
+    if (3 + nMid != (ey >> 24) && (ey >> 24) != 255) die("LUT task row count");
     for (uint32_t l = 0; l < 64; ++l)
-      for (uint32_t r = 0; r < kStreamRows; ++r) rows[l][r] = rowWord(rec, w, r, l);
+      for (uint32_t r = 0; r < kStreamRows; ++r) rows[l][r] = r < 3 + nMid ? rowWord(rec, w, r, l) : 0xDEADBEEFu;
     for (uint32_t l = 0; l < 64; ++l) {
       const uint32_t x = rows[l][0], y = rows[l][1];
       const uint32_t lt = x & 0xFFFFu, o0 = x >> 16, o1 = y & 0xFFFFu, o2 = y >> 16;
@@ -181,17 +183,19 @@ struct Interp {
   void partBegin(const uint32_t *rec, uint32_t &constBase) {
     const uint32_t nJ = rowWord(rec, 0, 0, S_NJ);
     constBase = rowWord(rec, 0, 0, S_CONST_BASE);
-    const uint32_t ex = rec[1];
+    for (uint32_t w = 1; w < nc; ++w)
+      for (uint32_t k = 0; k < S_WORDS; ++k)
+        if (rowWord(rec, w, 0, k) != rowWord(rec, 0, 0, k)) die("partition header differs between waves");
     for (uint32_t t = 0; t < nTc; ++t) {
-      const uint32_t w = t / 64, l = t % 64;
+      const uint32_t w = t / 64, l = t % 64, ex = rec[1 + 2 * w];
       for (uint32_t j = 0; j < nJ; ++j) {
         uint32_t src, slot;
         if (j < 2) {
           src = rowWord(rec, w, 1 + 2 * j, l);
           slot = rowWord(rec, w, 2 + 2 * j, l);
         } else {
-          src = word(uint64_t(ex) + kRow + (2 * j) * nTc + t);
-          slot = word(uint64_t(ex) + kRow + (2 * j + 1) * nTc + t);
+          src = word(uint64_t(ex) + (1 + 2 * j) * kRow + l);
+          slot = word(uint64_t(ex) + (2 + 2 * j) * kRow + l);
         }
         const uint32_t kind = src >> 30, idx = src & 0x3FFFFFFFu;
         if (kind == 3) continue;
