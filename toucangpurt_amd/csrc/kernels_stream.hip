@@ -258,7 +258,6 @@ constexpr uint32_t kSlots = 2;
 // flight per wave (the wait before the stores is a plain wait for all).
 constexpr uint32_t kLoaders = 3;
 
-template <int NC>
 // Two (NC = 8) or three workgroups per CU: the register budget that keeps
 // every partition slot of a 1M-LUT4 design co-resident.
 template <int NC>
