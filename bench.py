@@ -117,8 +117,8 @@ def run_single(args):
     # pick the faster of the persistent kernels on a short run
     kernels = [KERNEL_STREAM, KERNEL_WAVE64] if args.kernel == "auto" else \
         [{"stream": KERNEL_STREAM, "wave64": KERNEL_WAVE64}[args.kernel]]
-    best = None
-    for k in kernels:
+    best = (0.0, kernels[0]) if len(kernels) == 1 else None
+    for k in (kernels if len(kernels) > 1 else []):
         sim = rt.Simulator(0)
         assert sim.init(0, d.design, d.symbols, False, False) == 0
         sim.set_kernel(k)

@@ -161,10 +161,20 @@ __device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, cons
     const uint32_t v1 = a1 ? l0Value(c, draws, p.r3) : 0u;
     if (a0) vp[p.r2] = uint8_t(v0);
     if (a1) vp[p.r4] = uint8_t(v1);
-    for (uint32_t j = 2; j < nJ; ++j) {
-      const uint32_t src = W[p.ex + (1 + 2 * j) * kRow + lane];
-      const uint32_t slot = W[p.ex + (2 + 2 * j) * kRow + lane];
-      if ((src >> 30) != 3) vp[slot] = uint8_t(l0Value(c, draws, src));
+    // the remaining entries four at a time: all row loads, then all gathers
+    for (uint32_t j = 2; j < nJ; j += 4) {
+      uint32_t src[4], slot[4], val[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const bool in = j + u < nJ;
+        src[u] = in ? W[p.ex + (1 + 2 * (j + u)) * kRow + lane] : kL0None;
+        slot[u] = in ? W[p.ex + (2 + 2 * (j + u)) * kRow + lane] : 0u;
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) val[u] = (src[u] >> 30) != 3 ? l0Value(c, draws, src[u]) : 0u;
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u)
+        if ((src[u] >> 30) != 3) vp[slot[u]] = uint8_t(val[u]);
     }
   }
 }
@@ -258,10 +268,11 @@ constexpr uint32_t kSlots = 2;
 // flight per wave (the wait before the stores is a plain wait for all).
 constexpr uint32_t kLoaders = 3;
 
-// Two (NC = 8) or three workgroups per CU: the register budget that keeps
-// every partition slot of a 1M-LUT4 design co-resident.
+// At least six waves per SIMD (HIP's second launch bound): two NC = 8 or
+// three NC = 4 workgroups per CU, which keeps every partition slot of a
+// 1M-LUT4 design co-resident.
 template <int NC>
-__global__ __launch_bounds__((NC + kLoaders) * 64, NC == 8 ? 2 : 3) void streamCycles(StreamArgs a) {
+__global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamArgs a) {
   extern __shared__ __align__(16) uint8_t sLds[];
   constexpr uint32_t nTc = NC * 64;
   constexpr uint32_t rec = 1 + 2 * NC;

@@ -278,7 +278,9 @@ int buildStream(tgs_ctx *c) {
                                                 c->design.parts.size() - c->design.regionPartitionIds[0].size());
   const uint32_t cus = c->prop.multiProcessorCount;
   const size_t limit = c->prop.sharedMemPerBlock;
-  uint32_t groups = c->streamGroups ? c->streamGroups : std::max<uint32_t>(std::min<uint32_t>(maxRegion, 4 * cus), 1);
+  // one workgroup per CU by default: a second workgroup on a CU slows both
+  // (shared LDS and issue), and the schedule pairs the short partitions
+  uint32_t groups = c->streamGroups ? c->streamGroups : std::max<uint32_t>(std::min<uint32_t>(maxRegion, cus), 1);
   uint32_t lds = 0;
   for (int attempt = 0;; ++attempt) {
     c->simg = buildStreamImage(c->design, c->stim, nc, groups);
