@@ -38,14 +38,16 @@ def design(name):
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="stream_check not built")
 @pytest.mark.parametrize("dname", list(DESIGNS))
-@pytest.mark.parametrize("nc,groups", [(2, 1), (4, 3), (4, 1000), (2, 7)])
-def test_stream_image_matches_oracle(tmp_path, dname, nc, groups):
+@pytest.mark.parametrize("nc,groups,slots", [(2, 1, 1), (4, 3, 1), (4, 1000, 1), (2, 7, 1), (8, 3, 2), (8, 1000, 2),
+                                             (4, 2, 2), (8, 5, 4)])
+def test_stream_image_matches_oracle(tmp_path, dname, nc, groups, slots):
     d = design(dname)
     (tmp_path / "d.bin").write_bytes(d.design)
     (tmp_path / "in.u32").write_bytes(np.asarray(d.inputs, np.uint32).tobytes())
     cycles, seed = 6, 29
     out = subprocess.run([EXE, str(tmp_path / "d.bin"), str(tmp_path / "in.u32"), str(nc), str(groups), str(cycles),
-                          str(seed), str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=300)
+                          str(seed), str(tmp_path / "out.bin"), "-", str(slots)], capture_output=True, text=True,
+                         timeout=300)
     assert out.returncode == 0, out.stderr
     got = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.uint8)
     orc = OracleRun(d.design, inputs=d.inputs, seed=seed)
@@ -57,8 +59,8 @@ def test_stream_image_matches_oracle(tmp_path, dname, nc, groups):
 
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="stream_check not built")
-@pytest.mark.parametrize("nc", [2, 4])
-def test_stream_image_edge_programs(tmp_path, nc):
+@pytest.mark.parametrize("nc,slots", [(2, 1), (4, 1), (8, 2)])
+def test_stream_image_edge_programs(tmp_path, nc, slots):
     import edge_programs as E
     import oracle as O
     for case in E.all_cases():
@@ -66,7 +68,7 @@ def test_stream_image_edge_programs(tmp_path, nc):
         (tmp_path / "in.u32").write_bytes(b"")
         (tmp_path / "init.bin").write_bytes(case.reg.tobytes() + case.mem.tobytes())
         out = subprocess.run([EXE, str(tmp_path / "d.bin"), str(tmp_path / "in.u32"), str(nc), "2",
-                              str(case.cycles), "0", str(tmp_path / "out.bin"), str(tmp_path / "init.bin")],
+                              str(case.cycles), "0", str(tmp_path / "out.bin"), str(tmp_path / "init.bin"), str(slots)],
                              capture_output=True, text=True, timeout=120)
         assert out.returncode == 0, (case.name, out.stderr)
         got = np.frombuffer((tmp_path / "out.bin").read_bytes(), np.uint8)

@@ -693,12 +693,15 @@ uint64_t lpt(const std::vector<uint64_t> &cost, uint32_t bins, std::vector<uint3
 
 }  // namespace
 
-StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32_t nWaves, uint32_t groups) {
+StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32_t nC, uint32_t groups,
+                             uint32_t slots) {
   StreamImage img;
-  img.nC = nWaves = std::max<uint32_t>(nWaves, 1);
+  img.nC = nC = std::max<uint32_t>(nC, 1);
   img.groups = groups = std::max<uint32_t>(groups, 1);
+  img.slots = slots = std::max<uint32_t>(std::min(slots, nC), 1);
   img.n0 = d.regionPartitionIds[0].size();
   img.nTotal = d.parts.size();
+  const uint32_t nWaves = nC / slots;  // compute waves of one partition slot
   const uint32_t nT = nWaves * 64;
   auto &W = img.words;
   addRows(W, 1);  // zero row
@@ -858,48 +861,78 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
     partCost[pi] = cost + 8 + 4 * nJ + (p.ops_last_memWrite.empty() ? 0 : 8);
   }
   if (img.consts.empty()) img.consts.push_back(0);
-  // partitions -> workgroups per region, then the per-wave entry streams
-  std::vector<std::vector<uint32_t>> per(size_t(groups) * 2);
+  // partitions -> (workgroup, slot) bins per region, then the step streams
+  const size_t bins = size_t(groups) * slots;
+  std::vector<std::vector<uint32_t>> per(bins * 2);
   for (int r = 0; r < 2; ++r) {
     const auto &ids = d.regionPartitionIds[r];
     std::vector<uint64_t> c(ids.size());
     for (size_t k = 0; k < ids.size(); ++k) c[k] = partCost[ids[k]];
     std::vector<uint32_t> binOf;
-    img.maxGroupCost += lpt(c, groups, binOf);
+    img.maxGroupCost += lpt(c, uint32_t(bins), binOf);
     for (size_t k = 0; k < ids.size(); ++k) per[size_t(binOf[k]) * 2 + r].push_back(ids[k]);
   }
-  const uint32_t rec = 1 + 2 * nWaves;
-  // partition end / region steps: the same entry for every wave
+  const uint32_t rec = 1 + 2 * nC;
+  // region steps: the same entry for every wave
   auto step = [&](uint32_t type, uint32_t x, uint32_t y) {
     img.steps.push_back(type);
-    for (uint32_t w = 0; w < nWaves; ++w) {
+    for (uint32_t w = 0; w < nC; ++w) {
       img.steps.push_back(x);
       img.steps.push_back(y);
     }
   };
   const uint32_t none = kKindNone << 4 | 1u << 24;
+  const uint32_t noPart = ~0u;
   for (uint32_t g = 0; g < groups; ++g) {
     const uint32_t base = img.steps.size() / rec;
     for (int r = 0; r < 2; ++r) {
-      for (uint32_t pi : per[size_t(g) * 2 + r]) {
-        img.steps.push_back(S_PART);
-        for (uint32_t w = 0; w < nWaves; ++w) {
-          img.steps.push_back(blockOff[pi] + w * blockRows[pi] * kRow);
-          img.steps.push_back(std::min<uint32_t>(blockRows[pi], 255) << 24);
+      size_t rounds = 0;
+      for (uint32_t s = 0; s < slots; ++s) rounds = std::max(rounds, per[(size_t(g) * slots + s) * 2 + r].size());
+      for (size_t k = 0; k < rounds; ++k) {
+        // the partitions of this round, one per slot (or none)
+        std::vector<uint32_t> part(slots, noPart);
+        for (uint32_t s = 0; s < slots; ++s) {
+          const auto &lst = per[(size_t(g) * slots + s) * 2 + r];
+          if (k < lst.size()) part[s] = lst[k];
         }
-        for (const auto &lv : waveTasks[pi]) {
-          size_t nSub = 1;
-          for (const auto &mine : lv) nSub = std::max(nSub, mine.size());
-          for (size_t k = 0; k < nSub; ++k) {
-            img.steps.push_back(S_LEVEL);
-            for (uint32_t w = 0; w < nWaves; ++w) {
-              const bool has = k < lv[w].size();
-              img.steps.push_back(has ? lv[w][k].off : 0u);
-              img.steps.push_back(has ? lv[w][k].y : none);
+        img.steps.push_back(S_PART);
+        for (uint32_t w = 0; w < nC; ++w) {
+          const uint32_t pi = part[w / nWaves];
+          img.steps.push_back(pi == noPart ? 0u : blockOff[pi] + (w % nWaves) * blockRows[pi] * kRow);
+          img.steps.push_back(pi == noPart ? none : std::min<uint32_t>(blockRows[pi], 255) << 24);
+        }
+        // each slot's level steps (a level wider than the slot's waves takes several), merged
+        std::vector<std::vector<std::vector<const StreamTask *>>> subs(slots);
+        size_t nSteps = 0;
+        for (uint32_t s = 0; s < slots; ++s) {
+          if (part[s] == noPart) continue;
+          for (const auto &lv : waveTasks[part[s]]) {
+            size_t nSub = 1;
+            for (const auto &mine : lv) nSub = std::max(nSub, mine.size());
+            for (size_t q = 0; q < nSub; ++q) {
+              std::vector<const StreamTask *> st(nWaves, nullptr);
+              for (uint32_t wi = 0; wi < nWaves; ++wi)
+                if (q < lv[wi].size()) st[wi] = &lv[wi][q];
+              subs[s].push_back(std::move(st));
             }
           }
+          nSteps = std::max(nSteps, subs[s].size());
         }
-        step(S_PEND, blockOff[pi], 1u << 24);
+        for (size_t i = 0; i < nSteps; ++i) {
+          img.steps.push_back(S_LEVEL);
+          for (uint32_t w = 0; w < nC; ++w) {
+            const uint32_t s = w / nWaves;
+            const StreamTask *t = i < subs[s].size() ? subs[s][i][w % nWaves] : nullptr;
+            img.steps.push_back(t ? t->off : 0u);
+            img.steps.push_back(t ? t->y : none);
+          }
+        }
+        img.steps.push_back(S_PEND);
+        for (uint32_t w = 0; w < nC; ++w) {
+          const uint32_t pi = part[w / nWaves];
+          img.steps.push_back(pi == noPart ? 0u : blockOff[pi]);
+          img.steps.push_back(pi == noPart ? none : 1u << 24);
+        }
       }
       step(S_REGION, 0, none);
     }

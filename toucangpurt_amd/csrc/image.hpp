@@ -62,17 +62,21 @@ struct Wave64Schedule {
 Wave64Schedule scheduleWave64(const Wave64Image &img, uint32_t groups);
 
 // Stream image: 64-word rows plus, per workgroup, the step stream of one
-// cycle (partitions assigned per region by longest processing time, the
-// tasks of every exec level assigned to the nC compute waves the same way).
+// cycle (partitions assigned per region to workgroup slots by longest
+// processing time, the tasks of every exec level assigned to the slot's
+// compute waves the same way).
 struct StreamImage {
   std::vector<uint32_t> words;
   std::vector<uint32_t> steps;                // step records, 1 + 2 * nC words each
   std::vector<uint32_t> stepBase, stepCount;  // per group: first record, records per cycle
   std::vector<uint8_t> consts;
-  uint32_t maxVp = 0, n0 = 0, nTotal = 0, groups = 0, nC = 0, maxSteps = 0;
+  uint32_t maxVp = 0, n0 = 0, nTotal = 0, groups = 0, nC = 0, slots = 1, maxSteps = 0;
   uint64_t maxGroupCost = 0, tasks = 0;
 };
+// `slots` partitions run side by side in a workgroup, on nC / slots compute
+// waves each (the partition-begin, level and partition-end steps of the
+// partitions assigned to the slots are merged).
 StreamImage buildStreamImage(const toucanGPUSim::SimDesignInfo &d, const StimMap &stim, uint32_t nC,
-                             uint32_t groups);
+                             uint32_t groups, uint32_t slots);
 
 }  // namespace tgs

@@ -33,7 +33,8 @@ struct StreamArgs {
   const uint32_t *steps;      // step records, 1 + 2 * NC words each
   const uint32_t *stepBase;   // per workgroup: first record
   const uint32_t *stepCount;  // per workgroup: records per cycle
-  uint32_t vpBytes;           // value-pool bytes in LDS (16-byte multiple)
+  uint32_t vpBytes;           // value-pool bytes in LDS per slot (16-byte multiple)
+  uint32_t slots;             // partitions side by side per workgroup (NC / slots waves each)
   uint32_t barrierBase;
 };
 
@@ -82,7 +83,7 @@ uint32_t wave64LutBytes();
 // Streamed kernel with `nc` compute waves (2, 4 or 8) plus three loader waves.
 bool streamSupportsWaves(uint32_t nc);
 uint32_t streamBlock(uint32_t nc);
-uint32_t streamLdsBytes(uint32_t nc, uint32_t vpBytes, uint32_t maxSteps);  // dynamic part
+uint32_t streamLdsBytes(uint32_t nc, uint32_t slots, uint32_t vpBytes, uint32_t maxSteps);  // dynamic part
 uint32_t streamStaticLdsBytes(uint32_t nc);
 hipError_t launchStream(const StreamArgs &args, uint32_t nc, uint32_t grid, uint32_t lds, hipStream_t stream);
 hipError_t streamMaxBlocksPerCU(int *blocks, uint32_t nc, uint32_t lds);

@@ -148,9 +148,11 @@ __device__ __forceinline__ uint32_t l0Value(const CommonArgs &c, const uint64_t 
   return 0;
 }
 
-// Partition begin: L0 reads (SimEval.cu:582-622) by the compute threads.
+// Partition begin: L0 reads (SimEval.cu:582-622) by the compute threads of
+// the partition's slot; each wave reads its own section of the block.
 __device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, const uint32_t *__restrict__ W,
-                                          uint8_t *vp, const uint64_t *draws, uint32_t nTc, uint32_t &constBase) {
+                                          uint8_t *vp, const uint64_t *draws, uint32_t &constBase) {
+  if (((p.ey >> 4) & 15u) == kKindNone) return;  // no partition in this slot
   const uint32_t tid = threadIdx.x;
   const uint32_t nJ = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(p.r0, S_NJ));
   constBase = __builtin_amdgcn_readlane(p.r0, S_CONST_BASE);
@@ -179,10 +181,11 @@ __device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, cons
   }
 }
 
-// Partition end: last level (SimEval.cu:625-761) by the compute threads.
+// Partition end: last level (SimEval.cu:625-761) by the compute threads of
+// the partition's slot (thread `tid` of `nTc`).
 __device__ __forceinline__ void partEnd(const Pf &p, const CommonArgs &c, const uint32_t *__restrict__ W,
-                                        const uint8_t *vp, uint64_t cycle, uint32_t nTc) {
-  const uint32_t tid = threadIdx.x;
+                                        const uint8_t *vp, uint64_t cycle, uint32_t tid, uint32_t nTc) {
+  if (((p.ey >> 4) & 15u) == kKindNone) return;  // no partition in this slot
   const uint32_t ewdc = __builtin_amdgcn_readlane(p.r0, S_EW_DC);
   if (ewdc >> 16) {
     const uint32_t exg = __builtin_amdgcn_readlane(p.r0, S_EW_EXG), dat = ewdc & 0xFFFFu;
@@ -239,11 +242,13 @@ __device__ __forceinline__ void loadRows(RowSet<NC> &s, const uint32_t *rec, con
                                          uint32_t lane) {
   constexpr uint32_t words = 1 + 2 * NC;
   const uint32_t rv = rec[lane < words ? lane : 0];
+  // unconditional loads: making them depend on the rows an entry needs (exec
+  // masks or a uniform skip) measured 20-60% slower per step
 #pragma unroll
   for (int w = 0; w < NC; ++w) {
-    const u32x4 *rows = reinterpret_cast<const u32x4 *>(W + __builtin_amdgcn_readlane(rv, 1 + 2 * w));
-    s.lo[w] = rows[lane];
-    s.hi[w] = rows[64 + lane];
+    const u32x4 *src = reinterpret_cast<const u32x4 *>(W + __builtin_amdgcn_readlane(rv, 1 + 2 * w));
+    s.lo[w] = src[lane];
+    s.hi[w] = src[64 + lane];
   }
 }
 
@@ -279,8 +284,8 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
   constexpr uint32_t slotWords = NC * kStreamRows * 64;
   __shared__ __align__(16) uint32_t ring[kSlots * slotWords];
   uint8_t *lut = sLds;
-  uint8_t *vp = sLds + kLutBytesS;
-  uint64_t *draws = reinterpret_cast<uint64_t *>(vp + a.vpBytes);
+  uint8_t *vpAll = sLds + kLutBytesS;  // one value pool per partition slot
+  uint64_t *draws = reinterpret_cast<uint64_t *>(vpAll + a.slots * a.vpBytes);
   uint32_t *steps = reinterpret_cast<uint32_t *>(draws + kMaxDraws);
   const uint32_t g = blockIdx.x, tid = threadIdx.x, lane = tid & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
@@ -331,7 +336,11 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     }
     return;
   }
-  // compute waves
+  // compute waves: wave `wave` serves partition slot wave / (NC / slots)
+  const uint32_t slotWaves = NC / a.slots;
+  const uint32_t slot = wave / slotWaves;
+  const uint32_t ts = (wave - slot * slotWaves) * 64 + lane, nTs = slotWaves * 64;
+  uint8_t *vp = vpAll + slot * a.vpBytes;
   uint64_t xs = *c.xs;
   uint64_t barTicks = 0;
   const bool prof = g == 0;
@@ -371,10 +380,10 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
       else if (kind == T_MEMREAD) memReadS(cur, vp, c);
       __syncthreads();
     } else if (type == S_PART) {
-      partBegin(cur, c, W, vp, draws, nTc, constBase);
+      partBegin(cur, c, W, vp, draws, constBase);
       __syncthreads();
     } else if (type == S_PEND) {
-      partEnd(cur, c, W, vp, c.cycleBase + cyc, nTc);
+      partEnd(cur, c, W, vp, c.cycleBase + cyc, ts, nTs);
       __syncthreads();
     } else {
       barTicks += gridBarrier(&c.ctl[C_BARRIER], (++bar) * gridDim.x, prof);
@@ -403,9 +412,9 @@ const void *streamFn(uint32_t nc) {
 
 bool streamSupportsWaves(uint32_t nc) { return nc == 2 || nc == 4 || nc == 8; }
 
-// Dynamic LDS (the ring is static): LUT, value pool, draws, step records.
-uint32_t streamLdsBytes(uint32_t nc, uint32_t vpBytes, uint32_t maxSteps) {
-  return kLutBytesS + vpBytes + kMaxDraws * 8 + maxSteps * (1 + 2 * nc) * 4;
+// Dynamic LDS (the ring is static): LUT, value pools, draws, step records.
+uint32_t streamLdsBytes(uint32_t nc, uint32_t slots, uint32_t vpBytes, uint32_t maxSteps) {
+  return kLutBytesS + slots * vpBytes + kMaxDraws * 8 + maxSteps * (1 + 2 * nc) * 4;
 }
 
 uint32_t streamStaticLdsBytes(uint32_t nc) { return kSlots * nc * kStreamRows * 64 * 4; }

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iterator>
@@ -125,7 +126,7 @@ struct tgs_ctx {
   bool streamReady = false;
   StreamImage simg;
   DBuf sWords, sSteps, sStepBase, sStepCount, sConsts;
-  uint32_t streamNC = 0, streamNCUsed = 4, streamGroups = 0, streamGrid = 0, streamLds = 0;
+  uint32_t streamNC = 0, streamNCUsed = 4, streamSlots = 1, streamGroups = 0, streamGrid = 0, streamLds = 0;
 
   tgs_profile prof{};
 
@@ -257,6 +258,9 @@ int buildW64(tgs_ctx *c) {
 // limit), NC compute waves plus a loader wave each.
 int buildStream(tgs_ctx *c) {
   if (c->streamReady) return TGS_OK;
+  const uint32_t maxRegion = std::max<uint32_t>(c->design.regionPartitionIds[0].size(),
+                                                c->design.parts.size() - c->design.regionPartitionIds[0].size());
+  const uint32_t cus = c->prop.multiProcessorCount;
   uint32_t nc = c->streamNC;
   if (!nc) {
     // compute waves per workgroup (LUT micropartitions pair up in a wave)
@@ -274,18 +278,35 @@ int buildStream(tgs_ctx *c) {
     nc = widest > 4 ? 8 : widest > 2 ? 4 : 2;
   }
   c->streamNCUsed = nc;
-  const uint32_t maxRegion = std::max<uint32_t>(c->design.regionPartitionIds[0].size(),
-                                                c->design.parts.size() - c->design.regionPartitionIds[0].size());
-  const uint32_t cus = c->prop.multiProcessorCount;
+  // two partitions side by side on four waves each when a region has clearly
+  // more partitions than there are CUs and most exec levels fit in four
+  // waves (the widest still run, in several steps)
+  uint32_t slots = 1;
+  if (nc == 8 && 2 * maxRegion > 3 * cus) {
+    std::vector<uint32_t> per;
+    for (const auto &p : c->design.parts)
+      for (const auto &lv : p.exec_mParts) {
+        uint32_t luts = 0, other = 0;
+        for (const auto &m : lv) (m.isLUTPart ? luts : other)++;
+        per.push_back((luts + 1) / 2 + other);
+      }
+    if (!per.empty()) {
+      std::sort(per.begin(), per.end());
+      if (per[per.size() * 9 / 10] <= 4) slots = 2;
+    }
+  }
+  if (const char *e = std::getenv("TGS_STREAM_SLOTS")) slots = std::max(1, std::min(std::atoi(e), int(nc)));
+  if (nc % slots) slots = 1;
+  c->streamSlots = slots;
   const size_t limit = c->prop.sharedMemPerBlock;
   // one workgroup per CU by default: a second workgroup on a CU slows both
   // (shared LDS and issue), and the schedule pairs the short partitions
   uint32_t groups = c->streamGroups ? c->streamGroups : std::max<uint32_t>(std::min<uint32_t>(maxRegion, cus), 1);
   uint32_t lds = 0;
   for (int attempt = 0;; ++attempt) {
-    c->simg = buildStreamImage(c->design, c->stim, nc, groups);
+    c->simg = buildStreamImage(c->design, c->stim, nc, groups, slots);
     const uint32_t vpBytes = up16(std::max<uint32_t>(c->simg.maxVp, 16));
-    lds = streamLdsBytes(nc, vpBytes, c->simg.maxSteps);
+    lds = streamLdsBytes(nc, slots, vpBytes, c->simg.maxSteps);
     if (lds + streamStaticLdsBytes(nc) > limit)
       return c->fail(TGS_E_LIMIT, "streamed kernel needs %u B of LDS (value pool %u, %u steps per cycle); device allows %zu B",
                      lds + streamStaticLdsBytes(nc), c->simg.maxVp, c->simg.maxSteps, limit);
@@ -378,6 +399,7 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles) {
       a.stepBase = c->sStepBase.as<uint32_t>();
       a.stepCount = c->sStepCount.as<uint32_t>();
       a.vpBytes = up16(std::max<uint32_t>(c->simg.maxVp, 16));
+      a.slots = c->streamSlots;
       a.barrierBase = 0;
       HIPCHK(c, hipMemsetAsync(in.ctl.as<uint32_t>() + C_BARRIER, 0, 4, c->stream));
       HIPCHK(c, hipEventRecord(c->ev0, c->stream));

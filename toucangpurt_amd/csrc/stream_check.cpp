@@ -6,7 +6,7 @@
 // final register and memory pools so a test can compare them with the
 // oracle. It is not linked into the runtime and never runs on the device path.
 //
-// Usage: stream_check <design.bin> <inputs.u32> <nc> <groups> <cycles> <seed> <out.bin> [init.bin]
+// Usage: stream_check <design.bin> <inputs.u32> <nc> <groups> <cycles> <seed> <out.bin> [init.bin] [slots]
 // init.bin, if given, holds the initial register pool followed by the memory pool.
 #include <algorithm>
 #include <cstdio>
@@ -42,7 +42,7 @@ struct Interp {
   const StreamImage &im;
   const SimDesignInfo &d;
   uint32_t nc, nTc;
-  std::vector<uint8_t> reg, mem, exg, vp, lut;
+  std::vector<uint8_t> reg, mem, exg, vp, lut;  // vp: the value pool of the slot being evaluated
   std::vector<uint64_t> draws;
   uint32_t vpBytes;
 
@@ -180,14 +180,15 @@ struct Interp {
     }
   }
 
-  void partBegin(const uint32_t *rec, uint32_t &constBase) {
-    const uint32_t nJ = rowWord(rec, 0, 0, S_NJ);
-    constBase = rowWord(rec, 0, 0, S_CONST_BASE);
-    for (uint32_t w = 1; w < nc; ++w)
+  // L0 of the partition on waves [w0, w0 + nw)
+  void partBegin(const uint32_t *rec, uint32_t w0, uint32_t nw, uint32_t &constBase) {
+    const uint32_t nJ = rowWord(rec, w0, 0, S_NJ);
+    constBase = rowWord(rec, w0, 0, S_CONST_BASE);
+    for (uint32_t w = w0 + 1; w < w0 + nw; ++w)
       for (uint32_t k = 0; k < S_WORDS; ++k)
-        if (rowWord(rec, w, 0, k) != rowWord(rec, 0, 0, k)) die("partition header differs between waves");
-    for (uint32_t t = 0; t < nTc; ++t) {
-      const uint32_t w = t / 64, l = t % 64, ex = rec[1 + 2 * w];
+        if (rowWord(rec, w, 0, k) != rowWord(rec, w0, 0, k)) die("partition header differs between waves");
+    for (uint32_t t = 0; t < nw * 64; ++t) {
+      const uint32_t w = w0 + t / 64, l = t % 64, ex = rec[1 + 2 * w];
       for (uint32_t j = 0; j < nJ; ++j) {
         uint32_t src, slot;
         if (j < 2) {
@@ -208,9 +209,10 @@ struct Interp {
     }
   }
 
-  void partEnd(const uint32_t *rec, uint64_t cycle, std::vector<std::pair<uint64_t, uint32_t>> &log, bool &stop) {
-    auto h = [&](uint32_t k) { return rowWord(rec, 0, 0, k); };
-    const uint32_t ex = rec[1];
+  void partEnd(const uint32_t *rec, uint32_t w0, uint64_t cycle, std::vector<std::pair<uint64_t, uint32_t>> &log,
+               bool &stop) {
+    auto h = [&](uint32_t k) { return rowWord(rec, w0, 0, k); };
+    const uint32_t ex = rec[1 + 2 * w0];
     const uint32_t ewdc = h(S_EW_DC);
     auto copyWords = [&](std::vector<uint8_t> &pool, const char *name, uint32_t dst, uint32_t dat, uint32_t cnt) {
       const uint32_t words = (cnt + 3) / 4;
@@ -277,7 +279,10 @@ int main(int argc, char **argv) {
   std::memcpy(ids.data(), inb.data(), ids.size() * 4);
   StimMap stim;
   if (buildStimMap(d, ids.data(), ids.size(), &stim, &err) != TGS_OK) die("stimulus: " + err);
-  StreamImage im = buildStreamImage(d, stim, nc, groups);
+  const uint32_t slots = argc > 9 ? std::atoi(argv[9]) : 1;
+  StreamImage im = buildStreamImage(d, stim, nc, groups, slots);
+  if (im.slots != slots || nc % slots) die("slots must divide the compute waves");
+  const uint32_t ws = nc / slots;
   if (im.stepBase.size() != im.groups || im.stepCount.size() != im.groups) die("step tables");
   if (im.words.size() % kRow) die("image is not a whole number of rows");
   Interp it{im, d, nc, nc * 64};
@@ -286,7 +291,7 @@ int main(int argc, char **argv) {
   it.mem = d.memPool;
   it.mem.resize(d.memPoolSize, 0);
   it.exg.assign(d.exchangePoolSize, 0);
-  if (argc > 8) {
+  if (argc > 8 && std::strcmp(argv[8], "-") != 0) {
     auto init = slurp(argv[8]);
     if (init.size() != size_t(d.regPoolSize) + d.memPoolSize) die("init.bin must hold the register and memory pools");
     std::copy(init.begin(), init.begin() + d.regPoolSize, it.reg.begin());
@@ -308,8 +313,9 @@ int main(int argc, char **argv) {
       for (uint32_t g = 0; g < im.groups; ++g) {
         // find this group's steps of region r: regions end with S_REGION
         int region = 0;
-        uint32_t constBase = 0;
-        it.vp.assign(it.vpBytes, 0xEE);
+        std::vector<uint32_t> constBase(slots, 0);
+        std::vector<std::vector<uint8_t>> vps(slots, std::vector<uint8_t>(it.vpBytes, 0xEE));
+        auto kindOf = [&](const uint32_t *R, uint32_t w) { return (R[2 + 2 * w] >> 4) & 15u; };
         for (uint32_t s = 0; s < im.stepCount[g]; ++s) {
           const uint32_t *R = &im.steps.at(size_t(im.stepBase[g] + s) * rec);
           const uint32_t type = R[0];
@@ -318,24 +324,35 @@ int main(int argc, char **argv) {
             continue;
           }
           if (region != r) continue;
-          if (type == S_PART) {
-            it.vp.assign(it.vpBytes, 0xEE);  // uninitialized every visit
-            it.partBegin(R, constBase);
-          } else if (type == S_PEND) {
-            it.partEnd(R, cy, log, stop);
-          } else if (type == S_LEVEL) {
-            const uint8_t *consts = im.consts.data() + constBase;
-            const size_t nConsts = im.consts.size() - constBase;
-            for (uint32_t w = 0; w < nc; ++w) {
-              const uint32_t kind = (R[2 + 2 * w] >> 4) & 15u;
-              if (kind == T_LUT2) it.lutPair(R, w);
-              else if (kind == T_VECREAD) it.vecRead(R, w, consts, nConsts);
-              else if (kind == T_VECOP) it.vecOp(R, w, consts, nConsts);
-              else if (kind == T_MEMREAD) it.memRead(R, w);
-              else if (kind != kKindNone) die("unknown task kind");
+          for (uint32_t sl = 0; sl < slots; ++sl) {
+            const uint32_t w0 = sl * ws;
+            // a slot's waves agree on whether the slot has a partition in this step
+            if (type != S_LEVEL)
+              for (uint32_t w = w0 + 1; w < w0 + ws; ++w)
+                if ((kindOf(R, w) == kKindNone) != (kindOf(R, w0) == kKindNone)) die("slot waves disagree");
+            it.vp.swap(vps[sl]);
+            if (type == S_PART) {
+              if (kindOf(R, w0) != kKindNone) {
+                it.vp.assign(it.vpBytes, 0xEE);  // uninitialized every visit
+                it.partBegin(R, w0, ws, constBase[sl]);
+              }
+            } else if (type == S_PEND) {
+              if (kindOf(R, w0) != kKindNone) it.partEnd(R, w0, cy, log, stop);
+            } else if (type == S_LEVEL) {
+              const uint8_t *consts = im.consts.data() + constBase[sl];
+              const size_t nConsts = im.consts.size() - constBase[sl];
+              for (uint32_t w = w0; w < w0 + ws; ++w) {
+                const uint32_t kind = kindOf(R, w);
+                if (kind == T_LUT2) it.lutPair(R, w);
+                else if (kind == T_VECREAD) it.vecRead(R, w, consts, nConsts);
+                else if (kind == T_VECOP) it.vecOp(R, w, consts, nConsts);
+                else if (kind == T_MEMREAD) it.memRead(R, w);
+                else if (kind != kKindNone) die("unknown task kind");
+              }
+            } else {
+              die("unknown step type");
             }
-          } else {
-            die("unknown step type");
+            it.vp.swap(vps[sl]);
           }
         }
         if (region != 2) die("a group's stream must hold two region barriers");
