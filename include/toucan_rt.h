@@ -136,6 +136,13 @@ int tgs_read_pool(tgs_ctx *ctx, uint32_t instance, int pool, uint64_t offset, ui
 int tgs_write_pool(tgs_ctx *ctx, uint32_t instance, int pool, uint64_t offset, uint64_t n, const uint8_t *in);
 int tgs_pool_size(const tgs_ctx *ctx, int pool, uint64_t *size);
 
+/* Restart one instance from the loaded initial state (Init/Randomize pools,
+ * zero exchange pool, cycle 0, stop cleared) with the stimulus generator of
+ * instance `index` of seed `seed` (splitmix64(seed + index)). The serving
+ * loop reuses a slot for a new job with it; the reference has one global
+ * instance and no counterpart (ToucanSim.cu:102-204 reloads everything). */
+int tgs_reset_instance(tgs_ctx *ctx, uint32_t instance, uint64_t seed, uint32_t index);
+
 /* get_eval_done (src/SimEval.cu:1083-1087) and get_real_cycles (:1089-1093). */
 int tgs_get_stop(tgs_ctx *ctx, uint32_t instance, int *stopped);
 int tgs_get_cycles(tgs_ctx *ctx, uint32_t instance, uint64_t *cycles);

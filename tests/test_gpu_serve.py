@@ -1,0 +1,51 @@
+"""Serving on the GPU: jobs of several designs with uneven sizes, staggered
+arrivals and a slice much shorter than the jobs, so jobs join running
+batches and reuse freed slots. Every job's result must equal the same job
+run alone in its own context, and (for one job per design) the oracle."""
+import numpy as np
+import pytest
+
+from simharness import OracleRun
+from toucangpurt_amd import gen
+from toucangpurt_amd.serve import Engine, Job, run_alone
+
+pytestmark = pytest.mark.gpu
+
+DESIGNS = {
+    "small": dict(n_luts=1500, depth=6, n_regs=90, n_inputs=20, part_luts=300, seed=21),
+    "mixed": dict(n_luts=5000, depth=8, n_regs=250, n_inputs=70, n_vecops=300, n_srams=2, sram_depth=512,
+                  sram_width=32, part_luts=600, word_part_ops=32, seed=22),
+    "wide": dict(n_luts=12000, depth=10, n_regs=400, n_inputs=130, n_vecops=200, word_bits_mix=100,
+                 part_luts=512, word_part_ops=24, seed=23),
+}
+
+
+def test_serving_results_alone_equal_mix(gpu_available):
+    designs = {k: gen.generate(None, **v) for k, v in DESIGNS.items()}
+    rng = np.random.default_rng(5)
+    jobs = []
+    t = 0.0
+    for k in range(12):
+        name = list(DESIGNS)[k % 3]
+        jobs.append(Job(k, name, int(rng.integers(1, 40)), int(rng.integers(60, 400)), int(rng.integers(1 << 30)),
+                        arrival=t))
+        t += 0.002
+    print(f"case serving {len(jobs)} jobs", flush=True)
+    eng = Engine(designs, capacity=64, slice_cycles=48)
+    rep = eng.run(jobs)
+    eng.close()
+    assert rep["jobs"] == len(jobs)
+    assert rep["launches"] > len(jobs)
+    checked = set()
+    for j in jobs:
+        reg, mem = run_alone(designs[j.design], j)
+        for i in range(j.instances):
+            assert np.array_equal(j.reg[i], reg[i]), (j.job_id, i)
+            assert np.array_equal(j.mem[i], mem[i]), (j.job_id, i)
+        if j.design not in checked:
+            checked.add(j.design)
+            d = designs[j.design]
+            orc = OracleRun(d.design, inputs=d.inputs, seed=j.seed, instance=j.instances - 1)
+            orc.run(j.cycles)
+            assert np.array_equal(j.reg[-1], orc.pools()[0]), j.job_id
+    assert rep["p99_latency_s"] >= rep["p50_latency_s"] > 0

@@ -1,0 +1,101 @@
+"""Serving engine logic on the host: admission by slots and HBM budget,
+slicing, slot reuse, eviction, and the alone-vs-mix property, with the
+oracle standing in for the device (each instance slot an oracle state)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from simharness import initial_pools
+from toucangpurt_amd import gen
+from toucangpurt_amd.serve import POOL_MEM, POOL_REG, Engine, Job, poisson_jobs, run_alone
+
+
+class OracleBackend:
+    """Batched-context stand-in: one oracle state per instance slot."""
+
+    def __init__(self, design):
+        self.prog, self.reg0, self.mem0 = initial_pools(design.design, None, False)
+        self.inputs = None
+        self.st = []
+
+    def set_instances(self, n):
+        self.st = [O.State(self.prog, self.reg0, self.mem0) for _ in range(n)]
+
+    def set_stimulus(self, inputs, seed):
+        self.inputs = np.asarray(inputs, np.uint32)
+        for k, s in enumerate(self.st):
+            s.set_stimulus(self.inputs, seed, k)
+
+    def reset_instance(self, slot, seed, index):
+        s = O.State(self.prog, self.reg0, self.mem0)
+        s.set_stimulus(self.inputs, seed, index)
+        self.st[slot] = s
+
+    def run_batch(self, cycles):
+        assert len(cycles) == len(self.st)
+        for s, c in zip(self.st, cycles):
+            if c:
+                s.run(int(c))
+
+    def read_pool(self, pool, slot):
+        return self.st[slot].pool(pool).copy()
+
+    def cleanup(self):
+        self.st = []
+
+
+DESIGNS = {
+    "a": dict(n_luts=800, depth=5, n_regs=60, n_inputs=12, part_luts=200, seed=3),
+    "b": dict(n_luts=1500, depth=6, n_regs=90, n_inputs=20, n_vecops=60, n_srams=1, sram_depth=128,
+              sram_width=16, part_luts=300, seed=4),
+}
+
+
+@pytest.fixture(scope="module")
+def designs():
+    return {k: gen.generate(None, **v) for k, v in DESIGNS.items()}
+
+
+def test_jobs_alone_equal_mixed(designs):
+    jobs = [Job(0, "a", 3, 40, 11, arrival=0.0), Job(1, "b", 2, 25, 12, arrival=0.0),
+            Job(2, "a", 4, 17, 13, arrival=0.0), Job(3, "a", 2, 33, 14, arrival=0.0),
+            Job(4, "b", 5, 9, 15, arrival=0.0)]
+    eng = Engine(designs, capacity=8, slice_cycles=7, backend=OracleBackend)
+    rep = eng.run(jobs)
+    assert rep["jobs"] == len(jobs)
+    # slot reuse: 11 instances of design a through 8 slots
+    assert eng.launches > 3
+    for j in jobs:
+        reg, mem = run_alone(designs[j.design], j, backend=OracleBackend)
+        for i in range(j.instances):
+            np.testing.assert_array_equal(j.reg[i], reg[i], err_msg=f"job {j.job_id} instance {i}")
+            np.testing.assert_array_equal(j.mem[i], mem[i])
+    assert rep["instance_cycles"] == sum(j.instances * j.cycles for j in jobs)
+
+
+def test_budget_evicts_idle_context(designs):
+    eng = Engine(designs, capacity=4, slice_cycles=50, backend=OracleBackend)
+    one = max(eng.context_bytes(d) for d in designs.values())
+    eng.budget = one + 1  # room for one resident design at a time
+    jobs = [Job(0, "a", 2, 10, 1), Job(1, "b", 2, 10, 2), Job(2, "a", 1, 10, 3)]
+    rep = eng.run(jobs)
+    assert rep["jobs"] == 3
+    assert len(eng.ctx) == 1 and eng.used <= eng.budget
+
+
+def test_rejects_oversized_jobs(designs):
+    eng = Engine(designs, capacity=4, backend=OracleBackend)
+    with pytest.raises(ValueError):
+        eng.submit(Job(0, "a", 5, 10, 1))
+    with pytest.raises(ValueError):
+        eng.submit(Job(1, "a", 1, 0, 1))
+
+
+def test_poisson_jobs_shape():
+    rng = np.random.default_rng(0)
+    jobs = poisson_jobs(["x", "y"], 200, 200.0, rng)
+    t = np.array([j.arrival for j in jobs])
+    assert np.all(np.diff(t) >= 0)
+    # mean inter-arrival 5 ms within sampling error
+    assert 0.0035 < np.diff(t).mean() < 0.0065
+    assert all(1 <= j.instances <= 64 and 1000 <= j.cycles <= 100_000 for j in jobs)

@@ -164,7 +164,7 @@ int upload(tgs_ctx *c, DBuf &b, const void *src, size_t n) {
 
 // Reset one instance to the loaded initial state (copy_netlist_to_gpu,
 // SimEval.cu:1114-1139: stop flag and cycle counter cleared, exchange zeroed).
-int resetInstance(tgs_ctx *c, Instance &in, uint32_t index) {
+int resetInstance(tgs_ctx *c, Instance &in, uint32_t index, uint64_t seed) {
   const auto &d = c->design;
   HIPCHK(c, in.reg.alloc(up4(d.regPoolSize) + 16));
   HIPCHK(c, in.mem.alloc(size_t(d.memPoolSize) + 16));
@@ -184,7 +184,7 @@ int resetInstance(tgs_ctx *c, Instance &in, uint32_t index) {
   uint64_t never = UINT64_MAX;
   std::memcpy(&ctl[C_STOPCYC], &never, 8);
   HIPCHK(c, hipMemcpyAsync(in.ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice, c->stream));
-  std::vector<uint64_t> xs(1, seedFor(c->stimSeed, index));
+  std::vector<uint64_t> xs(1, seedFor(seed, index));
   HIPCHK(c, hipMemcpyAsync(in.xs.p, xs.data(), 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   in.cycles = 0;
@@ -479,7 +479,7 @@ int instUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
   for (uint32_t i = 0; i < h.size(); ++i) {
     c->inst.emplace_back(new Instance());
     Instance &in = *c->inst.back();
-    int rc = resetInstance(c, in, i);
+    int rc = resetInstance(c, in, i, c->stimSeed);
     if (rc) return rc;
     const auto &x = h[i];
     if (!x.reg.empty()) HIPCHK(c, hipMemcpy(in.reg.p, x.reg.data(), x.reg.size(), hipMemcpyHostToDevice));
@@ -922,7 +922,7 @@ int tgs_load(tgs_ctx *c, const uint8_t *design, size_t designSize, const uint8_t
   c->batchReady = false;
   c->kernel = c->kernel == TGS_KERNEL_BATCH ? TGS_KERNEL_WAVE64 : c->kernel;
   c->inst.emplace_back(new Instance());
-  if ((rc = resetInstance(c, *c->inst[0], 0))) return rc;
+  if ((rc = resetInstance(c, *c->inst[0], 0, c->stimSeed))) return rc;
   c->loaded = true;
   c->prof = tgs_profile{};
   return TGS_OK;
@@ -1151,6 +1151,29 @@ int tgs_get_stop(tgs_ctx *c, uint32_t instance, int *stopped) {
   uint64_t v = 0;
   HIPCHK(c, hipMemcpy(&v, c->bs.stop.as<uint64_t>() + c->bs.slotOf[instance], 8, hipMemcpyDeviceToHost));
   *stopped = v != 0;
+  return TGS_OK;
+}
+
+int tgs_reset_instance(tgs_ctx *c, uint32_t instance, uint64_t seed, uint32_t index) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if (instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "instance %u out of range", instance);
+  if ((rc = setDevice(c))) return rc;
+  if (!c->batched) return resetInstance(c, *c->inst[instance], index, seed);
+  auto &b = c->bs;
+  const auto &d = c->design;
+  std::vector<uint8_t> reg(d.regPoolSize, 0), mem(d.memPoolSize, 0), exg(d.exchangePoolSize, 0);
+  std::copy(c->initReg.begin(), c->initReg.end(), reg.begin());
+  std::copy(c->initMem.begin(), c->initMem.end(), mem.begin());
+  if ((rc = tgs_write_pool(c, instance, TGS_POOL_REG, 0, reg.size(), reg.data()))) return rc;
+  if ((rc = tgs_write_pool(c, instance, TGS_POOL_MEM, 0, mem.size(), mem.data()))) return rc;
+  if ((rc = tgs_write_pool(c, instance, TGS_POOL_EXCHANGE, 0, exg.size(), exg.data()))) return rc;
+  const uint32_t s = b.slotOf[instance];
+  const uint64_t xs = seedFor(seed, index), zero = 0, never = UINT64_MAX;
+  HIPCHK(c, hipMemcpy(b.xs.as<uint64_t>() + s, &xs, 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(b.stop.as<uint64_t>() + s, &zero, 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(b.stopCycle.as<uint64_t>() + s, &never, 8, hipMemcpyHostToDevice));
+  b.cycles[instance] = 0;
   return TGS_OK;
 }
 

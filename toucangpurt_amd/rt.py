@@ -207,6 +207,11 @@ class Simulator:
         self._check(self._lib.tgs_get_stop_cycle(self._h, instance, ctypes.byref(c)))
         return c.value
 
+    def reset_instance(self, instance, seed, index):
+        """Restart one instance from the initial state with stimulus generator
+        splitmix64(seed + index) (a slot reused for a new job)."""
+        self._check(self._lib.tgs_reset_instance(self._h, int(instance), int(seed), int(index)))
+
     def cycles(self, instance=0):
         c = ctypes.c_uint64()
         self._check(self._lib.tgs_get_cycles(self._h, instance, ctypes.byref(c)))

@@ -1,0 +1,275 @@
+"""Serving: continuous batching of simulation jobs on one GPU.
+
+A job is (design, instances, cycles, seed): run `instances` stimulus instances
+of one design for `cycles` cycles each and return their final register and
+memory pools. Instance i of a job is driven by the xorshift64 generator
+seeded with splitmix64(seed + i), so a job's result is exactly the result of
+running it alone in its own context (tgs_set_instances(n) +
+tgs_set_stimulus(inputs, seed)).
+
+The engine keeps one batched context per design resident in HBM (program
+image plus instance-minor state for a fixed number of instance slots) and
+runs in slices: every step makes one persistent launch per design with
+active slots, each active slot advancing min(remaining, slice) cycles.
+Requests that arrive while others run are admitted at the next step into
+slots that finished jobs freed, so jobs of one design overlap in the same
+launch and jobs of different designs alternate launch by launch. Admission
+is by slots and by an HBM budget: creating a design context charges its
+program image and its slot state; when the budget is short, idle design
+contexts are evicted (least recently used) before a job waits.
+
+The reference has no serving path (one simulator per process, global device
+state, src/ToucanSim.cu:102-204); this module is the batching/serving layer
+the C-ABI contexts make possible.
+"""
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+
+KERNEL_BATCH = _native.KERNEL_BATCH
+POOL_REG, POOL_MEM, POOL_EXCHANGE = _native.POOL_REG, _native.POOL_MEM, _native.POOL_EXCHANGE
+
+
+@dataclass
+class Job:
+    job_id: int
+    design: str          # key into the engine's design table
+    instances: int
+    cycles: int
+    seed: int
+    arrival: float = 0.0  # seconds after the engine's start
+    # filled by the engine
+    admitted: float = None
+    finished: float = None
+    slots: list = field(default_factory=list)
+    remaining: np.ndarray = None
+    reg: list = None      # per instance: final register pool
+    mem: list = None      # per instance: final memory pool
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+def gpu_backend(design, kernel=KERNEL_BATCH):
+    """A batched simulator context for a generated design (gen.Design)."""
+    from . import rt
+    sim = rt.Simulator(0)
+    if sim.init(0, design.design, design.symbols, False, False) != 0:
+        raise RuntimeError(f"cannot load design: {sim._err()}")
+    sim.set_kernel(kernel)
+    return sim
+
+
+class _Ctx:
+    """One resident design: a batched context with `capacity` instance slots."""
+
+    def __init__(self, key, design, sim, capacity, nbytes):
+        self.key, self.design, self.sim, self.capacity, self.nbytes = key, design, sim, capacity, nbytes
+        self.free = list(range(capacity))
+        self.jobs = []
+        self.last_used = 0.0
+
+
+class Engine:
+    """Continuous-batching job engine.
+
+    designs:   {name: gen.Design} (program, symbols, stimulus input ids)
+    budget:    HBM bytes the engine may hold resident (programs + state)
+    capacity:  instance slots per design context (a multiple of 64 is best)
+    slice:     cycles per step for each active slot
+    backend:   factory design -> simulator (gpu_backend by default; tests
+               pass a host stand-in with the same methods)
+    """
+
+    def __init__(self, designs, budget=250 << 30, capacity=128, slice_cycles=2000, backend=gpu_backend,
+                 clock=time.perf_counter):
+        self.designs = designs
+        self.budget = budget
+        self.capacity = capacity
+        self.slice = slice_cycles
+        self.backend = backend
+        self.clock = clock
+        self.ctx = {}
+        self.used = 0
+        self.queue = []          # admitted-order queue of waiting jobs
+        self.done = []
+        self.instance_cycles = 0
+        self.steps = 0
+        self.launches = 0
+        self._t0 = None
+
+    # ---- HBM accounting -------------------------------------------------
+    @staticmethod
+    def state_bytes(design):
+        """Per-instance resident state: register, memory and exchange pools."""
+        st = design.stats
+        return int(st["reg_pool"]) + int(st["mem_pool"]) + int(st["exchange_pool"]) + 64
+
+    def context_bytes(self, design):
+        """Program image (about the serialized size) plus all slots' state."""
+        return int(design.stats["design_bytes"]) + self.capacity * self.state_bytes(design)
+
+    def _make_room(self, need, now):
+        while self.used + need > self.budget:
+            idle = [c for c in self.ctx.values() if not c.jobs]
+            if not idle:
+                return False
+            victim = min(idle, key=lambda c: c.last_used)
+            victim.sim.cleanup()
+            self.used -= victim.nbytes
+            del self.ctx[victim.key]
+        return True
+
+    def _context(self, key, now):
+        c = self.ctx.get(key)
+        if c is not None:
+            return c
+        d = self.designs[key]
+        nbytes = self.context_bytes(d)
+        if nbytes > self.budget:
+            raise ValueError(f"design {key} alone needs {nbytes} B, more than the budget {self.budget} B")
+        if not self._make_room(nbytes, now):
+            return None
+        sim = self.backend(d)
+        sim.set_instances(self.capacity)
+        sim.set_stimulus(d.inputs, 0)
+        c = _Ctx(key, d, sim, self.capacity, nbytes)
+        self.ctx[key] = c
+        self.used += nbytes
+        return c
+
+    # ---- scheduling ---------------------------------------------------------
+    def submit(self, job):
+        if job.instances < 1 or job.instances > self.capacity:
+            raise ValueError(f"job {job.job_id}: 1..{self.capacity} instances per job")
+        if job.cycles < 1:
+            raise ValueError(f"job {job.job_id}: at least one cycle")
+        self.queue.append(job)
+
+    def _admit(self, now):
+        waiting = []
+        for job in self.queue:
+            if job.arrival > now:
+                waiting.append(job)
+                continue
+            c = self._context(job.design, now)
+            if c is None or len(c.free) < job.instances:
+                waiting.append(job)
+                continue
+            job.slots = [c.free.pop(0) for _ in range(job.instances)]
+            for i, s in enumerate(job.slots):
+                c.sim.reset_instance(s, job.seed, i)
+            job.remaining = np.full(job.instances, job.cycles, np.int64)
+            job.admitted = now
+            c.jobs.append(job)
+        self.queue = waiting
+
+    def step(self):
+        """Admit arrived jobs, run one slice of every design with active slots,
+        retire finished jobs. Returns the number of jobs still pending."""
+        if self._t0 is None:
+            self._t0 = self.clock()
+        now = self.clock() - self._t0
+        self._admit(now)
+        for c in list(self.ctx.values()):
+            if not c.jobs:
+                continue
+            cyc = np.zeros(c.capacity, np.uint32)
+            for job in c.jobs:
+                n = np.minimum(job.remaining, self.slice)
+                cyc[job.slots] = n
+                job.remaining -= n
+            c.sim.run_batch(cyc)
+            self.launches += 1
+            self.instance_cycles += int(cyc.sum())
+            t = self.clock() - self._t0
+            c.last_used = t
+            keep = []
+            for job in c.jobs:
+                if job.remaining.max() > 0:
+                    keep.append(job)
+                    continue
+                job.reg = [c.sim.read_pool(POOL_REG, s) for s in job.slots]
+                job.mem = [c.sim.read_pool(POOL_MEM, s) for s in job.slots]
+                job.finished = self.clock() - self._t0
+                c.free.extend(job.slots)
+                c.free.sort()
+                self.done.append(job)
+            c.jobs = keep
+        self.steps += 1
+        return len(self.queue) + sum(len(c.jobs) for c in self.ctx.values())
+
+    def run(self, jobs, idle_sleep=0.0005):
+        """Serve `jobs` (arrival times relative to the start) to completion."""
+        for j in sorted(jobs, key=lambda j: j.arrival):
+            self.submit(j)
+        self._t0 = self.clock()
+        while True:
+            pending = self.step()
+            if pending == 0:
+                break
+            if not any(c.jobs for c in self.ctx.values()):
+                # nothing on the device: wait for the next arrival
+                nxt = min((j.arrival for j in self.queue), default=None)
+                now = self.clock() - self._t0
+                if nxt is not None and nxt > now:
+                    time.sleep(min(nxt - now, 0.05))
+                elif nxt is not None:
+                    # arrived but not admissible (budget held by running jobs is impossible here)
+                    time.sleep(idle_sleep)
+        return self.report()
+
+    def report(self):
+        lat = np.array([j.latency for j in self.done]) if self.done else np.zeros(0)
+        wall = (self.clock() - self._t0) if self._t0 is not None else 0.0
+        return {
+            "jobs": len(self.done),
+            "p50_latency_s": float(np.percentile(lat, 50)) if lat.size else None,
+            "p99_latency_s": float(np.percentile(lat, 99)) if lat.size else None,
+            "mean_latency_s": float(lat.mean()) if lat.size else None,
+            "instance_cycles": self.instance_cycles,
+            "seconds": wall,
+            "cycles_per_s": self.instance_cycles / wall if wall > 0 else None,
+            "launches": self.launches,
+            "steps": self.steps,
+            "resident_bytes": self.used,
+        }
+
+    def close(self):
+        for c in self.ctx.values():
+            c.sim.cleanup()
+        self.ctx.clear()
+        self.used = 0
+
+
+def run_alone(design, job, backend=gpu_backend):
+    """The reference result of a job: its instances alone in a fresh context."""
+    sim = backend(design)
+    sim.set_instances(job.instances)
+    sim.set_stimulus(design.inputs, job.seed)
+    sim.run_batch(np.full(job.instances, job.cycles, np.uint32))
+    reg = [sim.read_pool(POOL_REG, i) for i in range(job.instances)]
+    mem = [sim.read_pool(POOL_MEM, i) for i in range(job.instances)]
+    sim.cleanup()
+    return reg, mem
+
+
+def poisson_jobs(design_names, n_jobs, rate, rng, min_inst=1, max_inst=64, min_cycles=1000, max_cycles=100_000):
+    """Poisson arrivals at `rate` jobs/s over the designs, instances and cycle
+    counts uniform in the given ranges."""
+    t = 0.0
+    jobs = []
+    for k in range(n_jobs):
+        t += rng.exponential(1.0 / rate)
+        jobs.append(Job(job_id=k, design=design_names[int(rng.integers(len(design_names)))],
+                        instances=int(rng.integers(min_inst, max_inst + 1)),
+                        cycles=int(rng.integers(min_cycles, max_cycles + 1)), seed=int(rng.integers(1 << 31)),
+                        arrival=t))
+    return jobs
+
+
+__all__ = ["Job", "Engine", "gpu_backend", "run_alone", "poisson_jobs"]
