@@ -12,9 +12,19 @@ steps cover the 100,000 cycles. Inputs are resident in HBM before timing.
 Batched leg: the 200,000-LUT4 design (depth 48, 8 SRAMs of 1024 x 32 bit)
 run as --instances instances (default 1,024) with distinct xorshift64 seeds
 and cycle counts drawn uniformly from [1,000, 50,000], in one persistent
-launch. An autotuner first sweeps the partition size (256..4,096 LUT4s) and
-the workgroup size on a short run; the logical design is identical for every
+launch per slice of --slice cycles. An autotuner first sweeps the partition
+size (256..4,096 LUT4s) and the workgroup size on a short run; the logical design is identical for every
 partition size, so results do not depend on its choice.
+
+Serving leg: --serve-designs seeded designs of log-uniform size, jobs of 1-64
+instances with Poisson arrivals, served by the continuous-batching engine
+(toucangpurt_amd.serve) under an HBM budget; reports p50/p99 job latency
+(arrival to result) and aggregate instance-cycles/s. The default sizes and
+cycle counts are scaled down from the 50K-5M LUT4 / 1K-100K cycle serving
+workload so the leg runs in about a minute; the flags restore the full one.
+
+The single leg picks the faster of the streamed and the wave64 persistent
+kernels on a short run (--kernel names one).
 
 Baselines: the plain kernel (reference execution model) on the same design,
 timed on a bounded cycle count (its rate is per cycle), and the C oracle on
@@ -38,7 +48,18 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["single", "batched", "both"], default="both")
+    ap.add_argument("--mode", choices=["single", "batched", "serve", "both", "all"], default="all",
+                    help="all = single + batched + serving legs")
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals, jobs/s")
+    ap.add_argument("--serve-designs", type=int, default=8)
+    ap.add_argument("--serve-min-luts", type=int, default=50_000)
+    ap.add_argument("--serve-max-luts", type=int, default=1_000_000)
+    ap.add_argument("--serve-min-cycles", type=int, default=100)
+    ap.add_argument("--serve-max-cycles", type=int, default=2_000)
+    ap.add_argument("--serve-slice", type=int, default=500)
+    ap.add_argument("--serve-capacity", type=int, default=128)
+    ap.add_argument("--serve-budget-gb", type=float, default=250.0)
     ap.add_argument("--chunk", type=int, default=10_000, help="cycles per step (single leg)")
     ap.add_argument("--kernel", choices=["auto", "stream", "wave64"], default="auto", help="single-leg kernel")
     ap.add_argument("--tune-single", type=int, default=2_000, help="cycles per kernel in the single-leg choice")
@@ -283,15 +304,70 @@ def run_batched(args):
     return out
 
 
+def run_serve(args):
+    """Serving leg: Poisson arrivals over seeded designs of log-uniform size,
+    continuous batching over resident batched contexts, HBM-budget admission."""
+    import numpy as np
+    from toucangpurt_amd import gen
+    from toucangpurt_amd.serve import Engine, poisson_jobs
+    rng = np.random.default_rng(args.seed + 7)
+    sizes = np.exp(rng.uniform(np.log(args.serve_min_luts), np.log(args.serve_max_luts), args.serve_designs))
+    designs = {}
+    t0 = time.perf_counter()
+    for k, n in enumerate(sizes):
+        n = int(n)
+        designs[f"d{k}"] = gen.generate(None, n_luts=n, depth=32, n_regs=max(64, n // 20), n_inputs=64,
+                                        part_luts=256, seed=args.seed + 100 + k)
+    log(f"serve: {len(designs)} designs generated in {time.perf_counter() - t0:.1f} s "
+        f"({', '.join(str(int(s)) for s in sorted(sizes))} LUT4)")
+    jobs = poisson_jobs(list(designs), args.serve_jobs, args.serve_rate, rng,
+                        min_cycles=args.serve_min_cycles, max_cycles=args.serve_max_cycles)
+    eng = Engine(designs, budget=int(args.serve_budget_gb * (1 << 30)), capacity=args.serve_capacity,
+                 slice_cycles=args.serve_slice)
+    done = [0]
+
+    def progress():
+        n = len(eng.done)
+        if n >= done[0] + 8:
+            done[0] = n
+            log(f"serve: {n} of {len(jobs)} jobs done, {eng.instance_cycles} instance-cycles")
+
+    orig = eng.step
+
+    def step():
+        r = orig()
+        progress()
+        return r
+    eng.step = step
+    rep = eng.run(jobs)
+    eng.close()
+    rep.update({"workload": f"{args.serve_designs} designs log-uniform {args.serve_min_luts}-{args.serve_max_luts} "
+                            f"LUT4 (depth 32, partitions of 256), {args.serve_jobs} jobs, Poisson "
+                            f"{args.serve_rate}/s, 1-64 instances x {args.serve_min_cycles}-{args.serve_max_cycles} "
+                            f"cycles, {args.serve_budget_gb} GB budget, {args.serve_capacity} slots per design, "
+                            f"slice {args.serve_slice} cycles",
+                "unit": "instance-cycles/s"})
+    log(f"serve: p50 {rep['p50_latency_s']:.3f} s p99 {rep['p99_latency_s']:.3f} s, "
+        f"{rep['cycles_per_s']:.3e} instance-cycles/s")
+    return rep
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is described "
               "in DESIGN.md and not built", file=sys.stderr)
         sys.exit(2)
-    single = run_single(args) if args.mode in ("single", "both") else None
-    batched = run_batched(args) if args.mode in ("batched", "both") else None
-    if single is not None:
+    single = run_single(args) if args.mode in ("single", "both", "all") else None
+    batched = run_batched(args) if args.mode in ("batched", "both", "all") else None
+    serving = run_serve(args) if args.mode in ("serve", "all") else None
+    if single is None and batched is None:
+        out = {"metric": "serving p50/p99 job latency and instance-cycles/s", "value": serving["cycles_per_s"],
+               "unit": "instance-cycles/s", "n_gpus": 1, "steps": serving["steps"], "warmup": 0,
+               "ms_per_step": serving["seconds"] / max(serving["steps"], 1) * 1e3, "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+               "config": {"workload": serving["workload"]}, "serving": serving}
+    elif single is not None:
         out = {"metric": "simulated cycles/sec @1M-LUT design; batched stimulus-cycles/sec vs plain",
                "value": single["value"], "unit": "simulated cycles/s", "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": single["dt"] / args.steps * 1e3, "higher_is_better": True,
@@ -300,7 +376,8 @@ def main():
                "barrier_stall_frac": single["barrier_stall_frac"],
                "plain_kernel": single.get("plain"),
                "speedup_vs_plain": (single["value"] / single["plain"]["value"]) if single.get("plain") else None,
-               "cpu_baseline": single.get("cpu"), "parity_vs_oracle": single.get("parity"), "batched": batched}
+               "cpu_baseline": single.get("cpu"), "parity_vs_oracle": single.get("parity"), "batched": batched,
+               "serving": serving}
     else:
         out = {"metric": "batched stimulus-cycles/sec @200K-LUT design x instances", "value": batched["value"],
                "unit": "stimulus-cycles/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -308,7 +385,7 @@ def main():
                "vs_baseline": None, "dtype": "u8", "data": "synthetic",
                "config": {"workload": batched["workload"], "part_luts": batched["part_luts"]},
                "roofline": batched["roofline"], "cpu_baseline": batched.get("cpu"),
-               "speedup_vs_plain": batched.get("speedup_vs_plain"), "batched": batched}
+               "speedup_vs_plain": batched.get("speedup_vs_plain"), "batched": batched, "serving": serving}
     print(json.dumps(out))
 
 
