@@ -56,7 +56,10 @@ class ToucanSimulator {
   std::vector<uint32_t> vcdWidth;
   std::vector<std::string> vcdIds;
   std::vector<std::vector<uint8_t>> vcdLast;
+  bool vcdPending = false;       // a snapshot started but not yet written
+  uint64_t vcdTicket = 0, vcdStamp = 0;
   void dumpVcdWorker(uint64_t cycle);
+  void writeVcdSnapshot(uint64_t ticket, uint64_t stamp);
   void flushPrints();
   uint64_t printsSeen = 0;
 };

@@ -143,6 +143,17 @@ int tgs_pool_size(const tgs_ctx *ctx, int pool, uint64_t *size);
  * instance and no counterpart (ToucanSim.cu:102-204 reloads everything). */
 int tgs_reset_instance(tgs_ctx *ctx, uint32_t instance, uint64_t seed, uint32_t index);
 
+/* SimVCD (unsupport/SimVCD.cpp:193-241) dump source: start an asynchronous
+ * snapshot of one instance's register pool. The copy is ordered after the
+ * cycles already run; it goes to a device staging buffer on the context
+ * stream and to pinned host memory on a side stream, so it overlaps the next
+ * eval. Up to 4 snapshots are in flight; a snapshot's bytes stay valid until
+ * the fourth snapshot after it is started. */
+int tgs_snapshot_begin(tgs_ctx *ctx, uint32_t instance, uint64_t *ticket);
+/* Wait for a snapshot; *data points to regPoolSize bytes of pinned memory,
+ * *cycle (optional) is the instance's cycle count when it was taken. */
+int tgs_snapshot_wait(tgs_ctx *ctx, uint64_t ticket, const uint8_t **data, uint64_t *cycle);
+
 /* get_eval_done (src/SimEval.cu:1083-1087) and get_real_cycles (:1089-1093). */
 int tgs_get_stop(tgs_ctx *ctx, uint32_t instance, int *stopped);
 int tgs_get_cycles(tgs_ctx *ctx, uint32_t instance, uint64_t *cycles);

@@ -11,6 +11,22 @@ for p in (ROOT, os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libtoucan_rt.so on the device)")
+    config.addinivalue_line("markers", "subprocess_gpu: GPU work runs in a child process (collected first)")
+
+
+def pytest_collection_modifyitems(session, config, items):
+    """Run the tests that drive the GPU from a child process first.
+
+    The persistent kernels are cooperative launches; once this process has
+    made one it holds the GPU's cooperative queue for its lifetime, and a
+    cooperative launch from another process on the same GPU is then refused
+    (hipErrorIllegalState). Tests whose GPU work runs in a subprocess
+    (the C++ simulator driver) therefore go before any in-process GPU test.
+    """
+    first = [it for it in items if it.get_closest_marker("subprocess_gpu")]
+    if first:
+        rest = [it for it in items if not it.get_closest_marker("subprocess_gpu")]
+        items[:] = first + rest
 
 
 @pytest.fixture(scope="session")

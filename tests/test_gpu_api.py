@@ -11,7 +11,7 @@ from simharness import OracleRun
 from toucangpurt_amd import gen
 from toucangpurt_amd.rt import parse_symbols
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.subprocess_gpu]
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 

@@ -55,6 +55,30 @@ struct DBuf {
   }
 };
 
+// Waveform snapshots (SimVCD): the register pool of one instance is copied
+// to a device staging slot on the context stream, ordered after the cycle
+// launches already queued, and from there to pinned host memory on a side
+// stream, so the device-to-host copy overlaps the next launches.
+struct SnapRing {
+  hipStream_t side = nullptr;
+  uint32_t slots = 0;
+  uint64_t bytes = 0;
+  std::vector<void *> host;  // pinned (hipHostMalloc)
+  std::vector<hipEvent_t> staged, done;
+  std::vector<uint64_t> ticketOf, cycleOf;
+  uint64_t next = 0;
+  void *staging = nullptr;   // slots * bytes of device memory
+  void release() {
+    if (side) (void)hipStreamSynchronize(side);
+    for (void *h : host) (void)hipHostFree(h);
+    for (auto e : staged) (void)hipEventDestroy(e);
+    for (auto e : done) (void)hipEventDestroy(e);
+    if (staging) (void)hipFree(staging);
+    if (side) (void)hipStreamDestroy(side);
+    *this = SnapRing();
+  }
+};
+
 struct Instance {
   DBuf reg, mem, exg, xs, ctl, logCycle, logMsg;
   uint64_t cycles = 0;
@@ -129,6 +153,7 @@ struct tgs_ctx {
   uint32_t streamNC = 0, streamNCUsed = 4, streamSlots = 1, streamGroups = 0, streamGrid = 0, streamLds = 0;
 
   tgs_profile prof{};
+  SnapRing snap;
 
   int fail(int code, const char *fmt, ...) {
     char buf[1024];
@@ -875,6 +900,7 @@ void tgs_destroy(tgs_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->snap.release();
   c->inst.clear();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1174,6 +1200,66 @@ int tgs_reset_instance(tgs_ctx *c, uint32_t instance, uint64_t seed, uint32_t in
   HIPCHK(c, hipMemcpy(b.stop.as<uint64_t>() + s, &zero, 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(b.stopCycle.as<uint64_t>() + s, &never, 8, hipMemcpyHostToDevice));
   b.cycles[instance] = 0;
+  return TGS_OK;
+}
+
+int tgs_snapshot_begin(tgs_ctx *c, uint32_t instance, uint64_t *ticket) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if (!ticket || instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "bad instance or pointer");
+  if ((rc = setDevice(c))) return rc;
+  auto &s = c->snap;
+  const uint64_t n = c->design.regPoolSize;
+  if (s.slots == 0 || s.bytes < n) {
+    s.release();
+    s.slots = 4;
+    s.bytes = (n + 255) & ~uint64_t(255);
+    HIPCHK(c, hipStreamCreateWithFlags(&s.side, hipStreamNonBlocking));
+    HIPCHK(c, hipMalloc(&s.staging, s.slots * std::max<uint64_t>(s.bytes, 256)));
+    s.host.assign(s.slots, nullptr);
+    s.staged.assign(s.slots, nullptr);
+    s.done.assign(s.slots, nullptr);
+    s.ticketOf.assign(s.slots, UINT64_MAX);
+    s.cycleOf.assign(s.slots, 0);
+    for (uint32_t k = 0; k < s.slots; ++k) {
+      HIPCHK(c, hipHostMalloc(&s.host[k], std::max<uint64_t>(s.bytes, 256), hipHostMallocDefault));
+      HIPCHK(c, hipEventCreateWithFlags(&s.staged[k], hipEventDisableTiming));
+      HIPCHK(c, hipEventCreateWithFlags(&s.done[k], hipEventDisableTiming));
+    }
+  }
+  const uint32_t k = uint32_t(s.next % s.slots);
+  if (s.ticketOf[k] != UINT64_MAX) HIPCHK(c, hipEventSynchronize(s.done[k]));  // slot still in flight
+  uint8_t *stage = static_cast<uint8_t *>(s.staging) + k * s.bytes;
+  if (n) {
+    if (c->batched) {
+      HIPCHK(c, launchSlotColumn(c->bs.reg.as<uint8_t>(), c->bs.instPad, c->bs.slotOf[instance], n, stage, false,
+                                 c->stream));
+    } else {
+      HIPCHK(c, hipMemcpyAsync(stage, c->inst[instance]->reg.p, n, hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  HIPCHK(c, hipEventRecord(s.staged[k], c->stream));
+  HIPCHK(c, hipStreamWaitEvent(s.side, s.staged[k], 0));
+  if (n) HIPCHK(c, hipMemcpyAsync(s.host[k], stage, n, hipMemcpyDeviceToHost, s.side));
+  HIPCHK(c, hipEventRecord(s.done[k], s.side));
+  s.ticketOf[k] = s.next;
+  s.cycleOf[k] = c->batched ? c->bs.cycles[instance] : c->inst[instance]->cycles;
+  *ticket = s.next++;
+  return TGS_OK;
+}
+
+int tgs_snapshot_wait(tgs_ctx *c, uint64_t ticket, const uint8_t **data, uint64_t *cycle) {
+  if (!c || !data) return TGS_E_ARG;
+  auto &s = c->snap;
+  if (s.slots == 0) return c->fail(TGS_E_STATE, "no snapshot was started");
+  const uint32_t k = uint32_t(ticket % s.slots);
+  if (s.ticketOf[k] != ticket) return c->fail(TGS_E_STATE, "snapshot %llu was overwritten or never started",
+                                               (unsigned long long)ticket);
+  int rc = setDevice(c);
+  if (rc) return rc;
+  HIPCHK(c, hipEventSynchronize(s.done[k]));
+  *data = static_cast<const uint8_t *>(s.host[k]);
+  if (cycle) *cycle = s.cycleOf[k];
   return TGS_OK;
 }
 

@@ -220,14 +220,31 @@ void ToucanSimulator::enableVCDDump(const std::string vcdFilename) {
   *vcdOfs << "$enddefinitions $end\n$dumpvars\n";
 }
 
-// unsupport/SimVCD.cpp:193-241: change-only values per dumped cycle.
+// unsupport/SimVCD.cpp:193-241: change-only values per dumped cycle. The
+// dump is pipelined: this cycle's register pool is snapshotted
+// asynchronously (pinned copy on a side stream) and the previous snapshot is
+// written, so the copy and the formatting overlap the next eval.
 void ToucanSimulator::dumpVcdWorker(uint64_t cycle) {
   if (!vcdOfs) return;
-  uint64_t regN = 0;
-  tgs_pool_size(ctx, TGS_POOL_REG, &regN);
-  std::vector<uint8_t> pool(regN);
-  tgs_read_pool(ctx, 0, TGS_POOL_REG, 0, regN, pool.data());
-  *vcdOfs << "#" << cycle << "\n";
+  uint64_t t = 0;
+  if (tgs_snapshot_begin(ctx, 0, &t) != TGS_OK) {
+    std::cerr << "VCD snapshot failed: " << lastError() << std::endl;
+    return;
+  }
+  if (vcdPending) writeVcdSnapshot(vcdTicket, vcdStamp);
+  vcdPending = true;
+  vcdTicket = t;
+  vcdStamp = cycle;
+}
+
+void ToucanSimulator::writeVcdSnapshot(uint64_t ticket, uint64_t stamp) {
+  vcdPending = false;
+  const uint8_t *pool = nullptr;
+  if (tgs_snapshot_wait(ctx, ticket, &pool, nullptr) != TGS_OK) {
+    std::cerr << "VCD snapshot failed: " << lastError() << std::endl;
+    return;
+  }
+  *vcdOfs << "#" << stamp << "\n";
   for (uint32_t i = 0; i < vcdRegs.size(); ++i) {
     std::vector<uint8_t> v;
     for (auto &f : vcdRegs[i]) v.push_back(pool[std::get<0>(f)]);
@@ -248,6 +265,7 @@ void ToucanSimulator::dumpVcdWorker(uint64_t cycle) {
 
 void ToucanSimulator::cleanup() {
   if (vcdOfs) {
+    if (vcdPending && ctx) writeVcdSnapshot(vcdTicket, vcdStamp);
     vcdOfs->close();
     delete vcdOfs;
     vcdOfs = nullptr;

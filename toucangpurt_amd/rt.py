@@ -93,9 +93,9 @@ class Simulator:
         """One cycle (src/ToucanSim.cu:206-230); returns the sticky stop flag."""
         st = ctypes.c_int()
         self._check(self._lib.tgs_eval(self._h, ctypes.byref(st)))
-        self.cycle_cnt += 1
         if self._vcd is not None:
             self._vcd.dump(self.cycle_cnt)
+        self.cycle_cnt += 1
         return bool(st.value)
 
     def eval_free_running(self, max_cycles):
@@ -103,6 +103,8 @@ class Simulator:
         st = ctypes.c_int()
         self._check(self._lib.tgs_eval_free_running(self._h, int(max_cycles), ctypes.byref(st)))
         self.cycle_cnt += int(max_cycles)
+        if self._vcd is not None:
+            self._vcd.dump(self.cycle_cnt)
         return bool(st.value)
 
     def setRegister(self, name, value, instance=0):
@@ -140,7 +142,9 @@ class Simulator:
         return (1 << 64) - 1
 
     def enableVCDDump(self, path):
-        """VCD writer (unsupport/SimVCD.cpp:99-241) over register snapshots after each eval()."""
+        """VCD writer (unsupport/SimVCD.cpp:99-241) over asynchronous register snapshots,
+        one per eval() (stamped with the cycle before it) and one after each
+        eval_free_running() (stamped with the cycle count after it), as the C++ class."""
         from .vcd import VcdWriter
         self._vcd = VcdWriter(self, path)
 
@@ -211,6 +215,20 @@ class Simulator:
         """Restart one instance from the initial state with stimulus generator
         splitmix64(seed + index) (a slot reused for a new job)."""
         self._check(self._lib.tgs_reset_instance(self._h, int(instance), int(seed), int(index)))
+
+    def snapshot_begin(self, instance=0):
+        """Start an asynchronous register-pool snapshot; returns its ticket."""
+        t = ctypes.c_uint64()
+        self._check(self._lib.tgs_snapshot_begin(self._h, int(instance), ctypes.byref(t)))
+        return t.value
+
+    def snapshot_wait(self, ticket):
+        """(register pool copy, cycle) of a started snapshot."""
+        ptr = _u8p()
+        cyc = ctypes.c_uint64()
+        self._check(self._lib.tgs_snapshot_wait(self._h, int(ticket), ctypes.byref(ptr), ctypes.byref(cyc)))
+        n = self.pool_size(POOL_REG)
+        return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n else np.zeros(0, np.uint8), cyc.value
 
     def cycles(self, instance=0):
         c = ctypes.c_uint64()
