@@ -154,6 +154,20 @@ int tgs_snapshot_begin(tgs_ctx *ctx, uint32_t instance, uint64_t *ticket);
  * *cycle (optional) is the instance's cycle count when it was taken. */
 int tgs_snapshot_wait(tgs_ctx *ctx, uint64_t ticket, const uint8_t **data, uint64_t *cycle);
 
+/* State checkpoint of one instance (no reference counterpart: the reference
+ * keeps one global instance and re-runs init, src/ToucanSim.cu:102-204).
+ * The image is a 64-byte header (magic "TGSCKPT1", FNV-1a 64 of the
+ * serialized design, pool sizes, stop flag, cycle count, stop cycle,
+ * stimulus generator state) followed by the register, memory and exchange
+ * pools, nibble per byte. It is the same in every kernel layout, so an
+ * instance saved from a batched context restores into a single-instance
+ * context of the same design and the other way round. The print log is
+ * output, not state, and is not part of it. Restoring an image of another
+ * design fails with TGS_E_ARG; a damaged one with TGS_E_FORMAT. */
+int tgs_checkpoint_size(tgs_ctx *ctx, uint64_t *bytes);
+int tgs_checkpoint_save(tgs_ctx *ctx, uint32_t instance, uint8_t *buf, uint64_t bytes);
+int tgs_checkpoint_restore(tgs_ctx *ctx, uint32_t instance, const uint8_t *buf, uint64_t bytes);
+
 /* get_eval_done (src/SimEval.cu:1083-1087) and get_real_cycles (:1089-1093). */
 int tgs_get_stop(tgs_ctx *ctx, uint32_t instance, int *stopped);
 int tgs_get_cycles(tgs_ctx *ctx, uint32_t instance, uint64_t *cycles);

@@ -11,6 +11,7 @@
  */
 #include "oracle.h"
 
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -633,20 +634,134 @@ static void eval_partition(orc_state *s, uint32_t partId, uint32_t epoch, int en
   }
 }
 
+static void apply_stimulus(orc_state *s) {
+  for (uint32_t d = 0; d * 64 < s->nIn; ++d) {
+    s->xs = orc_xorshift64(s->xs);
+    for (uint32_t j = 0; j < 64 && d * 64 + j < s->nIn; ++j)
+      s->reg[s->inIds[d * 64 + j]] = (uint8_t)((s->xs >> j) & 1);
+  }
+}
+
 int orc_run(orc_state *s, uint32_t cycles, int enable_print) {
   const orc_prog *p = s->p;
   for (uint32_t c = 0; c < cycles; ++c) {
     /* stimulus before the cycle */
-    for (uint32_t d = 0; d * 64 < s->nIn; ++d) {
-      s->xs = orc_xorshift64(s->xs);
-      for (uint32_t j = 0; j < 64 && d * 64 + j < s->nIn; ++j)
-        s->reg[s->inIds[d * 64 + j]] = (uint8_t)((s->xs >> j) & 1);
-    }
+    apply_stimulus(s);
     /* evalFreeRunningNCycles (SimEval.cu:889-995): region 0, barrier, region 1, barrier */
     uint32_t e0 = (uint32_t)(2 * (s->cycles & 0x7FFFFFFF) + 1);
     for (uint32_t i = 0; i < p->nRegion[0]; ++i) eval_partition(s, i, e0, enable_print);
     for (uint32_t i = p->nRegion[0]; i < p->nParts; ++i) eval_partition(s, i, e0 + 1, enable_print);
     s->cycles++;
   }
+  return s->stop;
+}
+
+/* ------------------------------------------------------------------ */
+/* multi-threaded run: the partitions of a region are taken one at a time */
+/* from a shared counter by `threads` threads, with a barrier between    */
+/* regions as the reference's grid sync (SimEval.cu:889-995). Each      */
+/* thread has its own value pool and undefined-behaviour counters; the  */
+/* pools are shared. Partitions of one region write disjoint pool words, */
+/* so the pools, stop flag and stop cycle equal orc_run's; print entries */
+/* of one cycle may come in another order. The cross-partition memory-  */
+/* sharing counter (viol[1]) is only exact single-threaded.             */
+
+typedef struct {
+  orc_state w;         /* shallow copy of the main state with private scratch */
+  orc_state *main;
+  uint32_t tid, nThreads, cycles;
+  int enablePrint;
+  pthread_barrier_t *bar;
+  uint32_t *next;      /* shared: next partition of region 0 / region 1 */
+} orc_worker;
+
+static void merge_cycle(orc_worker *ws, uint32_t n) {
+  orc_state *m = ws[0].main;
+  for (uint32_t t = 0; t < n; ++t) {
+    orc_state *w = &ws[t].w;
+    if (w->stop && !m->stop) { m->stop = 1; m->stopCycle = m->cycles; }
+    w->stop = 0;
+    for (uint64_t i = 0; i < w->nLog; ++i) {
+      if (m->nLog == m->capLog) {
+        m->capLog = m->capLog ? 2 * m->capLog : 1024;
+        m->logCycle = realloc(m->logCycle, m->capLog * 8);
+        m->logMsg = realloc(m->logMsg, m->capLog * 4);
+        if (!m->logCycle || !m->logMsg) abort();
+      }
+      m->logCycle[m->nLog] = w->logCycle[i];
+      m->logMsg[m->nLog] = w->logMsg[i];
+      m->nLog++;
+    }
+    w->nLog = 0;
+  }
+  m->cycles++;
+  ws[0].next[0] = 0;
+  ws[0].next[1] = m->p->nRegion[0];
+}
+
+static void *worker_main(void *arg) {
+  orc_worker *me = (orc_worker *)arg;
+  orc_worker *all = me - me->tid;
+  orc_state *m = me->main;
+  const orc_prog *p = m->p;
+  for (uint32_t c = 0; c < me->cycles; ++c) {
+    if (me->tid == 0) apply_stimulus(m);
+    pthread_barrier_wait(me->bar);
+    me->w.cycles = m->cycles;
+    uint32_t e0 = (uint32_t)(2 * (m->cycles & 0x7FFFFFFF) + 1);
+    for (;;) {
+      uint32_t i = __atomic_fetch_add(&me->next[0], 1, __ATOMIC_RELAXED);
+      if (i >= p->nRegion[0]) break;
+      eval_partition(&me->w, i, e0, me->enablePrint);
+    }
+    pthread_barrier_wait(me->bar);
+    for (;;) {
+      uint32_t i = __atomic_fetch_add(&me->next[1], 1, __ATOMIC_RELAXED);
+      if (i >= p->nParts) break;
+      eval_partition(&me->w, i, e0 + 1, me->enablePrint);
+    }
+    pthread_barrier_wait(me->bar);
+    if (me->tid == 0) merge_cycle(all, me->nThreads);
+  }
+  return NULL;
+}
+
+int orc_run_threads(orc_state *s, uint32_t cycles, int enable_print, int threads) {
+  if (threads <= 1) return orc_run(s, cycles, enable_print);
+  uint32_t n = (uint32_t)threads;
+  orc_worker *ws = xcalloc(n, sizeof(orc_worker));
+  pthread_t *th = xcalloc(n, sizeof(pthread_t));
+  pthread_barrier_t bar;
+  pthread_barrier_init(&bar, NULL, n);
+  uint32_t next[2] = {0, s->p->nRegion[0]};
+  for (uint32_t t = 0; t < n; ++t) {
+    ws[t].w = *s;
+    ws[t].w.vp = xcalloc(65536 + 64, 1);
+    ws[t].w.written = xcalloc(65536 + 64, 4);
+    ws[t].w.visit = 0;
+    memset(ws[t].w.viol, 0, sizeof(ws[t].w.viol));
+    ws[t].w.nLog = ws[t].w.capLog = 0;
+    ws[t].w.logCycle = NULL;
+    ws[t].w.logMsg = NULL;
+    ws[t].w.stop = 0;
+    ws[t].main = s;
+    ws[t].tid = t;
+    ws[t].nThreads = n;
+    ws[t].cycles = cycles;
+    ws[t].enablePrint = enable_print;
+    ws[t].bar = &bar;
+    ws[t].next = next;
+  }
+  for (uint32_t t = 1; t < n; ++t)
+    if (pthread_create(&th[t], NULL, worker_main, &ws[t]) != 0) abort();
+  worker_main(&ws[0]);
+  for (uint32_t t = 1; t < n; ++t) pthread_join(th[t], NULL);
+  for (uint32_t t = 0; t < n; ++t) {
+    for (int k = 0; k < 4; ++k) s->viol[k] += ws[t].w.viol[k];
+    free(ws[t].w.vp); free(ws[t].w.written); free(ws[t].w.logCycle); free(ws[t].w.logMsg);
+  }
+  pthread_barrier_destroy(&bar);
+  free(ws);
+  free(th);
   return s->stop;
 }

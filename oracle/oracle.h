@@ -43,6 +43,9 @@ int orc_set_stimulus(orc_state *s, const uint32_t *ids, uint32_t n, uint64_t see
 
 /* Run `cycles` cycles. Returns the sticky stop flag. */
 int orc_run(orc_state *s, uint32_t cycles, int enable_print);
+/* The same with the partitions of each region split over `threads` threads
+ * and a barrier between regions; results equal orc_run's. */
+int orc_run_threads(orc_state *s, uint32_t cycles, int enable_print, int threads);
 uint64_t orc_cycles(const orc_state *s);
 int orc_stopped(const orc_state *s);
 uint64_t orc_stop_cycle(const orc_state *s);

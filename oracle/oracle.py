@@ -36,6 +36,7 @@ def lib():
         L.orc_state_copy_pool.argtypes = [vp, ctypes.c_int, ctypes.c_int, _u8p, ctypes.c_size_t]
         L.orc_set_stimulus.argtypes = [vp, _u32p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32]
         L.orc_run.argtypes = [vp, ctypes.c_uint32, ctypes.c_int]
+        L.orc_run_threads.argtypes = [vp, ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
         L.orc_cycles.argtypes = [vp]
         L.orc_cycles.restype = ctypes.c_uint64
         L.orc_stopped.argtypes = [vp]
@@ -110,7 +111,9 @@ class State:
         if rc != 0:
             raise ValueError("stimulus input id out of range")
 
-    def run(self, cycles, enable_print=False):
+    def run(self, cycles, enable_print=False, threads=1):
+        if threads > 1:
+            return bool(lib().orc_run_threads(self.h, cycles, 1 if enable_print else 0, threads))
         return bool(lib().orc_run(self.h, cycles, 1 if enable_print else 0))
 
     @property

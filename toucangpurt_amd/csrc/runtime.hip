@@ -125,6 +125,7 @@ struct tgs_ctx {
   SimDesignInfo design;
   SimDebugInfo symbols;
   std::vector<uint8_t> initReg, initMem;
+  uint64_t designHash = 0;  // FNV-1a of the serialized design (checkpoint identity)
   tgs_program_stats stats{};
   int kernel = TGS_KERNEL_WAVE64;
   int printEnable = 0;
@@ -937,6 +938,8 @@ int tgs_load(tgs_ctx *c, const uint8_t *design, size_t designSize, const uint8_t
   if (randomize) c->design.Randomize(0, c->symbols);
   c->initReg = c->design.regPool;
   c->initMem = c->design.memPool;
+  c->designHash = 0xCBF29CE484222325ull;
+  for (size_t i = 0; i < designSize; ++i) c->designHash = (c->designHash ^ design[i]) * 0x100000001B3ull;
   c->stim = StimMap();
   c->printEnable = enablePrint ? 1 : 0;
   programStats(c->design, 0, &c->stats);
@@ -1260,6 +1263,111 @@ int tgs_snapshot_wait(tgs_ctx *c, uint64_t ticket, const uint8_t **data, uint64_
   HIPCHK(c, hipEventSynchronize(s.done[k]));
   *data = static_cast<const uint8_t *>(s.host[k]);
   if (cycle) *cycle = s.cycleOf[k];
+  return TGS_OK;
+}
+
+// Checkpoint image: a fixed header, then the register, memory and exchange
+// pools of one instance (nibble per byte, instance-major whatever the
+// context's layout).
+namespace {
+struct CkptHeader {
+  char magic[8];
+  uint64_t designHash;
+  uint32_t regBytes, memBytes, exgBytes, stop;
+  uint64_t cycles, stopCycle, xs, reserved;
+};
+static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
+constexpr char kCkptMagic[8] = {'T', 'G', 'S', 'C', 'K', 'P', 'T', '1'};
+
+uint64_t ckptBytes(const SimDesignInfo &d) {
+  return sizeof(CkptHeader) + uint64_t(d.regPoolSize) + d.memPoolSize + d.exchangePoolSize;
+}
+}  // namespace
+
+int tgs_checkpoint_size(tgs_ctx *c, uint64_t *bytes) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if (!bytes) return TGS_E_ARG;
+  *bytes = ckptBytes(c->design);
+  return TGS_OK;
+}
+
+int tgs_checkpoint_save(tgs_ctx *c, uint32_t instance, uint8_t *buf, uint64_t bytes) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  const auto &d = c->design;
+  if (instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "instance %u out of range", instance);
+  if (!buf || bytes < ckptBytes(d)) return c->fail(TGS_E_ARG, "checkpoint buffer smaller than %llu bytes",
+                                                   (unsigned long long)ckptBytes(d));
+  if ((rc = setDevice(c))) return rc;
+  CkptHeader h{};
+  std::memcpy(h.magic, kCkptMagic, 8);
+  h.designHash = c->designHash;
+  h.regBytes = d.regPoolSize;
+  h.memBytes = d.memPoolSize;
+  h.exgBytes = d.exchangePoolSize;
+  uint8_t *p = buf + sizeof h;
+  if ((rc = poolIO(c, instance, TGS_POOL_REG, 0, h.regBytes, p, false))) return rc;
+  if ((rc = poolIO(c, instance, TGS_POOL_MEM, 0, h.memBytes, p + h.regBytes, false))) return rc;
+  if ((rc = poolIO(c, instance, TGS_POOL_EXCHANGE, 0, h.exgBytes, p + h.regBytes + h.memBytes, false))) return rc;
+  if (c->batched) {
+    const auto &b = c->bs;
+    const uint32_t s = b.slotOf[instance];
+    uint64_t st = 0;
+    HIPCHK(c, hipMemcpy(&h.xs, b.xs.as<uint64_t>() + s, 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&st, b.stop.as<uint64_t>() + s, 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(&h.stopCycle, b.stopCycle.as<uint64_t>() + s, 8, hipMemcpyDeviceToHost));
+    h.stop = st != 0;
+    h.cycles = b.cycles[instance];
+  } else {
+    Instance &in = *c->inst[instance];
+    uint32_t ctl[C_WORDS];
+    HIPCHK(c, hipMemcpy(&h.xs, in.xs.p, 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ctl, in.ctl.p, sizeof ctl, hipMemcpyDeviceToHost));
+    h.stop = ctl[C_STOP] != 0;
+    std::memcpy(&h.stopCycle, &ctl[C_STOPCYC], 8);
+    h.cycles = in.cycles;
+  }
+  std::memcpy(buf, &h, sizeof h);
+  return TGS_OK;
+}
+
+int tgs_checkpoint_restore(tgs_ctx *c, uint32_t instance, const uint8_t *buf, uint64_t bytes) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  const auto &d = c->design;
+  if (instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "instance %u out of range", instance);
+  CkptHeader h;
+  if (!buf || bytes < sizeof h) return c->fail(TGS_E_FORMAT, "checkpoint shorter than its header");
+  std::memcpy(&h, buf, sizeof h);
+  if (std::memcmp(h.magic, kCkptMagic, 8) != 0) return c->fail(TGS_E_FORMAT, "not a checkpoint image");
+  if (h.designHash != c->designHash || h.regBytes != d.regPoolSize || h.memBytes != d.memPoolSize ||
+      h.exgBytes != d.exchangePoolSize)
+    return c->fail(TGS_E_ARG, "checkpoint was taken on another design");
+  if (bytes < ckptBytes(d)) return c->fail(TGS_E_FORMAT, "checkpoint truncated");
+  if ((rc = setDevice(c))) return rc;
+  uint8_t *p = const_cast<uint8_t *>(buf) + sizeof h;
+  if ((rc = poolIO(c, instance, TGS_POOL_REG, 0, h.regBytes, p, true))) return rc;
+  if ((rc = poolIO(c, instance, TGS_POOL_MEM, 0, h.memBytes, p + h.regBytes, true))) return rc;
+  if ((rc = poolIO(c, instance, TGS_POOL_EXCHANGE, 0, h.exgBytes, p + h.regBytes + h.memBytes, true))) return rc;
+  if (c->batched) {
+    auto &b = c->bs;
+    const uint32_t s = b.slotOf[instance];
+    const uint64_t st = h.stop;
+    HIPCHK(c, hipMemcpy(b.xs.as<uint64_t>() + s, &h.xs, 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b.stop.as<uint64_t>() + s, &st, 8, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(b.stopCycle.as<uint64_t>() + s, &h.stopCycle, 8, hipMemcpyHostToDevice));
+    b.cycles[instance] = h.cycles;
+  } else {
+    Instance &in = *c->inst[instance];
+    uint32_t ctl[C_WORDS];
+    HIPCHK(c, hipMemcpy(ctl, in.ctl.p, sizeof ctl, hipMemcpyDeviceToHost));
+    ctl[C_STOP] = h.stop;
+    std::memcpy(&ctl[C_STOPCYC], &h.stopCycle, 8);
+    HIPCHK(c, hipMemcpy(in.ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(in.xs.p, &h.xs, 8, hipMemcpyHostToDevice));
+    in.cycles = h.cycles;
+  }
   return TGS_OK;
 }
 

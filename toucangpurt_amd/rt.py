@@ -230,6 +230,19 @@ class Simulator:
         n = self.pool_size(POOL_REG)
         return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n else np.zeros(0, np.uint8), cyc.value
 
+    def checkpoint(self, instance=0):
+        """One instance's state (pools, cycle count, stop flag and cycle, stimulus
+        generator) as bytes; restores into any context of the same design."""
+        n = ctypes.c_uint64()
+        self._check(self._lib.tgs_checkpoint_size(self._h, ctypes.byref(n)))
+        buf = np.zeros(n.value, np.uint8)
+        self._check(self._lib.tgs_checkpoint_save(self._h, int(instance), buf.ctypes.data_as(_u8p), buf.size))
+        return buf.tobytes()
+
+    def restore(self, image, instance=0):
+        buf = np.frombuffer(bytes(image), np.uint8)
+        self._check(self._lib.tgs_checkpoint_restore(self._h, int(instance), buf.ctypes.data_as(_u8p), buf.size))
+
     def cycles(self, instance=0):
         c = ctypes.c_uint64()
         self._check(self._lib.tgs_get_cycles(self._h, instance, ctypes.byref(c)))
