@@ -12,6 +12,7 @@
 #include <fstream>
 #include <string>
 #include <tuple>
+#include <unordered_map>
 #include <vector>
 
 #include "SimConfigs.h"
@@ -52,7 +53,8 @@ class ToucanSimulator {
   bool dumpVCD = false;
   std::ofstream *vcdOfs = nullptr;
   std::vector<std::string> vcdNames;
-  std::vector<std::vector<std::tuple<uint32_t, uint32_t>>> vcdRegs;
+  // per VCD variable: (offset in the snapshot, bit width) per fragment
+  std::vector<std::vector<std::tuple<uint32_t, uint32_t>>> vcdSrc;
   std::vector<uint32_t> vcdWidth;
   std::vector<std::string> vcdIds;
   std::vector<std::vector<uint8_t>> vcdLast;
@@ -62,6 +64,11 @@ class ToucanSimulator {
   void writeVcdSnapshot(uint64_t ticket, uint64_t stamp);
   void flushPrints();
   uint64_t printsSeen = 0;
+  // signal probes (one per signal fragment) and where each signal starts
+  std::vector<uint32_t> probeParts, probeSlots;
+  std::unordered_map<std::string, uint32_t> probeFirst;
+  uint64_t probeEpoch = UINT64_MAX;  // cycle_cnt when the probe list last changed
+  uint32_t addSignalProbes(const std::vector<std::string> &names);
 };
 
 }  // namespace toucanGPUSim

@@ -150,9 +150,23 @@ int tgs_reset_instance(tgs_ctx *ctx, uint32_t instance, uint64_t seed, uint32_t 
  * eval. Up to 4 snapshots are in flight; a snapshot's bytes stay valid until
  * the fourth snapshot after it is started. */
 int tgs_snapshot_begin(tgs_ctx *ctx, uint32_t instance, uint64_t *ticket);
-/* Wait for a snapshot; *data points to regPoolSize bytes of pinned memory,
- * *cycle (optional) is the instance's cycle count when it was taken. */
+/* Wait for a snapshot; *data points to pinned memory holding the register
+ * pool (regPoolSize bytes) followed by the signal-probe values in probe-list
+ * order (tgs_set_probes; none when no probes are set); *cycle (optional) is
+ * the instance's cycle count when it was taken. */
 int tgs_snapshot_wait(tgs_ctx *ctx, uint64_t ticket, const uint8_t **data, uint64_t *cycle);
+
+/* Signal probes: the values of value-pool slots (SimDebugInfo signal
+ * locations, (partition, valId) pairs, include/ToucanGPUGenDataTypes.h:232)
+ * as each partition leaves them after its exec levels in the last cycle run.
+ * The reference's readSignal is a stub (src/ToucanSim.cu:291-300) and its
+ * SimVCD reads host copies (unsupport/SimVCD.cpp:193-241); here every kernel
+ * copies the listed slots out at the end of the partition's visit.
+ * tgs_set_probes replaces the list (n = 0 clears it; a slot outside the
+ * partition's value pool is TGS_E_ARG) and zeroes the values;
+ * tgs_read_probes returns the n values in list order for one instance. */
+int tgs_set_probes(tgs_ctx *ctx, const uint32_t *parts, const uint32_t *slots, uint32_t n);
+int tgs_read_probes(tgs_ctx *ctx, uint32_t instance, uint8_t *out, uint32_t n);
 
 /* State checkpoint of one instance (no reference counterpart: the reference
  * keeps one global instance and re-runs init, src/ToucanSim.cu:102-204).
@@ -162,8 +176,10 @@ int tgs_snapshot_wait(tgs_ctx *ctx, uint64_t ticket, const uint8_t **data, uint6
  * pools, nibble per byte. It is the same in every kernel layout, so an
  * instance saved from a batched context restores into a single-instance
  * context of the same design and the other way round. The print log is
- * output, not state, and is not part of it. Restoring an image of another
- * design fails with TGS_E_ARG; a damaged one with TGS_E_FORMAT. */
+ * output, not state, and is not part of it; the stimulus input list is
+ * configuration (tgs_set_stimulus, which resets every generator, goes
+ * before a restore). Restoring an image of another design fails with
+ * TGS_E_ARG; a damaged one with TGS_E_FORMAT. */
 int tgs_checkpoint_size(tgs_ctx *ctx, uint64_t *bytes);
 int tgs_checkpoint_save(tgs_ctx *ctx, uint32_t instance, uint8_t *buf, uint64_t bytes);
 int tgs_checkpoint_restore(tgs_ctx *ctx, uint32_t instance, const uint8_t *buf, uint64_t bytes);

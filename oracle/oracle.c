@@ -75,6 +75,9 @@ struct orc_state {
   uint32_t nIn; uint32_t *inIds; uint64_t xs;
   uint64_t viol[4];
   uint64_t nLog, capLog; uint64_t *logCycle; uint32_t *logMsg;
+  /* signal probes: per partition a CSR list of value-pool slots whose value
+   * at the end of the partition's visit is kept in probeVal */
+  uint32_t nProbe; uint32_t *probeOff, *probeSlot, *probeIdx; uint8_t *probeVal;
 };
 
 /* ------------------------------------------------------------------ */
@@ -361,6 +364,7 @@ void orc_state_free(orc_state *s) {
   free(s->reg); free(s->mem); free(s->exg); free(s->vp); free(s->written);
   free(s->memEpoch); free(s->memPart); free(s->memREpoch); free(s->inIds);
   free(s->logCycle); free(s->logMsg);
+  free(s->probeOff); free(s->probeSlot); free(s->probeIdx); free(s->probeVal);
   free(s);
 }
 
@@ -411,6 +415,67 @@ void orc_violations(const orc_state *s, uint64_t out[4]) { memcpy(out, s->viol, 
 uint64_t orc_print_log(const orc_state *s, uint64_t *cycles, uint32_t *msgs, uint64_t cap) {
   for (uint64_t i = 0; i < s->nLog && i < cap; ++i) { cycles[i] = s->logCycle[i]; msgs[i] = s->logMsg[i]; }
   return s->nLog;
+}
+
+int orc_set_probes(orc_state *s, const uint32_t *parts, const uint32_t *slots, uint32_t n) {
+  const orc_prog *p = s->p;
+  for (uint32_t i = 0; i < n; ++i)
+    if (parts[i] >= p->nParts || slots[i] >= 65536) return -1;
+  free(s->probeOff); free(s->probeSlot); free(s->probeIdx); free(s->probeVal);
+  s->probeOff = NULL; s->probeSlot = s->probeIdx = NULL; s->probeVal = NULL;
+  s->nProbe = n;
+  if (!n) return 0;
+  s->probeOff = xcalloc(p->nParts + 1, 4);
+  s->probeSlot = xcalloc(n, 4);
+  s->probeIdx = xcalloc(n, 4);
+  s->probeVal = xcalloc(n, 1);
+  for (uint32_t i = 0; i < n; ++i) s->probeOff[parts[i] + 1]++;
+  for (uint32_t q = 0; q < p->nParts; ++q) s->probeOff[q + 1] += s->probeOff[q];
+  uint32_t *fill = xcalloc(p->nParts, 4);
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t at = s->probeOff[parts[i]] + fill[parts[i]]++;
+    s->probeSlot[at] = slots[i];
+    s->probeIdx[at] = i;
+  }
+  free(fill);
+  return 0;
+}
+
+void orc_read_probes(const orc_state *s, uint8_t *out) {
+  if (s->nProbe) memcpy(out, s->probeVal, s->nProbe);
+}
+
+/* Slots every visit of partition `part` writes, whatever the state: L0
+ * destinations and the results of LUT, VecRead and VecOp microparts
+ * (a MemRead writes only when enabled, so its results are left out). */
+uint32_t orc_written_slots(const orc_prog *p, uint32_t pi, uint32_t *out, uint32_t cap) {
+  if (pi >= p->nParts) return 0;
+  const part *q = &p->parts[pi];
+  uint8_t *seen = xcalloc(65536, 1);
+  uint32_t n = 0;
+#define PUT(x) do { uint32_t v_ = (x) & 0xFFFFu; if (!seen[v_]) { seen[v_] = 1; if (n < cap) out[n] = v_; n++; } } while (0)
+  for (uint32_t i = 0; i < q->nRegRead; ++i) PUT(q->rrDst[i]);
+  for (uint32_t i = 0; i < q->nExgRead; ++i) PUT(q->erDst[i]);
+  for (uint32_t l = 0; l < q->nLevels; ++l)
+    for (uint32_t k = 0; k < q->levelN[l]; ++k) {
+      const mpart *m = &q->levels[l][k];
+      if (m->isLut) {
+        for (uint32_t i = 0; i < m->nLast && i < 32; ++i) PUT(m->last[i].result);
+      } else if (m->nVr) {
+        for (uint32_t i = 0; i < m->nVr && i < 32; ++i) PUT(m->vr[i].result);
+      } else if (m->nVo) {
+        for (uint32_t i = 0; i < m->nVo && i < 32; ++i) {
+          const vecop_op *o = &m->vo[i];
+          if (o->op <= 2)
+            for (uint32_t j = 0; j < o->len; ++j) PUT((uint16_t)(o->result + j));
+          else
+            PUT(o->result);
+        }
+      }
+    }
+#undef PUT
+  free(seen);
+  return n;
 }
 
 /* ------------------------------------------------------------------ */
@@ -594,6 +659,10 @@ static void eval_partition(orc_state *s, uint32_t partId, uint32_t epoch, int en
       else eval_memread(s, partId, epoch, m);
     }
   }
+  /* signal probes: the slot values this visit leaves behind */
+  if (s->probeOff)
+    for (uint32_t k = s->probeOff[partId]; k < s->probeOff[partId + 1]; ++k)
+      s->probeVal[s->probeIdx[k]] = s->vp[s->probeSlot[k]];
   /* last level (SimEval.cu:846-859) */
   if (pp->ewCount != 0) {
     word_copy(s, s->exg, pp->ewExg, pp->ewDat, pp->ewCount);

@@ -55,6 +55,14 @@ uint64_t orc_stop_cycle(const orc_state *s);
  * [2] out-of-range memory accesses, [3] out-of-range LUT positions. */
 void orc_violations(const orc_state *s, uint64_t out[4]);
 
+/* Signal probes (value-pool slot values a partition leaves behind in its
+ * last visit, SimDebugInfo::signalDebugInfo's (part, valId) locations):
+ * set the list, read the n values in list order after a run. */
+int orc_set_probes(orc_state *s, const uint32_t *parts, const uint32_t *slots, uint32_t n);
+void orc_read_probes(const orc_state *s, uint8_t *out);
+/* Slots every visit of `part` writes; returns the count (stores up to cap). */
+uint32_t orc_written_slots(const orc_prog *p, uint32_t part, uint32_t *out, uint32_t cap);
+
 /* Print log: (cycle, msg) pairs. Returns the number stored in total. */
 uint64_t orc_print_log(const orc_state *s, uint64_t *cycles, uint32_t *msgs, uint64_t cap);
 

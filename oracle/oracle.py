@@ -46,6 +46,11 @@ def lib():
         L.orc_violations.restype = None
         L.orc_print_log.argtypes = [vp, _u64p, _u32p, ctypes.c_uint64]
         L.orc_print_log.restype = ctypes.c_uint64
+        L.orc_set_probes.argtypes = [vp, _u32p, _u32p, ctypes.c_uint32]
+        L.orc_read_probes.argtypes = [vp, _u8p]
+        L.orc_read_probes.restype = None
+        L.orc_written_slots.argtypes = [vp, ctypes.c_uint32, _u32p, ctypes.c_uint32]
+        L.orc_written_slots.restype = ctypes.c_uint32
         L.orc_xorshift64.argtypes = [ctypes.c_uint64]
         L.orc_xorshift64.restype = ctypes.c_uint64
         L.orc_splitmix64.argtypes = [ctypes.c_uint64]
@@ -67,6 +72,15 @@ class Program:
 
     def pool_size(self, which):
         return lib().orc_pool_size(self.h, which)
+
+    def num_parts(self):
+        return lib().orc_num_parts(self.h, 0) + lib().orc_num_parts(self.h, 1)
+
+    def written_slots(self, part):
+        n = lib().orc_written_slots(self.h, part, None, 0)
+        out = np.zeros(max(n, 1), np.uint32)
+        lib().orc_written_slots(self.h, part, out.ctypes.data_as(_u32p), n)
+        return out[:n]
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -115,6 +129,20 @@ class State:
         if threads > 1:
             return bool(lib().orc_run_threads(self.h, cycles, 1 if enable_print else 0, threads))
         return bool(lib().orc_run(self.h, cycles, 1 if enable_print else 0))
+
+    def set_probes(self, parts, slots):
+        self._probe_parts = np.ascontiguousarray(parts, dtype=np.uint32)
+        self._probe_slots = np.ascontiguousarray(slots, dtype=np.uint32)
+        rc = lib().orc_set_probes(self.h, self._probe_parts.ctypes.data_as(_u32p),
+                                  self._probe_slots.ctypes.data_as(_u32p), self._probe_parts.size)
+        if rc != 0:
+            raise ValueError("probe partition or slot out of range")
+
+    def probes(self):
+        n = getattr(self, "_probe_parts", np.zeros(0)).size
+        out = np.zeros(max(n, 1), np.uint8)
+        lib().orc_read_probes(self.h, out.ctypes.data_as(_u8p))
+        return out[:n]
 
     @property
     def cycles(self):

@@ -1,6 +1,7 @@
 """The drop-in C++ ToucanSimulator (libtoucan_sim.so) on the GPU: init error
 code, setRegister/readRegister round trip, eval + eval_free_running against
-the oracle by register name, VCD output structure."""
+the oracle by register name, readSignal against the oracle's signal probes,
+VCD output structure."""
 import os
 import subprocess
 
@@ -27,11 +28,14 @@ def test_cpp_simulator_class(gpu_available, tmp_path, kernel):
                           str(tmp_path / "s.bin"), str(vcd), "5", str(kernel)], capture_output=True, text=True,
                          timeout=120)
     assert out.returncode == 0, out.stderr[-2000:]
-    got = {}
+    got, sig = {}, {}
     for line in out.stdout.splitlines():
         if line.startswith("reg "):
             _, name, val = line.split()
             got[name] = int(val)
+        if line.startswith("sig "):
+            _, name, val = line.split()
+            sig[name] = int(val)
     assert "cycles 10" in out.stdout
     syms = parse_symbols(d.symbols)["reg"]
     orc = OracleRun(d.design, d.symbols, True)
@@ -44,8 +48,22 @@ def test_cpp_simulator_class(gpu_available, tmp_path, kernel):
             reg[frags[k][0]] = v & 15
             v >>= 4
     orc.st.set_pool(0, reg)
+    signals = parse_symbols(d.symbols)["signal"]
+    names = sorted(signals)
+    parts = [p for n in names for p, _, _ in signals[n]]
+    slots = [v for n in names for _, v, _ in signals[n]]
+    orc.st.set_probes(parts, slots)
     orc.run(10)
     reg = orc.st.pool(0)
+    vals = orc.st.probes()
+    k = 0
+    assert names and sorted(sig) == names
+    for n in names:
+        want = 0
+        for _, _, w in signals[n]:
+            want = (want << w) | (int(vals[k]) & ((1 << w) - 1))
+            k += 1
+        assert sig[n] == want, n
     for name, frags in syms.items():
         want = 0
         for rid, _ in frags:
@@ -53,5 +71,9 @@ def test_cpp_simulator_class(gpu_available, tmp_path, kernel):
         assert got[name] == want, name
     text = vcd.read_text()
     assert "$enddefinitions $end" in text and "$dumpvars" in text
+    # signals first, in (partition, value slot) order of their first fragment
+    order = sorted(names, key=lambda n: (signals[n][0][0], signals[n][0][1]))
+    decl = [l.split()[4] for l in text.splitlines() if l.startswith("$var")]
+    assert decl[:len(order)] == order
     stamps = [l for l in text.splitlines() if l.startswith("#")]
     assert stamps == ["#0", "#1", "#2", "#3", "#4", "#10"]

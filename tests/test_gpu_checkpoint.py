@@ -55,6 +55,9 @@ def test_split_run_equals_uninterrupted(gpu_available, src, dst):
     if dst == KERNEL_BATCH:
         b.set_instances(4)
         slot = 2
+    # the stimulus input list is configuration, not state: set it (any seed),
+    # then the restore brings back the generator state
+    b.set_stimulus(d.inputs, 99)
     b.restore(image, slot)
     assert b.cycles(slot) == first
     if dst == KERNEL_BATCH:

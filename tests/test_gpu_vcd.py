@@ -1,4 +1,4 @@
-"""Asynchronous register snapshots and the VCD writer on the GPU."""
+"""Asynchronous snapshots (register pool + signal probes) and the VCD writer on the GPU."""
 import numpy as np
 import pytest
 
@@ -72,7 +72,11 @@ def test_python_vcd_trace_matches_oracle(gpu_available, tmp_path):
     stamps, vals = parse_vcd(path.read_text())
     assert stamps == [0, 1, 2, 3, 4, 10]
     syms = sim.symbols["reg"]
+    sigs = sim.symbols["signal"]
+    assert sigs and not set(sigs) & set(syms)
     orc = OracleRun(d.design, inputs=d.inputs, seed=3)
+    names = sorted(sigs)
+    orc.st.set_probes([p for n in names for p, _, _ in sigs[n]], [v for n in names for _, v, _ in sigs[n]])
     state, seen = {}, {}
     for stamp in stamps:
         orc.run((stamp + 1 if stamp < 5 else 10) - orc.st.cycles)
@@ -82,8 +86,16 @@ def test_python_vcd_trace_matches_oracle(gpu_available, tmp_path):
             for rid, w in frags:
                 v = (v << w) | (int(reg[rid]) & ((1 << w) - 1))
             state[name] = v
-        # change-only dumps: the accumulated waveform equals the oracle's registers at every stamp
+        probes, k = orc.st.probes(), 0
+        for name in names:
+            v = 0
+            for _, _, w in sigs[name]:
+                v = (v << w) | (int(probes[k]) & ((1 << w) - 1))
+                k += 1
+            state[name] = v
+        # change-only dumps: the accumulated waveform equals the oracle's registers
+        # and signals (of the last cycle run) at every stamp
         seen.update(vals[stamp])
-        assert set(seen) == set(syms)
-        for name in syms:
+        assert set(seen) == set(syms) | set(sigs)
+        for name in state:
             assert seen[name] == state[name], (stamp, name)

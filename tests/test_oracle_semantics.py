@@ -130,3 +130,38 @@ def test_partitioning_does_not_change_results():
         syms = parse_symbols(d.symbols)["reg"]
         out.append({name: [int(reg[i]) for i, _ in frags] for name, frags in syms.items()})
     assert out[0] == out[1] == out[2]
+
+
+def test_probes_closed_form():
+    """Probes hold a partition's slot values after its exec levels: the L0 reads
+    of registers 0-7 into slots 20-27 and the LUT results in slots 64-71 of the
+    top_operands program, whose values are known in closed form."""
+    c = E.top_operands()
+    st = O.State(O.Program(c.prog), c.reg, c.mem)
+    st.set_probes([0] * 16, list(range(20, 28)) + list(range(64, 72)))
+    assert st.probes().tolist() == [0] * 16
+    st.run(1)
+    v = st.probes().tolist()
+    assert v[:8] == [int(c.reg[k]) for k in range(8)]
+    assert v[8:] == [c.want_reg[16 + k] for k in range(8)]
+    # written-slot listing: L0 destinations and the last-level results
+    assert set(O.Program(c.prog).written_slots(0).tolist()) == set(range(20, 28)) | set(range(64, 72))
+
+
+def test_probes_threads_agree():
+    d = gen.generate(None, n_luts=4000, depth=8, n_regs=200, n_inputs=50, n_vecops=200, n_srams=2, sram_depth=512,
+                     sram_width=32, part_luts=500, seed=12)
+    prog = O.Program(d.design)
+    rng = np.random.default_rng(1)
+    parts = rng.integers(prog.num_parts(), size=200)
+    slots = [int(prog.written_slots(int(p))[rng.integers(prog.written_slots(int(p)).size)]) for p in parts]
+    got = []
+    for threads in (1, 4):
+        st = O.State(prog)
+        st.set_stimulus(d.inputs, 3)
+        st.set_probes(parts, slots)
+        st.run(7, threads=threads)
+        got.append(st.probes())
+    np.testing.assert_array_equal(got[0], got[1])
+    with pytest.raises(ValueError):
+        O.State(prog).set_probes([prog.num_parts()], [0])

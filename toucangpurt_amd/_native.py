@@ -156,7 +156,7 @@ RT_SYMBOLS = [
     "tgs_load_files", "tgs_set_kernel", "tgs_set_launch", "tgs_set_print", "tgs_set_instances", "tgs_set_stimulus",
     "tgs_eval", "tgs_eval_free_running", "tgs_run_batch", "tgs_read_regs", "tgs_write_regs",
     "tgs_read_pool", "tgs_write_pool", "tgs_pool_size", "tgs_reset_instance", "tgs_snapshot_begin",
-    "tgs_snapshot_wait", "tgs_checkpoint_size", "tgs_checkpoint_save", "tgs_checkpoint_restore",
+    "tgs_snapshot_wait", "tgs_set_probes", "tgs_read_probes", "tgs_checkpoint_size", "tgs_checkpoint_save", "tgs_checkpoint_restore",
     "tgs_get_stop", "tgs_get_cycles",
     "tgs_get_stop_cycle", "tgs_get_print_log", "tgs_get_profile", "tgs_reset_profile",
     "tgs_get_program_stats",
@@ -198,6 +198,8 @@ def rt_lib():
         lib.tgs_reset_instance.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32]
         lib.tgs_snapshot_begin.argtypes = [vp, ctypes.c_uint32, _u64p]
         lib.tgs_snapshot_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(_u8p), _u64p]
+        lib.tgs_set_probes.argtypes = [vp, _u32p, _u32p, ctypes.c_uint32]
+        lib.tgs_read_probes.argtypes = [vp, ctypes.c_uint32, _u8p, ctypes.c_uint32]
         lib.tgs_checkpoint_size.argtypes = [vp, _u64p]
         lib.tgs_checkpoint_save.argtypes = [vp, ctypes.c_uint32, _u8p, ctypes.c_uint64]
         lib.tgs_checkpoint_restore.argtypes = [vp, ctypes.c_uint32, _u8p, ctypes.c_uint64]

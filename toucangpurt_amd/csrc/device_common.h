@@ -36,6 +36,10 @@ struct CommonArgs {
   uint32_t printEnable;
   uint64_t cycleBase;      // cycles completed before this launch
   uint32_t nCycles;
+  // signal probes (null when none): per partition a CSR range of value-pool
+  // slots whose end-of-visit values go to probeVal (CSR order)
+  const uint32_t *probeOff, *probeSlot;
+  uint8_t *probeVal;
 };
 
 // xorshift64, Marsaglia (2003), shifts (13, 7, 17).
@@ -72,6 +76,16 @@ __device__ __forceinline__ void logPrint(const CommonArgs &a, uint64_t cycle, ui
 __device__ __forceinline__ void raiseStop(const CommonArgs &a, uint64_t cycle) {
   atomicOr(&a.ctl[C_STOP], 1u);
   atomicMin(reinterpret_cast<unsigned long long *>(&a.ctl[C_STOPCYC]), (unsigned long long)cycle);
+}
+
+// Signal probes (SimDebugInfo::signalDebugInfo locations): after the exec
+// levels of partition `part`, copy the listed value-pool slots out, `tid` of
+// `nt` threads. The values are those the partition computed this cycle.
+__device__ __forceinline__ void spillProbes(const CommonArgs &a, uint32_t part, const uint8_t *vp, uint32_t tid,
+                                            uint32_t nt) {
+  if (a.probeOff == nullptr) return;
+  const uint32_t b = a.probeOff[part], e = a.probeOff[part + 1];
+  for (uint32_t k = b + tid; k < e; k += nt) a.probeVal[k] = vp[a.probeSlot[k]];
 }
 
 // LUT lookup (SimEval.cu:349-356,406-413) with the u16 index arithmetic of the

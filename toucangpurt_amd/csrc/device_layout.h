@@ -119,6 +119,7 @@ enum StreamHdr : uint32_t {
   S_VP = 0, S_NJ, S_RW_REG, S_RW_DC, S_EW_EXG, S_EW_DC, S_NMW, S_NPR, S_NST,
   S_OFF_LAST,       // word offset (from the first section) of memwrite/print/stop lists
   S_CONST_BASE,     // byte offset of the constant-vector pool in the const image
+  S_PART_ID,        // partition index (signal probes)
   S_WORDS
 };
 constexpr uint32_t kL0None = 3u << 30;

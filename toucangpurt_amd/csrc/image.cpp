@@ -845,6 +845,7 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
     h[S_NST] = exg ? 0 : p.ops_last_stop.size();
     h[S_OFF_LAST] = nWaves * section * kRow;
     h[S_CONST_BASE] = img.consts.size();
+    h[S_PART_ID] = pi;
     for (uint32_t w = 0; w < nWaves; ++w) std::copy(h, h + kRow, W.begin() + off + w * section * kRow);
     img.consts.insert(img.consts.end(), p.constVecPool.begin(), p.constVecPool.end());
     uint32_t q = off + h[S_OFF_LAST];

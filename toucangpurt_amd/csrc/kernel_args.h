@@ -64,6 +64,8 @@ struct BatchArgs {
   uint32_t logCap;
   uint32_t printEnable;
   uint32_t *barrier;
+  const uint32_t *probeOff, *probeSlot;  // signal probes (null when none), CSR by partition
+  uint8_t *probeVal;                     // [probe][instPad]
 };
 
 hipError_t launchPlain(const PlainArgs &args, uint32_t grid, uint32_t block, uint32_t lds, hipStream_t stream);

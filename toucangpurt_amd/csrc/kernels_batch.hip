@@ -239,8 +239,8 @@ __device__ __forceinline__ void wideWrite(uint8_t *pool, size_t instPad, uint32_
 }
 
 // One (partition, group) evaluated by one wave, no workgroup barriers.
-__device__ __forceinline__ void waveItem(const BatchArgs &a, const uint32_t *P, uint32_t g, uint8_t *vp, uint8_t *lvl,
-                                         const uint8_t *lut, const uint64_t *draws, uint32_t c) {
+__device__ __forceinline__ void waveItem(const BatchArgs &a, uint32_t part, const uint32_t *P, uint32_t g, uint8_t *vp,
+                                         uint8_t *lvl, const uint8_t *lut, const uint64_t *draws, uint32_t c) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t slot0 = g * 64u, inst = slot0 + lane;
   const bool liveLane = c < a.runCycles[inst];
@@ -274,6 +274,10 @@ __device__ __forceinline__ void waveItem(const BatchArgs &a, const uint32_t *P, 
       else memReadB(cx, T);
     }
   }
+  // signal probes of the live lanes (slot-major, [probe][instPad])
+  if (a.probeOff != nullptr && liveLane)
+    for (uint32_t k = a.probeOff[part]; k < a.probeOff[part + 1]; ++k)
+      a.probeVal[size_t(k) * IP + inst] = uint8_t(vpAt(cx, a.probeSlot[k]));
   // last level (SimEval.cu:625-761); lanes past their cycle count write nothing
   const uint32_t ewc = P[H_EW_DC] >> 16;
   if (ewc) {
@@ -355,7 +359,7 @@ __device__ __forceinline__ void batchRegion(const BatchArgs &a, uint32_t pb, uin
     const uint32_t p = pb + k / nChunks, g = (k % nChunks) * W + wave;
     stageProgram(prog, a.words + a.partOff[p], a.partWords[p]);
     __syncthreads();
-    if (g < active) waveItem(a, prog, g, vp, lvl, lut, draws, c);
+    if (g < active) waveItem(a, p, prog, g, vp, lvl, lut, draws, c);
     __syncthreads();
   }
 }

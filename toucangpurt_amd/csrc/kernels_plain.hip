@@ -157,6 +157,7 @@ __device__ void plainPartition(const PlainArgs &a, uint32_t pid, uint8_t *vp, co
     }
     __syncthreads();
   }
+  spillProbes(a.c, pid, vp, tid, nt);
   // evalLastLevel / evalLastLevelExgWrite (SimEval.cu:625-761)
   if (P.hasExgWrite) {
     const uint8_t *r = net + P.offExgWrite;

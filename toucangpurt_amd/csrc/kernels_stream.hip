@@ -186,6 +186,7 @@ __device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, cons
 __device__ __forceinline__ void partEnd(const Pf &p, const CommonArgs &c, const uint32_t *__restrict__ W,
                                         const uint8_t *vp, uint64_t cycle, uint32_t tid, uint32_t nTc) {
   if (((p.ey >> 4) & 15u) == kKindNone) return;  // no partition in this slot
+  spillProbes(c, __builtin_amdgcn_readlane(p.r0, S_PART_ID), vp, tid, nTc);
   const uint32_t ewdc = __builtin_amdgcn_readlane(p.r0, S_EW_DC);
   if (ewdc >> 16) {
     const uint32_t exg = __builtin_amdgcn_readlane(p.r0, S_EW_EXG), dat = ewdc & 0xFFFFu;

@@ -164,8 +164,8 @@ __device__ __forceinline__ void memReadTask(const uint32_t *T, uint8_t *vp, cons
   vp[o[3] & 0xFFFFu] = at < c.memSize ? c.mem[at] : uint8_t(0);
 }
 
-__device__ __forceinline__ void w64Partition(const W64Args &a, const uint32_t *P, uint8_t *vp, const uint8_t *lut,
-                                             const uint64_t *draws, uint64_t cycle) {
+__device__ __forceinline__ void w64Partition(const W64Args &a, uint32_t part, const uint32_t *P, uint8_t *vp,
+                                             const uint8_t *lut, const uint64_t *draws, uint64_t cycle) {
   const uint32_t tid = threadIdx.x, nt = blockDim.x;
   const uint32_t wave = tid >> 6, nWaves = nt >> 6;
   // L0 (SimEval.cu:582-622) plus stimulus reads
@@ -195,6 +195,7 @@ __device__ __forceinline__ void w64Partition(const W64Args &a, const uint32_t *P
     }
     __syncthreads();
   }
+  spillProbes(a.c, part, vp, tid, nt);
   // last level (SimEval.cu:625-761)
   const uint32_t ewc = P[H_EW_DC] >> 16;
   if (ewc) {
@@ -268,9 +269,9 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
     __syncthreads();
     if (g == 0)
       for (uint32_t k = tid; k < a.c.nIn; k += nt) a.c.reg[a.c.inIds[k]] = uint8_t(stimBit(draws, k));
-    for (uint32_t s = s0; s < s1; ++s) w64Partition(a, progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
+    for (uint32_t s = s0; s < s1; ++s) w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
     barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, prof);
-    for (uint32_t s = s1; s < s2; ++s) w64Partition(a, progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
+    for (uint32_t s = s1; s < s2; ++s) w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
     barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, prof);
   }
   if (g == 0 && tid == 0) {

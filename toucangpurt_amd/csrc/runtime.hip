@@ -66,6 +66,8 @@ struct SnapRing {
   std::vector<void *> host;  // pinned (hipHostMalloc)
   std::vector<hipEvent_t> staged, done;
   std::vector<uint64_t> ticketOf, cycleOf;
+  std::vector<uint32_t> probes;  // probe values staged after the register pool
+  std::vector<bool> ordered;     // probe values already in probe-list order
   uint64_t next = 0;
   void *staging = nullptr;   // slots * bytes of device memory
   void release() {
@@ -80,7 +82,7 @@ struct SnapRing {
 };
 
 struct Instance {
-  DBuf reg, mem, exg, xs, ctl, logCycle, logMsg;
+  DBuf reg, mem, exg, xs, ctl, logCycle, logMsg, probe;
   uint64_t cycles = 0;
 };
 
@@ -88,7 +90,7 @@ struct Instance {
 // k * instPad + s. Slots are instances reordered by decreasing cycle count.
 struct BatchState {
   uint32_t n = 0, instPad = 0;
-  DBuf reg, mem, exg, xs, draws, stop, stopCycle, logCount, logCycle, logMsg, barrier;
+  DBuf reg, mem, exg, xs, draws, stop, stopCycle, logCount, logCycle, logMsg, barrier, probe;
   DBuf runCycles, cycleBase, groupsActive, instOfD, perm, scratch;
   std::vector<uint32_t> slotOf, instOf;  // instance -> slot, slot -> instance
   std::vector<uint64_t> cycles;          // per instance
@@ -96,7 +98,7 @@ struct BatchState {
 
 // Host copy of one instance's architectural state (layout conversions).
 struct HostInst {
-  std::vector<uint8_t> reg, mem, exg;
+  std::vector<uint8_t> reg, mem, exg, probe;
   uint64_t xs = 0, stopCycle = UINT64_MAX, cycles = 0;
   uint32_t stop = 0;
 };
@@ -155,6 +157,10 @@ struct tgs_ctx {
 
   tgs_profile prof{};
   SnapRing snap;
+  // signal probes: request order -> CSR position (CSR = grouped by partition)
+  std::vector<uint32_t> probeCsrOf;
+  DBuf prOff, prSlot;
+  uint32_t nProbe() const { return uint32_t(probeCsrOf.size()); }
 
   int fail(int code, const char *fmt, ...) {
     char buf[1024];
@@ -199,6 +205,8 @@ int resetInstance(tgs_ctx *c, Instance &in, uint32_t index, uint64_t seed) {
   HIPCHK(c, in.ctl.alloc(C_WORDS * 4));
   HIPCHK(c, in.logCycle.alloc(kLogCap * 8));
   HIPCHK(c, in.logMsg.alloc(kLogCap * 4));
+  HIPCHK(c, in.probe.alloc(std::max<uint32_t>(c->nProbe(), 1)));
+  HIPCHK(c, hipMemsetAsync(in.probe.p, 0, in.probe.n, c->stream));
   HIPCHK(c, hipMemsetAsync(in.reg.p, 0, in.reg.n, c->stream));
   HIPCHK(c, hipMemsetAsync(in.mem.p, 0, in.mem.n, c->stream));
   HIPCHK(c, hipMemsetAsync(in.exg.p, 0, in.exg.n, c->stream));
@@ -379,6 +387,11 @@ CommonArgs commonArgs(tgs_ctx *c, Instance &in, uint32_t cycles) {
   a.printEnable = c->printEnable;
   a.cycleBase = in.cycles;
   a.nCycles = cycles;
+  if (c->nProbe()) {
+    a.probeOff = c->prOff.as<uint32_t>();
+    a.probeSlot = c->prSlot.as<uint32_t>();
+    a.probeVal = in.probe.as<uint8_t>();
+  }
   return a;
 }
 
@@ -496,6 +509,7 @@ int initialHost(tgs_ctx *c, uint32_t n, std::vector<HostInst> &h) {
     std::copy(c->initReg.begin(), c->initReg.end(), h[i].reg.begin());
     std::copy(c->initMem.begin(), c->initMem.end(), h[i].mem.begin());
     h[i].xs = seedFor(c->stimSeed, i);
+    h[i].probe.assign(c->nProbe(), 0);
   }
   return TGS_OK;
 }
@@ -512,6 +526,8 @@ int instUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
     if (!x.mem.empty()) HIPCHK(c, hipMemcpy(in.mem.p, x.mem.data(), x.mem.size(), hipMemcpyHostToDevice));
     if (!x.exg.empty()) HIPCHK(c, hipMemcpy(in.exg.p, x.exg.data(), x.exg.size(), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(in.xs.p, &x.xs, 8, hipMemcpyHostToDevice));
+    if (x.probe.size() == c->nProbe() && !x.probe.empty())
+      HIPCHK(c, hipMemcpy(in.probe.p, x.probe.data(), x.probe.size(), hipMemcpyHostToDevice));
     uint32_t ctl[C_WORDS] = {0};
     ctl[C_STOP] = x.stop;
     std::memcpy(&ctl[C_STOPCYC], &x.stopCycle, 8);
@@ -534,6 +550,8 @@ int instDownload(tgs_ctx *c, std::vector<HostInst> &h) {
     if (!x.mem.empty()) HIPCHK(c, hipMemcpy(x.mem.data(), in.mem.p, x.mem.size(), hipMemcpyDeviceToHost));
     if (!x.exg.empty()) HIPCHK(c, hipMemcpy(x.exg.data(), in.exg.p, x.exg.size(), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(&x.xs, in.xs.p, 8, hipMemcpyDeviceToHost));
+    x.probe.resize(c->nProbe());
+    if (!x.probe.empty()) HIPCHK(c, hipMemcpy(x.probe.data(), in.probe.p, x.probe.size(), hipMemcpyDeviceToHost));
     uint32_t ctl[C_WORDS];
     HIPCHK(c, hipMemcpy(ctl, in.ctl.p, sizeof ctl, hipMemcpyDeviceToHost));
     x.stop = ctl[C_STOP];
@@ -569,7 +587,8 @@ int batchUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
   HIPCHK(c, b.cycleBase.alloc(8 * P));
   HIPCHK(c, b.instOfD.alloc(4 * P));
   HIPCHK(c, b.perm.alloc(4 * P));
-  HIPCHK(c, b.scratch.alloc((std::max({regRows(d), memRows(d), exgRows(d)}) + 8) * P));
+  HIPCHK(c, b.probe.alloc(std::max<uint64_t>(c->nProbe(), 1) * P));
+  HIPCHK(c, b.scratch.alloc((std::max<uint64_t>({regRows(d), memRows(d), exgRows(d), uint64_t(c->nProbe())}) + 8) * P));
   b.slotOf.resize(n);
   b.instOf.assign(P, 0xFFFFFFFFu);
   b.cycles.assign(n, 0);
@@ -591,6 +610,9 @@ int batchUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
   if ((rc = put(b.reg, regRows(d), [](const HostInst &x) -> const std::vector<uint8_t> & { return x.reg; }))) return rc;
   if ((rc = put(b.mem, memRows(d), [](const HostInst &x) -> const std::vector<uint8_t> & { return x.mem; }))) return rc;
   if ((rc = put(b.exg, exgRows(d), [](const HostInst &x) -> const std::vector<uint8_t> & { return x.exg; }))) return rc;
+  if (c->nProbe() &&
+      (rc = put(b.probe, c->nProbe(), [](const HostInst &x) -> const std::vector<uint8_t> & { return x.probe; })))
+    return rc;
   std::vector<uint64_t> xs(P, 0), st(P, 0), sc(P, UINT64_MAX);
   for (uint32_t i = 0; i < n; ++i) {
     xs[i] = h[i].xs;
@@ -624,6 +646,9 @@ int batchDownload(tgs_ctx *c, std::vector<HostInst> &h) {
   if ((rc = get(b.reg, regRows(d), d.regPoolSize, [](HostInst &x) -> std::vector<uint8_t> & { return x.reg; }))) return rc;
   if ((rc = get(b.mem, memRows(d), d.memPoolSize, [](HostInst &x) -> std::vector<uint8_t> & { return x.mem; }))) return rc;
   if ((rc = get(b.exg, exgRows(d), d.exchangePoolSize, [](HostInst &x) -> std::vector<uint8_t> & { return x.exg; }))) return rc;
+  if (c->nProbe() &&
+      (rc = get(b.probe, c->nProbe(), c->nProbe(), [](HostInst &x) -> std::vector<uint8_t> & { return x.probe; })))
+    return rc;
   std::vector<uint64_t> xs(P), st(P), sc(P);
   HIPCHK(c, hipMemcpy(xs.data(), b.xs.p, 8 * P, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(st.data(), b.stop.p, 8 * P, hipMemcpyDeviceToHost));
@@ -710,7 +735,8 @@ int batchOrder(tgs_ctx *c, const uint32_t *cycles) {
   };
   int rc;
   if ((rc = perm8(b.reg, regRows(d))) || (rc = perm8(b.mem, memRows(d))) || (rc = perm8(b.exg, exgRows(d))) ||
-      (rc = perm64(b.xs)) || (rc = perm64(b.stop)) || (rc = perm64(b.stopCycle)))
+      (rc = perm64(b.xs)) || (rc = perm64(b.stop)) || (rc = perm64(b.stopCycle)) ||
+      (c->nProbe() && (rc = perm8(b.probe, c->nProbe()))))
     return rc;
   for (uint32_t i = 0; i < n; ++i) b.slotOf[i] = newSlot[i];
   std::fill(b.instOf.begin(), b.instOf.end(), 0xFFFFFFFFu);
@@ -783,6 +809,11 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
     a.logCap = kLogCap;
     a.printEnable = c->printEnable;
     a.barrier = b.barrier.as<uint32_t>();
+    if (c->nProbe()) {
+      a.probeOff = c->prOff.as<uint32_t>();
+      a.probeSlot = c->prSlot.as<uint32_t>();
+      a.probeVal = b.probe.as<uint8_t>();
+    }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     HIPCHK(c, launchBatch(a, c->batchGrid, c->batchBlockEff, c->batchLds, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -922,6 +953,7 @@ int tgs_load(tgs_ctx *c, const uint8_t *design, size_t designSize, const uint8_t
   if (rc) return rc;
   c->loaded = false;
   c->plainReady = c->w64Ready = c->streamReady = false;
+  c->probeCsrOf.clear();
   c->inst.clear();
   c->design = SimDesignInfo();
   c->symbols = SimDebugInfo();
@@ -1212,7 +1244,7 @@ int tgs_snapshot_begin(tgs_ctx *c, uint32_t instance, uint64_t *ticket) {
   if (!ticket || instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "bad instance or pointer");
   if ((rc = setDevice(c))) return rc;
   auto &s = c->snap;
-  const uint64_t n = c->design.regPoolSize;
+  const uint64_t nReg = c->design.regPoolSize, nPr = c->nProbe(), n = nReg + nPr;
   if (s.slots == 0 || s.bytes < n) {
     s.release();
     s.slots = 4;
@@ -1224,6 +1256,8 @@ int tgs_snapshot_begin(tgs_ctx *c, uint32_t instance, uint64_t *ticket) {
     s.done.assign(s.slots, nullptr);
     s.ticketOf.assign(s.slots, UINT64_MAX);
     s.cycleOf.assign(s.slots, 0);
+    s.probes.assign(s.slots, 0);
+    s.ordered.assign(s.slots, false);
     for (uint32_t k = 0; k < s.slots; ++k) {
       HIPCHK(c, hipHostMalloc(&s.host[k], std::max<uint64_t>(s.bytes, 256), hipHostMallocDefault));
       HIPCHK(c, hipEventCreateWithFlags(&s.staged[k], hipEventDisableTiming));
@@ -1233,13 +1267,16 @@ int tgs_snapshot_begin(tgs_ctx *c, uint32_t instance, uint64_t *ticket) {
   const uint32_t k = uint32_t(s.next % s.slots);
   if (s.ticketOf[k] != UINT64_MAX) HIPCHK(c, hipEventSynchronize(s.done[k]));  // slot still in flight
   uint8_t *stage = static_cast<uint8_t *>(s.staging) + k * s.bytes;
-  if (n) {
-    if (c->batched) {
-      HIPCHK(c, launchSlotColumn(c->bs.reg.as<uint8_t>(), c->bs.instPad, c->bs.slotOf[instance], n, stage, false,
+  if (c->batched) {
+    const auto &b = c->bs;
+    if (nReg) HIPCHK(c, launchSlotColumn(b.reg.as<uint8_t>(), b.instPad, b.slotOf[instance], nReg, stage, false, c->stream));
+    if (nPr)
+      HIPCHK(c, launchSlotColumn(b.probe.as<uint8_t>(), b.instPad, b.slotOf[instance], nPr, stage + nReg, false,
                                  c->stream));
-    } else {
-      HIPCHK(c, hipMemcpyAsync(stage, c->inst[instance]->reg.p, n, hipMemcpyDeviceToDevice, c->stream));
-    }
+  } else {
+    if (nReg) HIPCHK(c, hipMemcpyAsync(stage, c->inst[instance]->reg.p, nReg, hipMemcpyDeviceToDevice, c->stream));
+    if (nPr)
+      HIPCHK(c, hipMemcpyAsync(stage + nReg, c->inst[instance]->probe.p, nPr, hipMemcpyDeviceToDevice, c->stream));
   }
   HIPCHK(c, hipEventRecord(s.staged[k], c->stream));
   HIPCHK(c, hipStreamWaitEvent(s.side, s.staged[k], 0));
@@ -1247,6 +1284,8 @@ int tgs_snapshot_begin(tgs_ctx *c, uint32_t instance, uint64_t *ticket) {
   HIPCHK(c, hipEventRecord(s.done[k], s.side));
   s.ticketOf[k] = s.next;
   s.cycleOf[k] = c->batched ? c->bs.cycles[instance] : c->inst[instance]->cycles;
+  s.probes[k] = uint32_t(nPr);
+  s.ordered[k] = false;
   *ticket = s.next++;
   return TGS_OK;
 }
@@ -1261,8 +1300,66 @@ int tgs_snapshot_wait(tgs_ctx *c, uint64_t ticket, const uint8_t **data, uint64_
   int rc = setDevice(c);
   if (rc) return rc;
   HIPCHK(c, hipEventSynchronize(s.done[k]));
+  // probe values arrive grouped by partition: put them in probe-list order once
+  if (!s.ordered[k] && s.probes[k] && s.probes[k] == c->nProbe()) {
+    uint8_t *pv = static_cast<uint8_t *>(s.host[k]) + c->design.regPoolSize;
+    std::vector<uint8_t> csr(pv, pv + s.probes[k]);
+    for (uint32_t i = 0; i < s.probes[k]; ++i) pv[i] = csr[c->probeCsrOf[i]];
+  }
+  s.ordered[k] = true;
   *data = static_cast<const uint8_t *>(s.host[k]);
   if (cycle) *cycle = s.cycleOf[k];
+  return TGS_OK;
+}
+
+int tgs_set_probes(tgs_ctx *c, const uint32_t *parts, const uint32_t *slots, uint32_t n) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if (n && (!parts || !slots)) return TGS_E_ARG;
+  const auto &d = c->design;
+  const uint32_t nParts = d.parts.size();
+  for (uint32_t i = 0; i < n; ++i) {
+    if (parts[i] >= nParts) return c->fail(TGS_E_ARG, "probe %u: partition %u out of range", i, parts[i]);
+    if (slots[i] >= d.parts[parts[i]].valuePoolSize)
+      return c->fail(TGS_E_ARG, "probe %u: slot %u outside partition %u's value pool", i, slots[i], parts[i]);
+  }
+  // carry every instance's state across the re-allocation of its probe buffer
+  std::vector<HostInst> h;
+  if ((rc = downloadAll(c, h))) return rc;
+  std::vector<uint32_t> off(nParts + 1, 0), fill(nParts, 0), slot(n), csrOf(n);
+  for (uint32_t i = 0; i < n; ++i) off[parts[i] + 1]++;
+  for (uint32_t q = 0; q < nParts; ++q) off[q + 1] += off[q];
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t at = off[parts[i]] + fill[parts[i]]++;
+    slot[at] = slots[i];
+    csrOf[i] = at;
+  }
+  c->probeCsrOf = csrOf;
+  if ((rc = upload(c, c->prOff, off.data(), 4ull * off.size()))) return rc;
+  if ((rc = upload(c, c->prSlot, slot.data(), 4ull * n))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &x : h) x.probe.assign(n, 0);
+  return uploadAll(c, h);
+}
+
+int tgs_read_probes(tgs_ctx *c, uint32_t instance, uint8_t *out, uint32_t n) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if (instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "instance %u out of range", instance);
+  if (n != c->nProbe()) return c->fail(TGS_E_ARG, "%u probes are set, not %u", c->nProbe(), n);
+  if (n == 0) return TGS_OK;
+  if (!out) return TGS_E_ARG;
+  std::vector<uint8_t> v(n);
+  if (c->batched) {
+    auto &b = c->bs;
+    uint8_t *dense = b.scratch.as<uint8_t>();
+    HIPCHK(c, launchSlotColumn(b.probe.as<uint8_t>(), b.instPad, b.slotOf[instance], n, dense, false, c->stream));
+    HIPCHK(c, hipMemcpyAsync(v.data(), dense, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  } else {
+    HIPCHK(c, hipMemcpy(v.data(), c->inst[instance]->probe.p, n, hipMemcpyDeviceToHost));
+  }
+  for (uint32_t i = 0; i < n; ++i) out[i] = v[c->probeCsrOf[i]];
   return TGS_OK;
 }
 

@@ -1,7 +1,8 @@
 // Exercises toucanGPUSim::ToucanSimulator the way a reference testbench does
 // (main.cpp:43-60 plus setRegister / readRegister, src/ToucanSim.cu:271-289).
 // Usage: sim_api_check <design.bin> <symbols.bin> <vcd path> <cycles> <kernel>
-// Prints "reg <name> <value>" lines after the run for the test to compare.
+// Prints "reg <name> <value>" and "sig <name> <value>" lines after the run for
+// the test to compare.
 #include <algorithm>
 #include <cstdlib>
 #include <iostream>
@@ -37,11 +38,20 @@ int main(int argc, char **argv) {
       return 1;
     }
   }
+  std::vector<std::string> sigs;
+  for (auto &kv : sim.symbols.signalDebugInfo) sigs.push_back(kv.first);
+  std::sort(sigs.begin(), sigs.end());
+  // a signal is observable from the first cycle after it is first named
+  if (!sigs.empty() && sim.readSignal(sigs[0]) != UINT64_MAX) {
+    std::cerr << "readSignal before any probed cycle must return UINT64_MAX\n";
+    return 1;
+  }
   sim.enableVCDDump(argv[3]);
   int cycles = std::atoi(argv[4]);
   for (int c = 0; c < cycles; ++c) sim.eval();
   sim.eval_free_running(uint32_t(cycles));
   for (auto &n : names) std::cout << "reg " << n << " " << sim.readRegister(n) << "\n";
+  for (auto &n : sigs) std::cout << "sig " << n << " " << sim.readSignal(n) << "\n";
   std::cout << "cycles " << sim.cycle_cnt << "\n";
   sim.printProfile();
   sim.cleanup();

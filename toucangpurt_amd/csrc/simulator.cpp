@@ -13,6 +13,7 @@
 #include <iterator>
 #include <map>
 #include <sstream>
+#include <unordered_map>
 
 #include "ToucanGPUSim.h"
 #include "toucan_rt.h"
@@ -39,6 +40,10 @@ int ToucanSimulator::init(const int gpu_id, const std::string designBinFilename,
   vcdOfs = nullptr;
   cycle_cnt = 0;
   printsSeen = 0;
+  probeParts.clear();
+  probeSlots.clear();
+  probeFirst.clear();
+  probeEpoch = UINT64_MAX;
   std::vector<uint8_t> d, s;
   if (!slurp(designBinFilename, d)) {
     std::cerr << "Error opening file: " << designBinFilename << std::endl;
@@ -152,10 +157,58 @@ uint64_t ToucanSimulator::readRegister(const std::string &signalName) {
   return r;
 }
 
-// src/ToucanSim.cu:291-300: value-pool signals are not observable after a cycle.
-uint64_t ToucanSimulator::readSignal(const std::string &) {
-  std::cerr << "Not implemented! This might not be necessary" << std::endl;
-  return UINT64_MAX;
+// Probe list management: every fragment of a probed signal is one
+// (partition, value slot) probe; probeFirst maps a signal to its first probe.
+uint32_t ToucanSimulator::addSignalProbes(const std::vector<std::string> &names) {
+  uint32_t added = 0;
+  for (auto &name : names) {
+    if (probeFirst.count(name)) continue;
+    auto it = symbols.signalDebugInfo.find(name);
+    if (it == symbols.signalDebugInfo.end()) continue;
+    probeFirst[name] = uint32_t(probeParts.size());
+    for (auto &f : it->second) {
+      probeParts.push_back(std::get<0>(f));
+      probeSlots.push_back(std::get<1>(f));
+    }
+    ++added;
+  }
+  if (added) {
+    if (tgs_set_probes(ctx, probeParts.data(), probeSlots.data(), uint32_t(probeParts.size())) != TGS_OK) {
+      std::cerr << "signal probes failed: " << lastError() << std::endl;
+      std::abort();
+    }
+    probeEpoch = cycle_cnt;  // the new list starts zeroed: values from the next cycle on
+  }
+  return added;
+}
+
+// src/ToucanSim.cu:291-300 is a stub that returns UINT64_MAX. Here a signal
+// (value-pool slots named by SimDebugInfo::signalDebugInfo) is read from the
+// signal probes: the value its partition computed in the last cycle run.
+// Fragments are concatenated most significant first, `width` bits each, as
+// the VCD writer prints them (unsupport/SimVCD.cpp:222-233). A signal is
+// probed from the first call that names it (or from enableVCDDump) on; until
+// a cycle has run after that, the call returns UINT64_MAX like the reference.
+uint64_t ToucanSimulator::readSignal(const std::string &signalName) {
+  assert(symbols.signalDebugInfo.count(signalName));
+  auto &locs = symbols.signalDebugInfo[signalName];
+  uint32_t width = 0;
+  for (auto &f : locs) width += std::get<2>(f);
+  assert(width <= 64 && "Signal wider than 64 bits are not supported by this function");
+  if (!probeFirst.count(signalName)) addSignalProbes({signalName});
+  if (probeEpoch == cycle_cnt) return UINT64_MAX;
+  std::vector<uint8_t> vals(probeParts.size());
+  if (tgs_read_probes(ctx, 0, vals.data(), uint32_t(vals.size())) != TGS_OK) {
+    std::cerr << "readSignal failed: " << lastError() << std::endl;
+    std::abort();
+  }
+  const uint32_t first = probeFirst[signalName];
+  uint64_t r = 0;
+  for (size_t k = 0; k < locs.size(); ++k) {
+    const uint32_t w = std::get<2>(locs[k]);
+    r = (w >= 64 ? 0 : r << w) | (vals[first + k] & ((w >= 32 ? ~0u : (1u << w) - 1u)));
+  }
+  return r;
 }
 
 // unsupport/SimVCD.cpp:88-97: printable identifiers from '!' in base 94.
@@ -168,26 +221,55 @@ static std::string vcdIdentifier(uint32_t id) {
   return s;
 }
 
-// VCD of every register in the symbol table (unsupport/SimVCD.cpp:99-176):
-// dotted names become nested scopes, names without a dot go in scope "top".
+// unsupport/SimVCD.cpp:99-176: the signals of the symbol table, ordered by
+// (partition, value slot) of their first fragment, identifiers in that
+// order, widths the sum of the fragment widths; dotted names become nested
+// scopes (names without a dot go in scope "top"). The registers follow as
+// further variables, in name order: their values are architectural state the
+// reference's writer has no access to.
 void ToucanSimulator::enableVCDDump(const std::string vcdFilename) {
   dumpVCD = true;
   vcdOfs = new std::ofstream(vcdFilename, std::ios::out);
-  std::vector<std::string> names;
-  for (auto &kv : symbols.regDebugInfo) names.push_back(kv.first);
-  std::sort(names.begin(), names.end());
-  vcdNames = names;
-  vcdRegs.clear();
+  std::vector<std::pair<uint64_t, std::string>> sigs;
+  for (auto &kv : symbols.signalDebugInfo)
+    if (!kv.second.empty())
+      sigs.push_back({(uint64_t(std::get<0>(kv.second[0])) << 32) + std::get<1>(kv.second[0]), kv.first});
+  std::sort(sigs.begin(), sigs.end());
+  std::vector<std::string> sigNames;
+  for (auto &x : sigs) sigNames.push_back(x.second);
+  addSignalProbes(sigNames);
+  std::vector<std::string> regNames;
+  for (auto &kv : symbols.regDebugInfo) regNames.push_back(kv.first);
+  std::sort(regNames.begin(), regNames.end());
+  vcdNames.clear();
+  vcdSrc.clear();
   vcdWidth.clear();
   vcdIds.clear();
-  for (uint32_t i = 0; i < names.size(); ++i) {
-    vcdRegs.push_back(symbols.regDebugInfo[names[i]]);
-    uint32_t w = 0;
-    for (auto &f : vcdRegs.back()) w += std::get<1>(f);
+  const uint32_t regBytes = uint32_t(design.regPoolSize);
+  for (auto &n : sigNames) {
+    std::vector<std::tuple<uint32_t, uint32_t>> src;
+    uint32_t w = 0, k = probeFirst[n];
+    for (auto &f : symbols.signalDebugInfo[n]) {
+      src.push_back({regBytes + k++, std::get<2>(f)});
+      w += std::get<2>(f);
+    }
+    vcdNames.push_back(n);
+    vcdSrc.push_back(src);
     vcdWidth.push_back(w);
-    vcdIds.push_back(vcdIdentifier(i));
   }
-  vcdLast.assign(names.size(), {});
+  for (auto &n : regNames) {
+    std::vector<std::tuple<uint32_t, uint32_t>> src;
+    uint32_t w = 0;
+    for (auto &f : symbols.regDebugInfo[n]) {
+      src.push_back({std::get<0>(f), std::get<1>(f)});
+      w += std::get<1>(f);
+    }
+    vcdNames.push_back(n);
+    vcdSrc.push_back(src);
+    vcdWidth.push_back(w);
+  }
+  for (uint32_t i = 0; i < vcdNames.size(); ++i) vcdIds.push_back(vcdIdentifier(i));
+  vcdLast.assign(vcdNames.size(), {});
   *vcdOfs << "$date\n    " << __DATE__ << " " << __TIME__ << "\n$end\n"
           << "$version\n    VCD generator\n$end\n"
           << "$timescale\n    1ps\n$end\n";
@@ -196,9 +278,9 @@ void ToucanSimulator::enableVCDDump(const std::string vcdFilename) {
     std::map<std::string, Node> kids;
     std::vector<uint32_t> sigs;
   } root;
-  for (uint32_t i = 0; i < names.size(); ++i) {
+  for (uint32_t i = 0; i < vcdNames.size(); ++i) {
     Node *n = &root;
-    std::string rest = names[i];
+    std::string rest = vcdNames[i];
     size_t dot;
     if (rest.find('.') == std::string::npos) n = &root.kids["top"];
     while ((dot = rest.find('.')) != std::string::npos) {
@@ -221,9 +303,10 @@ void ToucanSimulator::enableVCDDump(const std::string vcdFilename) {
 }
 
 // unsupport/SimVCD.cpp:193-241: change-only values per dumped cycle. The
-// dump is pipelined: this cycle's register pool is snapshotted
-// asynchronously (pinned copy on a side stream) and the previous snapshot is
-// written, so the copy and the formatting overlap the next eval.
+// dump is pipelined: the register pool and the signal probes of this cycle
+// are snapshotted asynchronously (pinned copy on a side stream) and the
+// previous snapshot is written, so the copy and the formatting overlap the
+// next eval.
 void ToucanSimulator::dumpVcdWorker(uint64_t cycle) {
   if (!vcdOfs) return;
   uint64_t t = 0;
@@ -239,15 +322,15 @@ void ToucanSimulator::dumpVcdWorker(uint64_t cycle) {
 
 void ToucanSimulator::writeVcdSnapshot(uint64_t ticket, uint64_t stamp) {
   vcdPending = false;
-  const uint8_t *pool = nullptr;
-  if (tgs_snapshot_wait(ctx, ticket, &pool, nullptr) != TGS_OK) {
+  const uint8_t *data = nullptr;
+  if (tgs_snapshot_wait(ctx, ticket, &data, nullptr) != TGS_OK) {
     std::cerr << "VCD snapshot failed: " << lastError() << std::endl;
     return;
   }
   *vcdOfs << "#" << stamp << "\n";
-  for (uint32_t i = 0; i < vcdRegs.size(); ++i) {
+  for (uint32_t i = 0; i < vcdSrc.size(); ++i) {
     std::vector<uint8_t> v;
-    for (auto &f : vcdRegs[i]) v.push_back(pool[std::get<0>(f)]);
+    for (auto &f : vcdSrc[i]) v.push_back(data[std::get<0>(f)]);
     if (!vcdLast[i].empty() && vcdLast[i] == v) continue;
     vcdLast[i] = v;
     if (vcdWidth[i] == 1) {
@@ -255,7 +338,7 @@ void ToucanSimulator::writeVcdSnapshot(uint64_t ticket, uint64_t stamp) {
     } else {
       *vcdOfs << "b";
       for (size_t k = 0; k < v.size(); ++k) {
-        uint32_t w = std::get<1>(vcdRegs[i][k]);
+        uint32_t w = std::get<1>(vcdSrc[i][k]);
         for (int b = int(w) - 1; b >= 0; --b) *vcdOfs << char('0' + ((v[k] >> b) & 1));
       }
       *vcdOfs << " " << vcdIds[i] << "\n";

@@ -49,11 +49,18 @@ class Simulator:
             raise SimError(rc, "no usable GPU (cooperative launch required)")
         self.symbols = None
         self.cycle_cnt = 0
+        self._reset_probes()
         self._design = None
         self._symbols_blob = None
         self._vcd = None
 
     # -- helpers -------------------------------------------------------------
+    def _reset_probes(self):
+        self._probe_first = {}
+        self._probe_parts, self._probe_slots = [], []
+        self._probe_epoch = None
+        self._n_probes = 0
+
     def _err(self):
         buf = ctypes.create_string_buffer(1024)
         self._lib.tgs_last_error(self._h, buf, len(buf))
@@ -87,6 +94,7 @@ class Simulator:
         self._symbols_blob = symbols
         self.symbols = parse_symbols(symbols) if symbols else None
         self.cycle_cnt = 0
+        self._reset_probes()
         return 0
 
     def eval(self):
@@ -137,14 +145,65 @@ class Simulator:
             r = (r << 4) | int(x)
         return r
 
-    def readSignal(self, name):
-        """Unimplemented in the reference (src/ToucanSim.cu:291-300): returns UINT64_MAX."""
-        return (1 << 64) - 1
+    def readSignal(self, name, instance=0):
+        """Value a signal (SimDebugInfo signal locations) had in the last cycle run.
+
+        The reference's readSignal is a stub returning UINT64_MAX
+        (src/ToucanSim.cu:291-300). Here a signal is observed through signal
+        probes from the first call that names it (or enableVCDDump) on;
+        until a cycle has run after that, UINT64_MAX as in the reference.
+        Fragments are concatenated most significant first, `width` bits each
+        (the VCD writer's order, unsupport/SimVCD.cpp:222-233)."""
+        locs = self.symbols["signal"][name]
+        if sum(w for _, _, w in locs) > 64:
+            raise ValueError("signal wider than 64 bits")
+        if name not in self._probe_first:
+            self.probe_signals([name])
+        if self._probe_epoch == self.cycle_cnt:
+            return (1 << 64) - 1
+        vals = self.read_probes(instance)
+        r, k = 0, self._probe_first[name]
+        for _, _, w in locs:
+            r = (r << w) | (int(vals[k]) & ((1 << w) - 1))
+            k += 1
+        return r
+
+    def probe_signals(self, names):
+        """Add signals to the probe list (one probe per fragment)."""
+        added = False
+        for n in names:
+            if n in self._probe_first:
+                continue
+            self._probe_first[n] = len(self._probe_parts)
+            for part, slot, _ in self.symbols["signal"][n]:
+                self._probe_parts.append(part)
+                self._probe_slots.append(slot)
+            added = True
+        if added:
+            self.set_probes(self._probe_parts, self._probe_slots)
+            self._probe_epoch = self.cycle_cnt
+        return added
+
+    def set_probes(self, parts, slots):
+        """Probe value-pool slots: (partition, slot) pairs, values read back in this order."""
+        p = np.ascontiguousarray(parts, dtype=np.uint32)
+        s = np.ascontiguousarray(slots, dtype=np.uint32)
+        if p.size != s.size:
+            raise ValueError("one slot per partition")
+        self._check(self._lib.tgs_set_probes(self._h, p.ctypes.data_as(_u32p), s.ctypes.data_as(_u32p), p.size))
+        self._n_probes = int(p.size)
+
+    def read_probes(self, instance=0):
+        out = np.zeros(max(self._n_probes, 1), np.uint8)
+        self._check(self._lib.tgs_read_probes(self._h, int(instance), out.ctypes.data_as(_u8p), self._n_probes))
+        return out[:self._n_probes]
 
     def enableVCDDump(self, path):
-        """VCD writer (unsupport/SimVCD.cpp:99-241) over asynchronous register snapshots,
-        one per eval() (stamped with the cycle before it) and one after each
-        eval_free_running() (stamped with the cycle count after it), as the C++ class."""
+        """VCD writer (unsupport/SimVCD.cpp:99-241) of the symbol table's signals
+        (through signal probes) followed by its registers, over asynchronous
+        snapshots: one per eval() (stamped with the cycle before it) and one
+        after each eval_free_running() (stamped with the cycle count after it),
+        as the C++ class."""
         from .vcd import VcdWriter
         self._vcd = VcdWriter(self, path)
 
@@ -223,11 +282,12 @@ class Simulator:
         return t.value
 
     def snapshot_wait(self, ticket):
-        """(register pool copy, cycle) of a started snapshot."""
+        """(register pool copy, cycle) of a started snapshot; the signal-probe
+        values, when probes are set, follow the register pool in list order."""
         ptr = _u8p()
         cyc = ctypes.c_uint64()
         self._check(self._lib.tgs_snapshot_wait(self._h, int(ticket), ctypes.byref(ptr), ctypes.byref(cyc)))
-        n = self.pool_size(POOL_REG)
+        n = self.pool_size(POOL_REG) + self._n_probes
         return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n else np.zeros(0, np.uint8), cyc.value
 
     def checkpoint(self, instance=0):

@@ -1,10 +1,14 @@
-"""VCD waveform writer over register snapshots (unsupport/SimVCD.cpp:99-241).
+"""VCD waveform writer over snapshots (unsupport/SimVCD.cpp:99-241).
 
-Every register of the symbol table becomes a VCD variable; dotted names
-become nested scopes and names without a dot go in scope "top"
-(SimVCD.cpp:99-176). Identifiers are printable base-94 numbers from '!'
-(SimVCD.cpp:88-97). A dump writes `#cycle` and the registers whose value
-changed since the previous dump (SimVCD.cpp:193-241).
+The variables are the symbol table's signals, ordered by (partition, value
+slot) of their first fragment as SimVCD.cpp:105-116 orders them and read
+through signal probes, followed by its registers in name order (state the
+reference's writer cannot reach). Dotted names become nested scopes and
+names without a dot go in scope "top" (SimVCD.cpp:99-176). Identifiers are
+printable base-94 numbers from '!' (SimVCD.cpp:88-97). A dump writes
+`#cycle` and the variables whose value changed since the previous dump
+(SimVCD.cpp:193-241); multi-fragment values are the fragments' bits, most
+significant fragment first.
 
 Dumps are pipelined: dump(cycle) starts an asynchronous snapshot of the
 register pool (device staging copy on the context stream, pinned host copy
@@ -12,6 +16,8 @@ on a side stream) and writes the previous snapshot, so the host copy and the
 file writing overlap the next eval. close() writes the last one.
 """
 import time
+
+from . import _native
 
 
 def vcd_identifier(i):
@@ -29,8 +35,14 @@ class VcdWriter:
         self.instance = instance
         self.f = open(path, "w")
         regs = sim.symbols["reg"] if sim.symbols else {}
-        self.names = sorted(regs)
-        self.frags = [regs[n] for n in self.names]
+        sigs = sim.symbols["signal"] if sim.symbols else {}
+        sig_names = sorted((n for n in sigs if sigs[n]), key=lambda n: (sigs[n][0][0], sigs[n][0][1]))
+        sim.probe_signals(sig_names)
+        base = sim.pool_size(_native.POOL_REG)
+        self.names = sig_names + sorted(regs)
+        # per variable: (offset in the snapshot, width) per fragment
+        self.frags = [[(base + sim._probe_first[n] + k, w) for k, (_, _, w) in enumerate(sigs[n])] for n in sig_names]
+        self.frags += [[(rid, w) for rid, w in regs[n]] for n in sorted(regs)]
         self.width = [sum(w for _, w in fr) for fr in self.frags]
         self.ids = [vcd_identifier(i) for i in range(len(self.names))]
         self.last = [None] * len(self.names)
