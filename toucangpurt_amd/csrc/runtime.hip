@@ -188,6 +188,22 @@ namespace {
 
 int setDevice(tgs_ctx *c) { return c->hip(hipSetDevice(c->device), "hipSetDevice"); }
 
+// Workgroups per CU that are certain to be co-resident, for the persistent
+// kernels whose grid barrier needs every workgroup of the grid resident.
+// The hardware allocates a workgroup's LDS in whole granules while the
+// occupancy query can count the requested bytes exactly, so a grid packed to
+// within a granule of the CU's LDS can exceed what is actually resident and
+// the grid barrier then never completes. Each workgroup is charged one more
+// granule's worth (kLdsGranuleMargin, at least the allocation granule) on top
+// of the query's answer.
+constexpr uint32_t kLdsGranuleMargin = 1280;
+int residentPerCU(const tgs_ctx *c, int queried, uint32_t ldsPerGroup) {
+  const uint64_t perCU = c->prop.maxSharedMemoryPerMultiProcessor ? c->prop.maxSharedMemoryPerMultiProcessor
+                                                                  : c->prop.sharedMemPerBlock;
+  const uint64_t byLds = ldsPerGroup ? perCU / (uint64_t(ldsPerGroup) + kLdsGranuleMargin) : uint64_t(queried);
+  return int(std::min<uint64_t>(uint64_t(std::max(queried, 0)), byLds));
+}
+
 int upload(tgs_ctx *c, DBuf &b, const void *src, size_t n) {
   HIPCHK(c, b.alloc(n));
   if (n) HIPCHK(c, hipMemcpyAsync(b.p, src, n, hipMemcpyHostToDevice, c->stream));
@@ -242,6 +258,7 @@ int buildPlain(tgs_ctx *c) {
   HIPCHK(c, plainSetLds(c->plainLds));
   int perCU = 0;
   HIPCHK(c, plainMaxBlocksPerCU(&perCU, c->plainBlock, c->plainLds));
+  perCU = residentPerCU(c, perCU, c->plainLds);
   if (perCU < 1) return c->fail(TGS_E_LIMIT, "plain kernel cannot be resident with %u B LDS", c->plainLds);
   // The reference forces one block per SM for the multi-cycle kernel (ToucanSim.cu:80,189).
   c->plainGrid = std::max<uint32_t>(1, std::min<uint32_t>(im.maxRegion, c->prop.multiProcessorCount));
@@ -261,7 +278,7 @@ int buildW64(tgs_ctx *c) {
   c->sched = scheduleWave64(im, groups);
   const uint32_t base = wave64LutBytes() + up16(std::max<uint32_t>(im.maxVp, 16)) + kMaxDraws * 8;
   const uint32_t resLds = base + c->sched.maxGroupWords * 4;
-  const size_t limit = c->prop.sharedMemPerBlock;
+  const size_t limit = c->prop.sharedMemPerBlock - kLdsGranuleMargin;
   uint32_t maxPart = 0;
   for (auto w : im.partWords) maxPart = std::max(maxPart, (w + 3) & ~3u);
   const uint32_t stagedLds = base + maxPart * 4;
@@ -280,6 +297,7 @@ int buildW64(tgs_ctx *c) {
   HIPCHK(c, wave64SetLds(c->w64Lds));
   int perCU = 0;
   HIPCHK(c, wave64MaxBlocksPerCU(&perCU, c->w64Mode, c->w64Block, c->w64Lds));
+  perCU = residentPerCU(c, perCU, c->w64Lds);
   if (perCU < 1) return c->fail(TGS_E_LIMIT, "wave64 kernel cannot be resident with %u B LDS", c->w64Lds);
   if (uint64_t(perCU) * c->prop.multiProcessorCount < groups)
     return c->fail(TGS_E_LIMIT, "%u workgroups exceed the co-resident limit", groups);
@@ -332,7 +350,7 @@ int buildStream(tgs_ctx *c) {
   if (const char *e = std::getenv("TGS_STREAM_SLOTS")) slots = std::max(1, std::min(std::atoi(e), int(nc)));
   if (nc % slots) slots = 1;
   c->streamSlots = slots;
-  const size_t limit = c->prop.sharedMemPerBlock;
+  const size_t limit = c->prop.sharedMemPerBlock - kLdsGranuleMargin;
   // one workgroup per CU by default: a second workgroup on a CU slows both
   // (shared LDS and issue), and the schedule pairs the short partitions
   uint32_t groups = c->streamGroups ? c->streamGroups : std::max<uint32_t>(std::min<uint32_t>(maxRegion, cus), 1);
@@ -347,6 +365,7 @@ int buildStream(tgs_ctx *c) {
     HIPCHK(c, streamSetLds(lds));
     int perCU = 0;
     HIPCHK(c, streamMaxBlocksPerCU(&perCU, nc, lds));
+    perCU = residentPerCU(c, perCU, lds + streamStaticLdsBytes(nc));
     if (perCU < 1) return c->fail(TGS_E_LIMIT, "streamed kernel cannot be resident with %u B LDS", lds);
     const uint64_t cap = uint64_t(perCU) * cus;
     if (groups <= cap) break;
@@ -683,7 +702,7 @@ int ensureBatchImage(tgs_ctx *c) {
   // Each wave holds one 64-instance group's value pool and level buffer; the
   // LUT and the staged partition program are shared by the workgroup.
   const uint32_t perWave = vpBytes * 64 + 4096, shared = wave64LutBytes() + maxPart * 4;
-  const uint32_t ldsMax = uint32_t(c->prop.sharedMemPerBlock);
+  const uint32_t ldsMax = uint32_t(c->prop.sharedMemPerBlock) - kLdsGranuleMargin;
   if (shared + perWave > ldsMax)
     return c->fail(TGS_E_LIMIT,
                    "batched execution needs %u B of LDS (value pool %u slots x 64 instances); device allows %u B",
@@ -694,6 +713,7 @@ int ensureBatchImage(tgs_ctx *c) {
   HIPCHK(c, batchSetLds(c->batchLds));
   int perCU = 0;
   HIPCHK(c, batchMaxBlocksPerCU(&perCU, c->batchBlockEff, c->batchLds));
+  perCU = residentPerCU(c, perCU, c->batchLds);
   if (perCU < 1) return c->fail(TGS_E_LIMIT, "batched kernel cannot be resident with %u B LDS", c->batchLds);
   perCU = std::min<int>(perCU, std::max<uint32_t>(1, 32 / waves));
   c->batchGrid = uint32_t(perCU) * c->prop.multiProcessorCount;
