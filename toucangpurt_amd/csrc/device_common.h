@@ -88,6 +88,29 @@ __device__ __forceinline__ void spillProbes(const CommonArgs &a, uint32_t part, 
   for (uint32_t k = b + tid; k < e; k += nt) a.probeVal[k] = vp[a.probeSlot[k]];
 }
 
+// Operand of a VecArithOrLogic op read from an LDS value pool (4-byte
+// aligned): `len` (<= 32) nibbles from slot `base`, value = OR of byte k
+// shifted left by 4k (SimEval.cu:491-503). The covering aligned words are
+// loaded together (one LDS round trip instead of one per nibble), realigned
+// to `base` with byte funnel shifts, and the bytes taken from registers.
+__device__ __forceinline__ unsigned __int128 vpNibbles(const uint8_t *vp, uint32_t base, uint32_t len) {
+  const uint32_t *w = reinterpret_cast<const uint32_t *>(vp) + (base >> 2);
+  const uint32_t skip = base & 3u, nw = (skip + len + 3u) >> 2;
+  uint32_t x[9];
+#pragma unroll
+  for (uint32_t i = 0; i < 9; ++i) x[i] = i < nw ? w[i] : 0u;
+  uint32_t y[8];
+#pragma unroll
+  for (uint32_t i = 0; i < 8; ++i) y[i] = __builtin_amdgcn_alignbyte(x[i + 1], x[i], skip);
+  unsigned __int128 v = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 32; ++k) {
+    const uint32_t byte = k < len ? (y[k >> 2] >> ((k & 3u) * 8u)) & 0xFFu : 0u;
+    v |= (unsigned __int128)byte << (4 * k);
+  }
+  return v;
+}
+
 // LUT lookup (SimEval.cu:349-356,406-413) with the u16 index arithmetic of the
 // reference; positions past the 4642-entry table read 0.
 __device__ __forceinline__ uint32_t lutIndex(uint32_t lut, uint32_t a, uint32_t b, uint32_t c) {

@@ -103,12 +103,18 @@ __device__ __forceinline__ void vecOpS(const Pf &p, uint8_t *vp, const uint8_t *
   if (lane >= ((p.ey >> 8) & 0xFFFFu)) return;
   const uint32_t b1 = p.r0 & 0xFFFFu, b2 = p.r0 >> 16;
   const uint32_t len = p.r1 & 0xFFFFu, name = (p.r1 >> 16) & 0xFFu, cf = p.r1 >> 24, res = p.r2;
-  unsigned __int128 v1 = 0, v2 = 0;
-  for (uint32_t k = 0; k < len; ++k) {
-    unsigned __int128 s1 = (cf & 2) ? consts[b1 + k] : vp[b1 + k];
-    unsigned __int128 s2 = (cf & 1) ? consts[b2 + k] : vp[b2 + k];
-    v1 |= s1 << (4 * k);
-    v2 |= s2 << (4 * k);
+  unsigned __int128 v1, v2;
+  if (cf == 0) {
+    v1 = vpNibbles(vp, b1, len);
+    v2 = vpNibbles(vp, b2, len);
+  } else {
+    v1 = v2 = 0;
+    for (uint32_t k = 0; k < len; ++k) {
+      unsigned __int128 s1 = (cf & 2) ? consts[b1 + k] : vp[b1 + k];
+      unsigned __int128 s2 = (cf & 1) ? consts[b2 + k] : vp[b2 + k];
+      v1 |= s1 << (4 * k);
+      v2 |= s2 << (4 * k);
+    }
   }
   if (len > 0 && len < 32) {  // sign_extend_128 (SimEval.cu:222-228)
     const uint32_t bit = 4 * len - 1;
@@ -158,13 +164,30 @@ __device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, cons
   constBase = __builtin_amdgcn_readlane(p.r0, S_CONST_BASE);
   const uint32_t lane = tid & 63u;
   if (nJ > 0) {
+    // Entries 0 and 1 come from the staged rows; the rows of entries 2..7 are
+    // loaded before any gather is issued, so the whole begin costs two global
+    // round trips (rows, then values) instead of one per dependent group.
+    constexpr uint32_t kE = 6;
+    uint32_t src[kE], slot[kE];
+#pragma unroll
+    for (uint32_t u = 0; u < kE; ++u) {
+      const bool in = 2 + u < nJ;
+      src[u] = in ? W[p.ex + (1 + 2 * (2 + u)) * kRow + lane] : kL0None;
+      slot[u] = in ? W[p.ex + (2 + 2 * (2 + u)) * kRow + lane] : 0u;
+    }
     const bool a0 = (p.r1 >> 30) != 3, a1 = nJ > 1 && (p.r3 >> 30) != 3;
     const uint32_t v0 = a0 ? l0Value(c, draws, p.r1) : 0u;
     const uint32_t v1 = a1 ? l0Value(c, draws, p.r3) : 0u;
+    uint32_t val[kE];
+#pragma unroll
+    for (uint32_t u = 0; u < kE; ++u) val[u] = (src[u] >> 30) != 3 ? l0Value(c, draws, src[u]) : 0u;
     if (a0) vp[p.r2] = uint8_t(v0);
     if (a1) vp[p.r4] = uint8_t(v1);
-    // the remaining entries four at a time: all row loads, then all gathers
-    for (uint32_t j = 2; j < nJ; j += 4) {
+#pragma unroll
+    for (uint32_t u = 0; u < kE; ++u)
+      if ((src[u] >> 30) != 3) vp[slot[u]] = uint8_t(val[u]);
+    // the rest (long L0 lists) four at a time: all row loads, then all gathers
+    for (uint32_t j = 2 + kE; j < nJ; j += 4) {
       uint32_t src[4], slot[4], val[4];
 #pragma unroll
       for (uint32_t u = 0; u < 4; ++u) {

@@ -126,12 +126,18 @@ __device__ __forceinline__ void vecOpTask(const uint32_t *T, uint8_t *vp, const 
   const uint32_t *o = T + 1 + 3 * lane;
   const uint32_t b1 = o[0] & 0xFFFFu, b2 = o[0] >> 16;
   const uint32_t len = o[1] & 0xFFFFu, name = (o[1] >> 16) & 0xFFu, cf = o[1] >> 24, res = o[2];
-  unsigned __int128 v1 = 0, v2 = 0;
-  for (uint32_t k = 0; k < len; ++k) {
-    unsigned __int128 s1 = (cf & 2) ? consts[b1 + k] : vp[b1 + k];
-    unsigned __int128 s2 = (cf & 1) ? consts[b2 + k] : vp[b2 + k];
-    v1 |= s1 << (4 * k);
-    v2 |= s2 << (4 * k);
+  unsigned __int128 v1, v2;
+  if (cf == 0) {
+    v1 = vpNibbles(vp, b1, len);
+    v2 = vpNibbles(vp, b2, len);
+  } else {
+    v1 = v2 = 0;
+    for (uint32_t k = 0; k < len; ++k) {
+      unsigned __int128 s1 = (cf & 2) ? consts[b1 + k] : vp[b1 + k];
+      unsigned __int128 s2 = (cf & 1) ? consts[b2 + k] : vp[b2 + k];
+      v1 |= s1 << (4 * k);
+      v2 |= s2 << (4 * k);
+    }
   }
   if (len > 0 && len < 32) {  // sign_extend_128 (SimEval.cu:222-228)
     const uint32_t bit = 4 * len - 1;
