@@ -373,6 +373,30 @@ int buildStream(tgs_ctx *c) {
       return c->fail(TGS_E_LIMIT, "%u workgroups exceed the co-resident limit %llu", groups, (unsigned long long)cap);
     groups = uint32_t(cap);
   }
+  // Two workgroups per CU when a region has several partitions per
+  // workgroup: each workgroup then runs half as many partitions in series,
+  // while a step takes about kTwoPerCUStep times as long (measured: the two
+  // workgroups' LDS traffic contends). Chosen when the longest-processing-time
+  // makespan, priced that way, is shorter.
+  constexpr double kTwoPerCUStep = 1.45;
+  if (!c->streamGroups && groups == cus && maxRegion > cus) {
+    const uint32_t g2 = std::min<uint32_t>(maxRegion, 2 * cus);
+    StreamImage img2 = buildStreamImage(c->design, c->stim, nc, g2, slots);
+    const uint32_t lds2 = streamLdsBytes(nc, slots, up16(std::max<uint32_t>(img2.maxVp, 16)), img2.maxSteps);
+    if (lds2 + streamStaticLdsBytes(nc) <= limit &&
+        double(img2.maxGroupCost) * kTwoPerCUStep < double(c->simg.maxGroupCost)) {
+      int perCU2 = 0;
+      HIPCHK(c, streamSetLds(std::max(lds, lds2)));
+      HIPCHK(c, streamMaxBlocksPerCU(&perCU2, nc, lds2));
+      perCU2 = residentPerCU(c, perCU2, lds2 + streamStaticLdsBytes(nc));
+      if (uint64_t(perCU2) * cus >= g2) {
+        c->simg = std::move(img2);
+        groups = g2;
+        lds = lds2;
+      }
+      HIPCHK(c, streamSetLds(lds));
+    }
+  }
   const auto &im = c->simg;
   int rc;
   if ((rc = upload(c, c->sWords, im.words.data(), im.words.size() * 4))) return rc;
