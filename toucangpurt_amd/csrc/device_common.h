@@ -90,25 +90,47 @@ __device__ __forceinline__ void spillProbes(const CommonArgs &a, uint32_t part, 
 
 // Operand of a VecArithOrLogic op read from an LDS value pool (4-byte
 // aligned): `len` (<= 32) nibbles from slot `base`, value = OR of byte k
-// shifted left by 4k (SimEval.cu:491-503). The covering aligned words are
-// loaded together (one LDS round trip instead of one per nibble), realigned
-// to `base` with byte funnel shifts, and the bytes taken from registers.
+// shifted left by 4k, truncated to 128 bits (SimEval.cu:491-503). The
+// covering aligned words are loaded together (one LDS round trip instead of
+// one per nibble) and realigned to `base` with byte funnel shifts; each word
+// of four bytes b0..b3 then becomes b0 | b1 << 4 | b2 << 8 | b3 << 12 with
+// word-wide operations (exact for any byte value, as the reference's OR).
 __device__ __forceinline__ unsigned __int128 vpNibbles(const uint8_t *vp, uint32_t base, uint32_t len) {
+  // Nine words from the one holding `base`: at most 35 bytes past the
+  // operand's last nibble, which stays inside the LDS allocation (the value
+  // pool is followed by the stimulus draws); bytes past `len` are masked.
   const uint32_t *w = reinterpret_cast<const uint32_t *>(vp) + (base >> 2);
-  const uint32_t skip = base & 3u, nw = (skip + len + 3u) >> 2;
+  const uint32_t skip = base & 3u;
   uint32_t x[9];
 #pragma unroll
-  for (uint32_t i = 0; i < 9; ++i) x[i] = i < nw ? w[i] : 0u;
-  uint32_t y[8];
+  for (uint32_t i = 0; i < 9; ++i) x[i] = w[i];
+  uint32_t c[8];
 #pragma unroll
-  for (uint32_t i = 0; i < 8; ++i) y[i] = __builtin_amdgcn_alignbyte(x[i + 1], x[i], skip);
-  unsigned __int128 v = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < 32; ++k) {
-    const uint32_t byte = k < len ? (y[k >> 2] >> ((k & 3u) * 8u)) & 0xFFu : 0u;
-    v |= (unsigned __int128)byte << (4 * k);
+  for (uint32_t i = 0; i < 8; ++i) {
+    uint32_t y = __builtin_amdgcn_alignbyte(x[i + 1], x[i], skip);
+    const int rem = int(len) - int(4 * i);  // bytes of this word inside the operand
+    y &= rem >= 4 ? ~0u : rem <= 0 ? 0u : (1u << (8 * rem)) - 1u;
+    const uint32_t even = (y & 0xFFu) | ((y >> 8) & 0xFF00u);        // b0 | b2 << 8
+    const uint32_t odd = ((y >> 8) & 0xFFu) | ((y >> 16) & 0xFF00u);  // b1 | b3 << 8
+    c[i] = even | (odd << 4);  // 20 bits at bit 16 i of the value
   }
-  return v;
+  const uint32_t z0 = c[0] | (c[1] << 16);
+  const uint32_t z1 = (c[1] >> 16) | c[2] | (c[3] << 16);
+  const uint32_t z2 = (c[3] >> 16) | c[4] | (c[5] << 16);
+  const uint32_t z3 = (c[5] >> 16) | c[6] | (c[7] << 16);
+  return ((unsigned __int128)z3 << 96) | ((unsigned __int128)z2 << 64) | ((unsigned __int128)z1 << 32) | z0;
+}
+
+// Result of a VecArithOrLogic ADD/SUB/MUL: nibbles 0..len-1 of r to value
+// slots res..res+len-1 (SimEval.cu:527-534).
+__device__ __forceinline__ void vpPutNibbles(uint8_t *vp, uint32_t res, uint32_t len, unsigned __int128 r) {
+  for (uint32_t k = 0; k < len; k += 8) {  // eight nibbles per 32-bit word of r
+    const uint32_t w = uint32_t(r);
+    r >>= 32;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+      if (k + j < len) vp[uint16_t(res + k + j)] = uint8_t((w >> (4 * j)) & 15u);
+  }
 }
 
 // LUT lookup (SimEval.cu:349-356,406-413) with the u16 index arithmetic of the

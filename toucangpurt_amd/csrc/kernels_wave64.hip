@@ -146,11 +146,8 @@ __device__ __forceinline__ void vecOpTask(const uint32_t *T, uint8_t *vp, const 
     if ((v2 >> bit) & 1) v2 |= ext;
   }
   if (name <= 2) {
-    unsigned __int128 r = name == 0 ? v1 + v2 : name == 1 ? v1 - v2 : v1 * v2;
-    for (uint32_t k = 0; k < len; ++k) {
-      vp[uint16_t(res + k)] = uint8_t(r & 15u);
-      r >>= 4;
-    }
+    const unsigned __int128 r = name == 0 ? v1 + v2 : name == 1 ? v1 - v2 : v1 * v2;
+    vpPutNibbles(vp, res, len, r);
   } else {
     const __int128 s1 = (__int128)v1, s2 = (__int128)v2;
     vp[res] = uint8_t(name == 3 ? (s1 == s2) : name == 4 ? (s1 < s2) : name == 5 ? (s1 <= s2) : 0);
