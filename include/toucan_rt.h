@@ -100,7 +100,10 @@ int tgs_load(tgs_ctx *ctx, const uint8_t *design, size_t design_size, const uint
 int tgs_load_files(tgs_ctx *ctx, const char *design_path, const char *symbol_path, int randomize,
                    int enable_print);
 
-/* Choose the kernel family used by eval / eval_free_running. */
+/* Choose the kernel family used by eval / eval_free_running. Until a
+ * family is chosen the streamed kernel runs, and a design whose value pool
+ * or step stream does not fit its LDS budget runs on the wave64 kernel
+ * instead; a chosen family that cannot run fails with TGS_E_LIMIT. */
 int tgs_set_kernel(tgs_ctx *ctx, int kernel);
 /* Launch shape of a kernel family: threads per workgroup (multiple of 64,
  * <= 1024) and workgroups (0 = automatic). The autotuner's knob; the

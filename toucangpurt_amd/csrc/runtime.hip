@@ -129,7 +129,8 @@ struct tgs_ctx {
   std::vector<uint8_t> initReg, initMem;
   uint64_t designHash = 0;  // FNV-1a of the serialized design (checkpoint identity)
   tgs_program_stats stats{};
-  int kernel = TGS_KERNEL_WAVE64;
+  int kernel = TGS_KERNEL_STREAM;
+  bool kernelChosen = false;  // tgs_set_kernel was called
   int printEnable = 0;
   StimMap stim;
   uint64_t stimSeed = 0;
@@ -471,7 +472,11 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles) {
       in.cycles += n;
       c->prof.cycles += n;
     } else if (c->kernel == TGS_KERNEL_STREAM) {
-      if ((rc = buildStream(c))) return rc;
+      if ((rc = buildStream(c))) {
+        if (rc != TGS_E_LIMIT || c->kernelChosen) return rc;
+        c->kernel = TGS_KERNEL_WAVE64;  // the default family does not fit: the wave64 kernel runs it
+        continue;
+      }
       uint32_t n = std::min(cycles, maxCyclesPerLaunch(c->streamGrid));
       StreamArgs a{};
       a.c = commonArgs(c, in, n);
@@ -1025,7 +1030,8 @@ int tgs_load(tgs_ctx *c, const uint8_t *design, size_t designSize, const uint8_t
   if ((rc = upload(c, c->inIds, nullptr, 0))) return rc;
   c->batched = false;
   c->batchReady = false;
-  c->kernel = c->kernel == TGS_KERNEL_BATCH ? TGS_KERNEL_WAVE64 : c->kernel;
+  if (!c->kernelChosen) c->kernel = TGS_KERNEL_STREAM;
+  c->kernel = c->kernel == TGS_KERNEL_BATCH ? TGS_KERNEL_STREAM : c->kernel;
   c->inst.emplace_back(new Instance());
   if ((rc = resetInstance(c, *c->inst[0], 0, c->stimSeed))) return rc;
   c->loaded = true;
@@ -1058,6 +1064,7 @@ int tgs_set_kernel(tgs_ctx *c, int kernel) {
     if ((rc = switchLayout(c, kernel == TGS_KERNEL_BATCH))) return rc;
   }
   c->kernel = kernel;
+  c->kernelChosen = true;
   return TGS_OK;
 }
 
