@@ -83,23 +83,29 @@ enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3 
 
 // ---------------------------------------------------------------------------
 // Stream image (streamed kernel). The program is a u32 word array of 64-word
-// rows. Each workgroup has nC compute waves and one loader wave and walks a
-// flat step stream for one cycle; every step ends with a workgroup barrier.
-// The loader wave copies the rows of upcoming steps from HBM into an LDS ring
-// (LDS-DMA loads) so the compute waves never wait on a global load for
-// program words.
+// rows. Each workgroup has nC compute waves and three loader waves and walks
+// a flat step stream for one cycle; every step ends with a workgroup
+// barrier. The loader waves copy the rows of upcoming steps from HBM into an
+// LDS ring so the compute waves never wait on a global load for program
+// words.
 // Word 0..63 of the image is a zero row (the rows of "no task" entries).
 //
 // Step record: 1 + 2 * nC words: type, then per compute wave an entry
 // {x = word offset (a multiple of 64) of the task or partition block,
-//  y = kind << 4 | n << 8 | rows << 24}.
+//  y = kind << 4 | split << 7 | n << 8 | rows << 24}.
+// A workgroup runs its partitions of a region in rounds. In a round with
+// partitions in several of its `slots`, the compute waves split: slot s
+// uses waves [s nC/slots, (s+1) nC/slots) and its own value pool, and every
+// entry of the round's steps carries kSplit. A round with one partition
+// gives it all nC waves (kSplit clear, value pool of slot 0).
 enum StreamStep : uint32_t {
   S_LEVEL = 0,   // exec level (or a slice of one): one task per compute wave
   S_PART = 1,    // partition begin: L0 reads by the compute waves
   S_PEND = 2,    // partition end: last level by the compute waves
   S_REGION = 3   // grid barrier (end of a region)
 };
-constexpr uint32_t kKindNone = 15;  // entry kind of a wave with no task in the step
+constexpr uint32_t kKindNone = 7;      // entry kind of a wave with no task in the step
+constexpr uint32_t kSplit = 1u << 7;   // entry flag: the waves are split over the slots
 constexpr uint32_t kRow = 64;
 // Task rows, lane l of a row belongs to lane l of the wave:
 //   T_LUT2   (n = nMid, rows 3 + nMid): row 0 top word {lut | op0 << 16},

@@ -701,19 +701,24 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
   img.slots = slots = std::max<uint32_t>(std::min(slots, nC), 1);
   img.n0 = d.regionPartitionIds[0].size();
   img.nTotal = d.parts.size();
-  const uint32_t nWaves = nC / slots;  // compute waves of one partition slot
-  const uint32_t nT = nWaves * 64;
+  // Layout 0: one partition on all nC compute waves. Layout 1 (slots > 1):
+  // nC / slots waves per partition, partitions side by side.
+  const uint32_t nLayouts = slots > 1 ? 2 : 1;
+  const uint32_t wavesOf[2] = {nC, nC / slots};
   auto &W = img.words;
   addRows(W, 1);  // zero row
-  std::vector<uint32_t> blockOff(d.parts.size()), blockRows(d.parts.size());
+  std::vector<uint32_t> blockOff[2], blockRows[2];
+  std::vector<std::vector<std::vector<std::vector<StreamTask>>>> waveTasks[2];  // [layout][part][level][wave]
+  for (uint32_t L = 0; L < nLayouts; ++L) {
+    blockOff[L].resize(d.parts.size());
+    blockRows[L].resize(d.parts.size());
+    waveTasks[L].resize(d.parts.size());
+  }
   std::vector<uint64_t> partCost(d.parts.size());
-  // per partition, per level, per wave: tasks
-  std::vector<std::vector<std::vector<std::vector<StreamTask>>>> waveTasks(d.parts.size());
   for (size_t pi = 0; pi < d.parts.size(); ++pi) {
     const auto &p = d.parts[pi];
     img.maxVp = std::max(img.maxVp, p.valuePoolSize);
-    auto &levels = waveTasks[pi];
-    levels.resize(p.exec_mParts.size());
+    for (uint32_t L = 0; L < nLayouts; ++L) waveTasks[L][pi].resize(p.exec_mParts.size());
     uint64_t cost = 0;
     for (size_t li = 0; li < p.exec_mParts.size(); ++li) {
       std::vector<StreamTask> tasks;
@@ -801,10 +806,14 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
       img.tasks += tasks.size();
       std::vector<uint64_t> tc(tasks.size());
       for (size_t t = 0; t < tasks.size(); ++t) tc[t] = tasks[t].cost;
-      std::vector<uint32_t> binOf;
-      cost += 2 + lpt(tc, nWaves, binOf);
-      levels[li].resize(nWaves);
-      for (size_t t = 0; t < tasks.size(); ++t) levels[li][binOf[t]].push_back(tasks[t]);
+      for (uint32_t L = 0; L < nLayouts; ++L) {
+        std::vector<uint32_t> binOf;
+        const uint64_t span = lpt(tc, wavesOf[L], binOf);
+        if (L == nLayouts - 1) cost += 2 + span;  // priced as the partition mostly runs
+        auto &lv = waveTasks[L][pi][li];
+        lv.resize(wavesOf[L]);
+        for (size_t t = 0; t < tasks.size(); ++t) lv[binOf[t]].push_back(tasks[t]);
+      }
     }
     // partition block: header row, L0 rows, last-level lists
     std::vector<uint32_t> src, slot;
@@ -817,53 +826,63 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
       src.push_back(o.exchange | 1u << 30);
       slot.push_back(o.result);
     }
-    const uint32_t nL0 = src.size(), nJ = (nL0 + nT - 1) / nT;
-    const uint32_t lastWords = 4 * p.ops_last_memWrite.size() + p.ops_last_print.size() + p.ops_last_stop.size();
-    const uint32_t section = 1 + 2 * nJ, lastRows = (lastWords + kRow - 1) / kRow;
-    const uint32_t off = addRows(W, nWaves * section + lastRows);
-    blockOff[pi] = off;
-    blockRows[pi] = section;
-    for (uint32_t w = 0; w < nWaves; ++w)
-      for (uint32_t j = 0; j < nJ; ++j)
-        for (uint32_t l = 0; l < 64; ++l) {
-          const uint32_t k = j * nT + w * 64 + l;
-          W[off + (w * section + 1 + 2 * j) * kRow + l] = k < nL0 ? src[k] : kL0None;
-          W[off + (w * section + 2 + 2 * j) * kRow + l] = k < nL0 ? slot[k] : 0u;
-        }
+    const uint32_t nL0 = src.size();
     const auto &ew = p.op_last_exchangeWrite;
     const auto &rw = p.op_last_regWrite;
     const bool exg = ew.count != 0;
-    uint32_t h[kRow] = {};
-    h[S_VP] = p.valuePoolSize;
-    h[S_NJ] = nJ;
-    h[S_RW_REG] = rw.reg;
-    h[S_RW_DC] = exg ? 0 : (uint32_t(rw.dat) | uint32_t(rw.count) << 16);
-    h[S_EW_EXG] = ew.exchange;
-    h[S_EW_DC] = uint32_t(ew.dat) | uint32_t(ew.count) << 16;
-    h[S_NMW] = exg ? 0 : p.ops_last_memWrite.size();
-    h[S_NPR] = exg ? 0 : p.ops_last_print.size();
-    h[S_NST] = exg ? 0 : p.ops_last_stop.size();
-    h[S_OFF_LAST] = nWaves * section * kRow;
-    h[S_CONST_BASE] = img.consts.size();
-    h[S_PART_ID] = pi;
-    for (uint32_t w = 0; w < nWaves; ++w) std::copy(h, h + kRow, W.begin() + off + w * section * kRow);
+    const uint32_t constBase = img.consts.size();
     img.consts.insert(img.consts.end(), p.constVecPool.begin(), p.constVecPool.end());
-    uint32_t q = off + h[S_OFF_LAST];
-    if (!exg) {
-      for (const auto &o : p.ops_last_memWrite) {
-        W[q++] = uint32_t(o.memBase);
-        W[q++] = uint32_t(o.memBase >> 32);
-        W[q++] = uint32_t(o.addrVec) | uint32_t(o.dat) << 16;
-        W[q++] = uint32_t(o.en) | uint32_t(o.hasMultipleWriter ? 1 : 0) << 16;
+    const uint32_t lastWords = 4 * p.ops_last_memWrite.size() + p.ops_last_print.size() + p.ops_last_stop.size();
+    const uint32_t lastRows = (lastWords + kRow - 1) / kRow;
+    uint32_t nJ0 = 0;
+    for (uint32_t L = 0; L < nLayouts; ++L) {
+      const uint32_t nWaves = wavesOf[L], nT = nWaves * 64;
+      const uint32_t nJ = (nL0 + nT - 1) / nT;
+      if (L == 0) nJ0 = nJ;
+      const uint32_t section = 1 + 2 * nJ;
+      const uint32_t off = addRows(W, nWaves * section + lastRows);
+      blockOff[L][pi] = off;
+      blockRows[L][pi] = section;
+      for (uint32_t w = 0; w < nWaves; ++w)
+        for (uint32_t j = 0; j < nJ; ++j)
+          for (uint32_t l = 0; l < 64; ++l) {
+            const uint32_t k = j * nT + w * 64 + l;
+            W[off + (w * section + 1 + 2 * j) * kRow + l] = k < nL0 ? src[k] : kL0None;
+            W[off + (w * section + 2 + 2 * j) * kRow + l] = k < nL0 ? slot[k] : 0u;
+          }
+      uint32_t h[kRow] = {};
+      h[S_VP] = p.valuePoolSize;
+      h[S_NJ] = nJ;
+      h[S_RW_REG] = rw.reg;
+      h[S_RW_DC] = exg ? 0 : (uint32_t(rw.dat) | uint32_t(rw.count) << 16);
+      h[S_EW_EXG] = ew.exchange;
+      h[S_EW_DC] = uint32_t(ew.dat) | uint32_t(ew.count) << 16;
+      h[S_NMW] = exg ? 0 : p.ops_last_memWrite.size();
+      h[S_NPR] = exg ? 0 : p.ops_last_print.size();
+      h[S_NST] = exg ? 0 : p.ops_last_stop.size();
+      h[S_OFF_LAST] = nWaves * section * kRow;
+      h[S_CONST_BASE] = constBase;
+      h[S_PART_ID] = pi;
+      for (uint32_t w = 0; w < nWaves; ++w) std::copy(h, h + kRow, W.begin() + off + w * section * kRow);
+      uint32_t q = off + h[S_OFF_LAST];
+      if (!exg) {
+        for (const auto &o : p.ops_last_memWrite) {
+          W[q++] = uint32_t(o.memBase);
+          W[q++] = uint32_t(o.memBase >> 32);
+          W[q++] = uint32_t(o.addrVec) | uint32_t(o.dat) << 16;
+          W[q++] = uint32_t(o.en) | uint32_t(o.hasMultipleWriter ? 1 : 0) << 16;
+        }
+        for (const auto &o : p.ops_last_print) W[q++] = uint32_t(o.en) | uint32_t(o.msg) << 16;
+        for (const auto &o : p.ops_last_stop) W[q++] = o.en;
       }
-      for (const auto &o : p.ops_last_print) W[q++] = uint32_t(o.en) | uint32_t(o.msg) << 16;
-      for (const auto &o : p.ops_last_stop) W[q++] = o.en;
     }
-    partCost[pi] = cost + 8 + 4 * nJ + (p.ops_last_memWrite.empty() ? 0 : 8);
+    partCost[pi] = cost + 8 + 4 * nJ0 + (p.ops_last_memWrite.empty() ? 0 : 8);
   }
   if (img.consts.empty()) img.consts.push_back(0);
-  // partitions -> (workgroup, slot) bins per region, then the step streams
+  // partitions -> (slot, workgroup) bins per region, slot-major so that every
+  // workgroup's first slot fills before any second slot; then the step streams
   const size_t bins = size_t(groups) * slots;
+  auto binOfSlot = [&](uint32_t g, uint32_t s) { return size_t(s) * groups + g; };
   std::vector<std::vector<uint32_t>> per(bins * 2);
   for (int r = 0; r < 2; ++r) {
     const auto &ids = d.regionPartitionIds[r];
@@ -888,26 +907,38 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
     const uint32_t base = img.steps.size() / rec;
     for (int r = 0; r < 2; ++r) {
       size_t rounds = 0;
-      for (uint32_t s = 0; s < slots; ++s) rounds = std::max(rounds, per[(size_t(g) * slots + s) * 2 + r].size());
+      for (uint32_t s = 0; s < slots; ++s) rounds = std::max(rounds, per[binOfSlot(g, s) * 2 + r].size());
       for (size_t k = 0; k < rounds; ++k) {
-        // the partitions of this round, one per slot (or none)
-        std::vector<uint32_t> part(slots, noPart);
+        // the partitions of this round: one per slot (or none); a round with
+        // a single partition runs it on all the compute waves
+        std::vector<uint32_t> part;
         for (uint32_t s = 0; s < slots; ++s) {
-          const auto &lst = per[(size_t(g) * slots + s) * 2 + r];
-          if (k < lst.size()) part[s] = lst[k];
+          const auto &lst = per[binOfSlot(g, s) * 2 + r];
+          part.push_back(k < lst.size() ? lst[k] : noPart);
+        }
+        uint32_t occupied = 0;
+        for (uint32_t pi : part) occupied += pi != noPart;
+        const uint32_t L = occupied > 1 ? 1 : 0;
+        const uint32_t ns = L ? slots : 1, nWaves = wavesOf[L];
+        const uint32_t flag = L ? kSplit : 0u;
+        if (!L) {  // the single partition goes to slot 0 on every wave
+          uint32_t only = noPart;
+          for (uint32_t pi : part)
+            if (pi != noPart) only = pi;
+          part.assign(1, only);
         }
         img.steps.push_back(S_PART);
         for (uint32_t w = 0; w < nC; ++w) {
           const uint32_t pi = part[w / nWaves];
-          img.steps.push_back(pi == noPart ? 0u : blockOff[pi] + (w % nWaves) * blockRows[pi] * kRow);
-          img.steps.push_back(pi == noPart ? none : std::min<uint32_t>(blockRows[pi], 255) << 24);
+          img.steps.push_back(pi == noPart ? 0u : blockOff[L][pi] + (w % nWaves) * blockRows[L][pi] * kRow);
+          img.steps.push_back((pi == noPart ? none : std::min<uint32_t>(blockRows[L][pi], 255) << 24) | flag);
         }
         // each slot's level steps (a level wider than the slot's waves takes several), merged
-        std::vector<std::vector<std::vector<const StreamTask *>>> subs(slots);
+        std::vector<std::vector<std::vector<const StreamTask *>>> subs(ns);
         size_t nSteps = 0;
-        for (uint32_t s = 0; s < slots; ++s) {
+        for (uint32_t s = 0; s < ns; ++s) {
           if (part[s] == noPart) continue;
-          for (const auto &lv : waveTasks[part[s]]) {
+          for (const auto &lv : waveTasks[L][part[s]]) {
             size_t nSub = 1;
             for (const auto &mine : lv) nSub = std::max(nSub, mine.size());
             for (size_t q = 0; q < nSub; ++q) {
@@ -925,14 +956,14 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
             const uint32_t s = w / nWaves;
             const StreamTask *t = i < subs[s].size() ? subs[s][i][w % nWaves] : nullptr;
             img.steps.push_back(t ? t->off : 0u);
-            img.steps.push_back(t ? t->y : none);
+            img.steps.push_back((t ? t->y : none) | flag);
           }
         }
         img.steps.push_back(S_PEND);
         for (uint32_t w = 0; w < nC; ++w) {
           const uint32_t pi = part[w / nWaves];
-          img.steps.push_back(pi == noPart ? 0u : blockOff[pi]);
-          img.steps.push_back(pi == noPart ? none : 1u << 24);
+          img.steps.push_back(pi == noPart ? 0u : blockOff[L][pi]);
+          img.steps.push_back((pi == noPart ? none : 1u << 24) | flag);
         }
       }
       step(S_REGION, 0, none);

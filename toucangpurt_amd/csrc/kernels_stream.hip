@@ -155,7 +155,7 @@ __device__ __forceinline__ uint32_t l0Value(const CommonArgs &c, const uint64_t 
 // the partition's slot; each wave reads its own section of the block.
 __device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, const uint32_t *__restrict__ W,
                                           uint8_t *vp, const uint64_t *draws, uint32_t &constBase) {
-  if (((p.ey >> 4) & 15u) == kKindNone) return;  // no partition in this slot
+  if (((p.ey >> 4) & 7u) == kKindNone) return;  // no partition in this slot
   const uint32_t tid = threadIdx.x;
   const uint32_t nJ = __builtin_amdgcn_readfirstlane(__builtin_amdgcn_readlane(p.r0, S_NJ));
   constBase = __builtin_amdgcn_readlane(p.r0, S_CONST_BASE);
@@ -205,7 +205,7 @@ __device__ __forceinline__ void partBegin(const Pf &p, const CommonArgs &c, cons
 // the partition's slot (thread `tid` of `nTc`).
 __device__ __forceinline__ void partEnd(const Pf &p, const CommonArgs &c, const uint32_t *__restrict__ W,
                                         const uint8_t *vp, uint64_t cycle, uint32_t tid, uint32_t nTc) {
-  if (((p.ey >> 4) & 15u) == kKindNone) return;  // no partition in this slot
+  if (((p.ey >> 4) & 7u) == kKindNone) return;  // no partition in this slot
   spillProbes(c, __builtin_amdgcn_readlane(p.r0, S_PART_ID), vp, tid, nTc);
   const uint32_t ewdc = __builtin_amdgcn_readlane(p.r0, S_EW_DC);
   if (ewdc >> 16) {
@@ -357,11 +357,9 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     }
     return;
   }
-  // compute waves: wave `wave` serves partition slot wave / (NC / slots)
+  // compute waves: in a split step wave `wave` serves partition slot
+  // wave / (NC / slots), otherwise every wave serves the one partition
   const uint32_t slotWaves = NC / a.slots;
-  const uint32_t slot = wave / slotWaves;
-  const uint32_t ts = (wave - slot * slotWaves) * 64 + lane, nTs = slotWaves * 64;
-  uint8_t *vp = vpAll + slot * a.vpBytes;
   uint64_t xs = *c.xs;
   uint64_t barTicks = 0;
   const bool prof = g == 0;
@@ -393,7 +391,12 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
         for (uint32_t k = tid; k < c.nIn; k += nTc) c.reg[c.inIds[k]] = uint8_t(stimBit(draws, k));
     }
     readStep(nxt, j + 1);
-    const uint32_t type = cur.ey & 15u, kind = (cur.ey >> 4) & 15u;
+    const uint32_t type = cur.ey & 15u, kind = (cur.ey >> 4) & 7u;
+    const bool split = (__builtin_amdgcn_readfirstlane(cur.ey) & kSplit) != 0;
+    const uint32_t sw = split ? slotWaves : uint32_t(NC);  // waves serving this wave's partition
+    const uint32_t slot = split ? wave / slotWaves : 0u;
+    uint8_t *vp = vpAll + slot * a.vpBytes;
+    const uint32_t ts = (wave - slot * sw) * 64 + lane, nTs = sw * 64;
     if (type == S_LEVEL) {
       if (kind == T_LUT2) lutPairS(cur, W, vp, lut);
       else if (kind == T_VECREAD) vecReadS(cur, vp, c.consts + constBase);

@@ -331,73 +331,71 @@ int buildStream(tgs_ctx *c) {
     nc = widest > 4 ? 8 : widest > 2 ? 4 : 2;
   }
   c->streamNCUsed = nc;
-  // two partitions side by side on four waves each when a region has clearly
-  // more partitions than there are CUs and most exec levels fit in four
-  // waves (the widest still run, in several steps)
-  uint32_t slots = 1;
-  if (nc == 8 && 2 * maxRegion > 3 * cus) {
-    std::vector<uint32_t> per;
-    for (const auto &p : c->design.parts)
-      for (const auto &lv : p.exec_mParts) {
-        uint32_t luts = 0, other = 0;
-        for (const auto &m : lv) (m.isLUTPart ? luts : other)++;
-        per.push_back((luts + 1) / 2 + other);
-      }
-    if (!per.empty()) {
-      std::sort(per.begin(), per.end());
-      if (per[per.size() * 9 / 10] <= 4) slots = 2;
-    }
+  // Launch shape: partition slots per workgroup (1, 2 or 4 side by side) and
+  // workgroups (one or two per CU). Each candidate's image gives its
+  // critical path in steps (the longest workgroup step stream of a cycle);
+  // a step costs more when the waves split over slots or two workgroups
+  // share a CU (LDS traffic contends). Per-step factors measured on the
+  // 10M-LUT4 design: 1, 1.12 and 1.18 for 1, 2 and 4 slots; x1.4 for two
+  // workgroups per CU. The cheapest candidate that fits runs.
+  auto slotFactor = [](uint32_t sl) { return sl == 1 ? 1.0 : sl == 2 ? 1.12 : 1.18; };
+  constexpr double kTwoPerCUStep = 1.4;
+  std::vector<uint32_t> slotList{1};
+  if (maxRegion > cus)  // some workgroup holds several partitions of a region
+    for (uint32_t sl : {2u, 4u})
+      if (nc % sl == 0 && nc / sl >= 2) slotList.push_back(sl);
+  if (const char *e = std::getenv("TGS_STREAM_SLOTS")) {
+    const uint32_t sl = uint32_t(std::max(1, std::min(std::atoi(e), int(nc))));
+    slotList.assign(1, nc % sl ? 1u : sl);
   }
-  if (const char *e = std::getenv("TGS_STREAM_SLOTS")) slots = std::max(1, std::min(std::atoi(e), int(nc)));
-  if (nc % slots) slots = 1;
-  c->streamSlots = slots;
+  std::vector<uint32_t> groupList;
+  if (c->streamGroups) {
+    groupList.push_back(c->streamGroups);
+  } else {
+    groupList.push_back(std::max<uint32_t>(std::min<uint32_t>(maxRegion, cus), 1));
+    if (maxRegion > cus) groupList.push_back(std::min<uint32_t>(maxRegion, 2 * cus));
+  }
   const size_t limit = c->prop.sharedMemPerBlock - kLdsGranuleMargin;
-  // one workgroup per CU by default: a second workgroup on a CU slows both
-  // (shared LDS and issue), and the schedule pairs the short partitions
-  uint32_t groups = c->streamGroups ? c->streamGroups : std::max<uint32_t>(std::min<uint32_t>(maxRegion, cus), 1);
-  uint32_t lds = 0;
-  for (int attempt = 0;; ++attempt) {
-    c->simg = buildStreamImage(c->design, c->stim, nc, groups, slots);
-    const uint32_t vpBytes = up16(std::max<uint32_t>(c->simg.maxVp, 16));
-    lds = streamLdsBytes(nc, slots, vpBytes, c->simg.maxSteps);
-    if (lds + streamStaticLdsBytes(nc) > limit)
-      return c->fail(TGS_E_LIMIT, "streamed kernel needs %u B of LDS (value pool %u, %u steps per cycle); device allows %zu B",
-                     lds + streamStaticLdsBytes(nc), c->simg.maxVp, c->simg.maxSteps, limit);
-    HIPCHK(c, streamSetLds(lds));
-    int perCU = 0;
-    HIPCHK(c, streamMaxBlocksPerCU(&perCU, nc, lds));
-    perCU = residentPerCU(c, perCU, lds + streamStaticLdsBytes(nc));
-    if (perCU < 1) return c->fail(TGS_E_LIMIT, "streamed kernel cannot be resident with %u B LDS", lds);
-    const uint64_t cap = uint64_t(perCU) * cus;
-    if (groups <= cap) break;
-    if (c->streamGroups || attempt > 4)
-      return c->fail(TGS_E_LIMIT, "%u workgroups exceed the co-resident limit %llu", groups, (unsigned long long)cap);
-    groups = uint32_t(cap);
-  }
-  // Two workgroups per CU when a region has several partitions per
-  // workgroup: each workgroup then runs half as many partitions in series,
-  // while a step takes about kTwoPerCUStep times as long (measured: the two
-  // workgroups' LDS traffic contends). Chosen when the longest-processing-time
-  // makespan, priced that way, is shorter.
-  constexpr double kTwoPerCUStep = 1.45;
-  if (!c->streamGroups && groups == cus && maxRegion > cus) {
-    const uint32_t g2 = std::min<uint32_t>(maxRegion, 2 * cus);
-    StreamImage img2 = buildStreamImage(c->design, c->stim, nc, g2, slots);
-    const uint32_t lds2 = streamLdsBytes(nc, slots, up16(std::max<uint32_t>(img2.maxVp, 16)), img2.maxSteps);
-    if (lds2 + streamStaticLdsBytes(nc) <= limit &&
-        double(img2.maxGroupCost) * kTwoPerCUStep < double(c->simg.maxGroupCost)) {
-      int perCU2 = 0;
-      HIPCHK(c, streamSetLds(std::max(lds, lds2)));
-      HIPCHK(c, streamMaxBlocksPerCU(&perCU2, nc, lds2));
-      perCU2 = residentPerCU(c, perCU2, lds2 + streamStaticLdsBytes(nc));
-      if (uint64_t(perCU2) * cus >= g2) {
-        c->simg = std::move(img2);
-        groups = g2;
-        lds = lds2;
+  double best = 0;
+  bool have = false;
+  uint32_t groups = 0, slots = 1, lds = 0;
+  std::string why;
+  for (uint32_t sl : slotList)
+    for (uint32_t g : groupList) {
+      StreamImage img = buildStreamImage(c->design, c->stim, nc, g, sl);
+      const uint32_t vpBytes = up16(std::max<uint32_t>(img.maxVp, 16));
+      const uint32_t l = streamLdsBytes(nc, sl, vpBytes, img.maxSteps);
+      if (l + streamStaticLdsBytes(nc) > limit) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "streamed kernel needs %u B of LDS (value pool %u, %u steps per cycle); device allows %zu B",
+                 l + streamStaticLdsBytes(nc), img.maxVp, img.maxSteps, limit);
+        why = buf;
+        continue;
       }
-      HIPCHK(c, streamSetLds(lds));
+      HIPCHK(c, streamSetLds(l));
+      int perCU = 0;
+      HIPCHK(c, streamMaxBlocksPerCU(&perCU, nc, l));
+      perCU = residentPerCU(c, perCU, l + streamStaticLdsBytes(nc));
+      if (uint64_t(perCU) * cus < g) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "%u workgroups exceed the co-resident limit %llu", g,
+                 (unsigned long long)(uint64_t(std::max(perCU, 0)) * cus));
+        why = buf;
+        continue;
+      }
+      const double score = double(img.maxSteps) * slotFactor(sl) * (g > cus ? kTwoPerCUStep : 1.0);
+      if (!have || score < best) {
+        have = true;
+        best = score;
+        groups = g;
+        slots = sl;
+        lds = l;
+        c->simg = std::move(img);
+      }
     }
-  }
+  if (!have) return c->fail(TGS_E_LIMIT, "%s", why.c_str());
+  c->streamSlots = slots;
+  HIPCHK(c, streamSetLds(lds));
   const auto &im = c->simg;
   int rc;
   if ((rc = upload(c, c->sWords, im.words.data(), im.words.size() * 4))) return rc;

@@ -282,7 +282,6 @@ int main(int argc, char **argv) {
   const uint32_t slots = argc > 9 ? std::atoi(argv[9]) : 1;
   StreamImage im = buildStreamImage(d, stim, nc, groups, slots);
   if (im.slots != slots || nc % slots) die("slots must divide the compute waves");
-  const uint32_t ws = nc / slots;
   if (im.stepBase.size() != im.groups || im.stepCount.size() != im.groups) die("step tables");
   if (im.words.size() % kRow) die("image is not a whole number of rows");
   Interp it{im, d, nc, nc * 64};
@@ -315,7 +314,7 @@ int main(int argc, char **argv) {
         int region = 0;
         std::vector<uint32_t> constBase(slots, 0);
         std::vector<std::vector<uint8_t>> vps(slots, std::vector<uint8_t>(it.vpBytes, 0xEE));
-        auto kindOf = [&](const uint32_t *R, uint32_t w) { return (R[2 + 2 * w] >> 4) & 15u; };
+        auto kindOf = [&](const uint32_t *R, uint32_t w) { return (R[2 + 2 * w] >> 4) & 7u; };
         for (uint32_t s = 0; s < im.stepCount[g]; ++s) {
           const uint32_t *R = &im.steps.at(size_t(im.stepBase[g] + s) * rec);
           const uint32_t type = R[0];
@@ -324,7 +323,14 @@ int main(int argc, char **argv) {
             continue;
           }
           if (region != r) continue;
-          for (uint32_t sl = 0; sl < slots; ++sl) {
+          // split steps: `slots` partitions side by side on nc / slots waves
+          // each; otherwise one partition on all waves (slot 0)
+          const bool split = (R[2] & kSplit) != 0;
+          for (uint32_t w = 1; w < nc; ++w)
+            if (((R[2 + 2 * w] & kSplit) != 0) != split) die("waves disagree on the split flag");
+          const uint32_t nsl = split ? slots : 1, ws = nc / nsl;
+          if (split && slots == 1) die("split step with one slot");
+          for (uint32_t sl = 0; sl < nsl; ++sl) {
             const uint32_t w0 = sl * ws;
             // a slot's waves agree on whether the slot has a partition in this step
             if (type != S_LEVEL)
