@@ -129,6 +129,6 @@ enum StreamHdr : uint32_t {
   S_WORDS
 };
 constexpr uint32_t kL0None = 3u << 30;
-constexpr uint32_t kStreamRows = 8;   // rows staged per wave and step (the loader copies rows 0..7 from x)
+constexpr uint32_t kStreamRows = 6;   // rows staged per wave and step (the loader copies rows 0..5 from x)
 
 }  // namespace tgs

@@ -9,8 +9,10 @@
 //     begin / end, region barriers), every step ending in a workgroup barrier;
 //   * each workgroup has NC compute waves and three loader waves. Loader k
 //     owns every third step: it loads that step's 64-word program rows from
-//     HBM (or the Infinity Cache) into registers three steps ahead (16 bytes
-//     per lane, a wave's six rows are contiguous) and writes them into a
+//     HBM (or the Infinity Cache) into registers three steps ahead (rows 0-5
+//     of each compute wave's entry, the only rows the compute waves read
+//     from the ring: 16 + 8 bytes per lane, a wave's rows are contiguous)
+//     and writes them into a
 //     two-slot LDS ring two steps before they are evaluated; the
 //     compute waves read the next step's rows from the ring while they
 //     evaluate the current step. No compute wave ever waits on a global load
@@ -245,13 +247,15 @@ __device__ __forceinline__ void partEnd(const Pf &p, const CommonArgs &c, const 
   }
 }
 
-// Rows 0..7 of each compute wave: rows 0-3 and rows 4-7, one 16-byte load
-// per lane each (a native vector type: HIP's uint4 wrapper keeps the set out
-// of registers).
+// Rows 0..5 of each compute wave: rows 0-3 as one 16-byte load per lane and
+// rows 4-5 as one 8-byte load per lane (native vector types: HIP's uint4
+// wrapper keeps the set out of registers).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 template <int NC>
 struct RowSet {
-  u32x4 lo[NC], hi[NC];
+  u32x4 lo[NC];  // rows 0-3
+  u32x2 hi[NC];  // rows 4-5
 };
 
 // Loader: load the rows of step record `rec` for every compute wave. The
@@ -267,9 +271,9 @@ __device__ __forceinline__ void loadRows(RowSet<NC> &s, const uint32_t *rec, con
   // masks or a uniform skip) measured 20-60% slower per step
 #pragma unroll
   for (int w = 0; w < NC; ++w) {
-    const u32x4 *src = reinterpret_cast<const u32x4 *>(W + __builtin_amdgcn_readlane(rv, 1 + 2 * w));
-    s.lo[w] = src[lane];
-    s.hi[w] = src[64 + lane];
+    const uint32_t *x = W + __builtin_amdgcn_readlane(rv, 1 + 2 * w);
+    s.lo[w] = reinterpret_cast<const u32x4 *>(x)[lane];
+    s.hi[w] = reinterpret_cast<const u32x2 *>(x + 4 * kRow)[lane];
   }
 }
 
@@ -277,9 +281,9 @@ template <int NC>
 __device__ __forceinline__ void storeRows(const RowSet<NC> &s, uint32_t *slot, uint32_t lane) {
 #pragma unroll
   for (int w = 0; w < NC; ++w) {
-    u32x4 *rows = reinterpret_cast<u32x4 *>(slot + w * kStreamRows * 64);
-    rows[lane] = s.lo[w];
-    rows[64 + lane] = s.hi[w];
+    uint32_t *rows = slot + w * kStreamRows * 64;
+    reinterpret_cast<u32x4 *>(rows)[lane] = s.lo[w];
+    reinterpret_cast<u32x2 *>(rows + 4 * kRow)[lane] = s.hi[w];
   }
 }
 
