@@ -375,8 +375,8 @@ def run_serve(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); multi-GPU sharding is described "
-              "in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): sharding by instance or by "
+              "partition across GPUs is not built", file=sys.stderr)
         sys.exit(2)
     single = run_single(args) if args.mode in ("single", "both", "all") else None
     batched = run_batched(args) if args.mode in ("batched", "both", "all") else None

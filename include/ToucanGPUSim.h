@@ -62,6 +62,7 @@ class ToucanSimulator {
   uint64_t vcdTicket = 0, vcdStamp = 0;
   void dumpVcdWorker(uint64_t cycle);
   void writeVcdSnapshot(uint64_t ticket, uint64_t stamp);
+  void closeVcd();
   void flushPrints();
   uint64_t printsSeen = 0;
   // signal probes (one per signal fragment) and where each signal starts

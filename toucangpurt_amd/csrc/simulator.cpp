@@ -36,8 +36,7 @@ std::string ToucanSimulator::lastError() const {
 // src/ToucanSim.cu:102-204.
 int ToucanSimulator::init(const int gpu_id, const std::string designBinFilename, const std::string symbolBinFilename,
                           bool shouldRandomize, bool enablePrint) {
-  dumpVCD = false;
-  vcdOfs = nullptr;
+  closeVcd();
   cycle_cnt = 0;
   printsSeen = 0;
   probeParts.clear();
@@ -140,7 +139,10 @@ void ToucanSimulator::setRegister(const std::string &signalName, uint64_t signal
     signalValue >>= 4;
   }
   assert(signalValue == 0 && "Given value is wider than register");
-  tgs_write_regs(ctx, 0, ids.data(), uint32_t(ids.size()), vals.data());
+  if (tgs_write_regs(ctx, 0, ids.data(), uint32_t(ids.size()), vals.data()) != TGS_OK) {
+    std::cerr << "setRegister failed: " << lastError() << std::endl;
+    std::abort();
+  }
 }
 
 // src/ToucanSim.cu:280-289 + read_reg_from_gpu (src/SimEval.cu:1058-1069).
@@ -151,7 +153,10 @@ uint64_t ToucanSimulator::readRegister(const std::string &signalName) {
   std::vector<uint32_t> ids;
   for (auto &l : locs) ids.push_back(std::get<0>(l));
   std::vector<uint8_t> vals(ids.size());
-  tgs_read_regs(ctx, 0, ids.data(), uint32_t(ids.size()), vals.data());
+  if (tgs_read_regs(ctx, 0, ids.data(), uint32_t(ids.size()), vals.data()) != TGS_OK) {
+    std::cerr << "readRegister failed: " << lastError() << std::endl;
+    std::abort();
+  }
   uint64_t r = 0;
   for (auto v : vals) r = (r << 4) | v;
   return r;
@@ -228,6 +233,7 @@ static std::string vcdIdentifier(uint32_t id) {
 // further variables, in name order: their values are architectural state the
 // reference's writer has no access to.
 void ToucanSimulator::enableVCDDump(const std::string vcdFilename) {
+  closeVcd();
   dumpVCD = true;
   vcdOfs = new std::ofstream(vcdFilename, std::ios::out);
   std::vector<std::pair<uint64_t, std::string>> sigs;
@@ -346,13 +352,20 @@ void ToucanSimulator::writeVcdSnapshot(uint64_t ticket, uint64_t stamp) {
   }
 }
 
-void ToucanSimulator::cleanup() {
+// Write the snapshot still in flight and close the VCD file, if any.
+void ToucanSimulator::closeVcd() {
   if (vcdOfs) {
     if (vcdPending && ctx) writeVcdSnapshot(vcdTicket, vcdStamp);
     vcdOfs->close();
     delete vcdOfs;
     vcdOfs = nullptr;
   }
+  vcdPending = false;
+  dumpVCD = false;
+}
+
+void ToucanSimulator::cleanup() {
+  closeVcd();
   if (ctx) {
     tgs_destroy(ctx);
     ctx = nullptr;
