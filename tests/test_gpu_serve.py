@@ -49,3 +49,31 @@ def test_serving_results_alone_equal_mix(gpu_available):
             orc.run(j.cycles)
             assert np.array_equal(j.reg[-1], orc.pools()[0]), j.job_id
     assert rep["p99_latency_s"] >= rep["p50_latency_s"] > 0
+
+
+def test_parked_jobs_resume_exactly(gpu_available):
+    """HBM budget for one design at a time: the second design's job parks the
+    first design's running jobs (device checkpoints to host memory), and they
+    resume later in fresh slots of a new context; every result equals the job
+    run alone."""
+    designs = {k: gen.generate(None, **DESIGNS[k]) for k in ("small", "mixed")}
+    t = [0.0]
+    eng = Engine(designs, capacity=64, slice_cycles=40, clock=lambda: t[0], park_after=0.5)
+    eng.budget = max(eng.context_bytes(d) for d in designs.values()) + 1
+    jobs = [Job(0, "small", 5, 300, 31, arrival=0.0), Job(1, "small", 3, 220, 32, arrival=0.0),
+            Job(2, "mixed", 4, 150, 33, arrival=0.1)]
+    print("case serving parking", flush=True)
+    for j in jobs:
+        eng.submit(j)
+    eng._t0 = 0.0
+    for _ in range(400):
+        t[0] += 0.1
+        if eng.step() == 0:
+            break
+    eng.close()
+    assert len(eng.done) == 3 and eng.parked >= 1
+    for j in jobs:
+        reg, mem = run_alone(designs[j.design], j)
+        for i in range(j.instances):
+            assert np.array_equal(j.reg[i], reg[i]), (j.job_id, i)
+            assert np.array_equal(j.mem[i], mem[i]), (j.job_id, i)

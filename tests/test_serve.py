@@ -99,3 +99,41 @@ def test_poisson_jobs_shape():
     # mean inter-arrival 5 ms within sampling error
     assert 0.0035 < np.diff(t).mean() < 0.0065
     assert all(1 <= j.instances <= 64 and 1000 <= j.cycles <= 100_000 for j in jobs)
+
+
+class CheckpointingOracleBackend(OracleBackend):
+    """Adds checkpoint/restore: an instance's state object moves out and back."""
+
+    def checkpoint(self, slot):
+        st = self.st[slot]
+        self.st[slot] = O.State(self.prog, self.reg0, self.mem0)
+        return st
+
+    def restore(self, image, slot):
+        self.st[slot] = image
+
+
+def test_parking_under_budget_pressure(designs):
+    """Room for one design: a job of the other design parks the running jobs
+    (checkpoints), runs, and the parked jobs resume; results equal alone runs."""
+    t = [0.0]
+    eng = Engine(designs, capacity=4, slice_cycles=5, backend=CheckpointingOracleBackend, clock=lambda: t[0],
+                 park_after=0.5)
+    eng.budget = max(eng.context_bytes(d) for d in designs.values()) + 1
+    jobs = [Job(0, "a", 2, 60, 21, arrival=0.0), Job(1, "a", 1, 45, 22, arrival=0.0),
+            Job(2, "b", 3, 20, 23, arrival=0.1)]
+    for j in jobs:
+        eng.submit(j)
+    eng._t0 = 0.0
+    for _ in range(200):
+        t[0] += 0.1
+        if eng.step() == 0:
+            break
+    assert len(eng.done) == 3 and eng.parked >= 1
+    assert jobs[0].parks >= 1
+    assert max(j.parks for j in jobs) <= 3  # time slices of park_after, no ping-pong every step
+    for j in jobs:
+        reg, mem = run_alone(designs[j.design], j, backend=OracleBackend)
+        for i in range(j.instances):
+            np.testing.assert_array_equal(j.reg[i], reg[i], err_msg=f"job {j.job_id} instance {i}")
+            np.testing.assert_array_equal(j.mem[i], mem[i])

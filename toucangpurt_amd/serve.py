@@ -16,7 +16,13 @@ slots that finished jobs freed, so jobs of one design overlap in the same
 launch and jobs of different designs alternate launch by launch. Admission
 is by slots and by an HBM budget: creating a design context charges its
 program image and its slot state; when the budget is short, idle design
-contexts are evicted (least recently used) before a job waits.
+contexts are evicted (least recently used) before a job waits. A job that
+has waited longer than `park_after` seconds for a context that does not fit
+parks the jobs of the least recently used busy context: each of their
+instances is checkpointed to host memory (pools, cycle count, stop flag,
+stimulus generator), the context is evicted, and the parked jobs resume
+later from their checkpoints in whatever slots are free, so parking never
+changes a result.
 
 The reference has no serving path (one simulator per process, global device
 state, src/ToucanSim.cu:102-204); this module is the batching/serving layer
@@ -46,6 +52,10 @@ class Job:
     finished: float = None
     slots: list = field(default_factory=list)
     remaining: np.ndarray = None
+    images: list = None   # per instance checkpoint while parked
+    parks: int = 0
+    waiting_since: float = None  # arrival, or the time it was last parked
+    running_since: float = None  # admission, or the time it last resumed
     reg: list = None      # per instance: final register pool
     mem: list = None      # per instance: final memory pool
 
@@ -86,9 +96,10 @@ class Engine:
     """
 
     def __init__(self, designs, budget=250 << 30, capacity=128, slice_cycles=2000, backend=gpu_backend,
-                 clock=time.perf_counter):
+                 clock=time.perf_counter, park_after=1.0):
         self.designs = designs
         self.budget = budget
+        self.park_after = park_after
         self.capacity = capacity
         self.slice = slice_cycles
         self.backend = backend
@@ -100,6 +111,7 @@ class Engine:
         self.instance_cycles = 0
         self.steps = 0
         self.launches = 0
+        self.parked = 0
         self._t0 = None
 
     # ---- HBM accounting -------------------------------------------------
@@ -150,23 +162,53 @@ class Engine:
             raise ValueError(f"job {job.job_id}: at least one cycle")
         self.queue.append(job)
 
+    def _park(self, c, now):
+        """Checkpoint every running job of context `c`, evict it; returns the jobs."""
+        jobs = c.jobs
+        for job in jobs:
+            job.images = [c.sim.checkpoint(s) for s in job.slots]
+            job.slots = []
+            job.parks += 1
+            job.waiting_since = now
+            self.parked += 1
+        c.jobs = []
+        c.sim.cleanup()
+        self.used -= c.nbytes
+        del self.ctx[c.key]
+        return jobs
+
     def _admit(self, now):
-        waiting = []
+        waiting, parked = [], []
         for job in self.queue:
             if job.arrival > now:
                 waiting.append(job)
                 continue
             c = self._context(job.design, now)
+            since = job.arrival if job.waiting_since is None else job.waiting_since
+            if c is None and now - since > self.park_after:
+                # only contexts whose jobs have all run at least park_after since
+                # they started or resumed (no ping-pong between two designs)
+                busy = [x for x in self.ctx.values() if x.jobs and x.key != job.design and
+                        all(now - j.running_since >= self.park_after for j in x.jobs)]
+                if busy:
+                    parked += self._park(min(busy, key=lambda x: x.last_used), now)
+                    c = self._context(job.design, now)
             if c is None or len(c.free) < job.instances:
                 waiting.append(job)
                 continue
             job.slots = [c.free.pop(0) for _ in range(job.instances)]
-            for i, s in enumerate(job.slots):
-                c.sim.reset_instance(s, job.seed, i)
-            job.remaining = np.full(job.instances, job.cycles, np.int64)
-            job.admitted = now
+            job.running_since = now
+            if job.images is not None:  # a parked job resumes where it stopped
+                for img, s in zip(job.images, job.slots):
+                    c.sim.restore(img, s)
+                job.images = None
+            else:
+                for i, s in enumerate(job.slots):
+                    c.sim.reset_instance(s, job.seed, i)
+                job.remaining = np.full(job.instances, job.cycles, np.int64)
+                job.admitted = now
             c.jobs.append(job)
-        self.queue = waiting
+        self.queue = parked + waiting
 
     def step(self):
         """Admit arrived jobs, run one slice of every design with active slots,
@@ -235,6 +277,7 @@ class Engine:
             "seconds": wall,
             "cycles_per_s": self.instance_cycles / wall if wall > 0 else None,
             "launches": self.launches,
+            "parked": self.parked,
             "steps": self.steps,
             "resident_bytes": self.used,
         }
