@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--serve-slice", type=int, default=500)
     ap.add_argument("--serve-capacity", type=int, default=128)
     ap.add_argument("--serve-budget-gb", type=float, default=250.0)
+    ap.add_argument("--serve-small-jobs", type=int, default=0,
+                    help="jobs of at most this many instances run on the streamed kernel (0: all batched)")
     ap.add_argument("--chunk", type=int, default=10_000, help="cycles per step (single leg)")
     ap.add_argument("--kernel", choices=["auto", "stream", "wave64"], default="auto", help="single-leg kernel")
     ap.add_argument("--tune-single", type=int, default=2_000, help="cycles per kernel in the single-leg choice")
@@ -343,7 +345,7 @@ def run_serve(args):
     jobs = poisson_jobs(list(designs), args.serve_jobs, args.serve_rate, rng,
                         min_cycles=args.serve_min_cycles, max_cycles=args.serve_max_cycles)
     eng = Engine(designs, budget=int(args.serve_budget_gb * (1 << 30)), capacity=args.serve_capacity,
-                 slice_cycles=args.serve_slice)
+                 slice_cycles=args.serve_slice, small_jobs=args.serve_small_jobs)
     done = [0]
 
     def progress():
@@ -365,7 +367,7 @@ def run_serve(args):
                             f"LUT4 (depth 32, partitions of 256), {args.serve_jobs} jobs, Poisson "
                             f"{args.serve_rate}/s, 1-64 instances x {args.serve_min_cycles}-{args.serve_max_cycles} "
                             f"cycles, {args.serve_budget_gb} GB budget, {args.serve_capacity} slots per design, "
-                            f"slice {args.serve_slice} cycles",
+                            f"slice {args.serve_slice} cycles, jobs of <= {args.serve_small_jobs} instances on the streamed kernel",
                 "unit": "instance-cycles/s"})
     log(f"serve: p50 {rep['p50_latency_s']:.3f} s p99 {rep['p99_latency_s']:.3f} s, "
         f"{rep['cycles_per_s']:.3e} instance-cycles/s")

@@ -31,7 +31,7 @@ def test_serving_results_alone_equal_mix(gpu_available):
                         arrival=t))
         t += 0.002
     print(f"case serving {len(jobs)} jobs", flush=True)
-    eng = Engine(designs, capacity=64, slice_cycles=48)
+    eng = Engine(designs, capacity=64, slice_cycles=48, small_jobs=8, small_capacity=32)
     rep = eng.run(jobs)
     eng.close()
     assert rep["jobs"] == len(jobs)

@@ -13,7 +13,7 @@ from toucangpurt_amd.serve import POOL_MEM, POOL_REG, Engine, Job, poisson_jobs,
 class OracleBackend:
     """Batched-context stand-in: one oracle state per instance slot."""
 
-    def __init__(self, design):
+    def __init__(self, design, kernel=None):
         self.prog, self.reg0, self.mem0 = initial_pools(design.design, None, False)
         self.inputs = None
         self.st = []
@@ -60,7 +60,7 @@ def test_jobs_alone_equal_mixed(designs):
     jobs = [Job(0, "a", 3, 40, 11, arrival=0.0), Job(1, "b", 2, 25, 12, arrival=0.0),
             Job(2, "a", 4, 17, 13, arrival=0.0), Job(3, "a", 2, 33, 14, arrival=0.0),
             Job(4, "b", 5, 9, 15, arrival=0.0)]
-    eng = Engine(designs, capacity=8, slice_cycles=7, backend=OracleBackend)
+    eng = Engine(designs, capacity=8, slice_cycles=7, backend=OracleBackend, small_jobs=0)
     rep = eng.run(jobs)
     assert rep["jobs"] == len(jobs)
     # slot reuse: 11 instances of design a through 8 slots
@@ -74,7 +74,7 @@ def test_jobs_alone_equal_mixed(designs):
 
 
 def test_budget_evicts_idle_context(designs):
-    eng = Engine(designs, capacity=4, slice_cycles=50, backend=OracleBackend)
+    eng = Engine(designs, capacity=4, slice_cycles=50, backend=OracleBackend, small_jobs=0)
     one = max(eng.context_bytes(d) for d in designs.values())
     eng.budget = one + 1  # room for one resident design at a time
     jobs = [Job(0, "a", 2, 10, 1), Job(1, "b", 2, 10, 2), Job(2, "a", 1, 10, 3)]
@@ -84,7 +84,7 @@ def test_budget_evicts_idle_context(designs):
 
 
 def test_rejects_oversized_jobs(designs):
-    eng = Engine(designs, capacity=4, backend=OracleBackend)
+    eng = Engine(designs, capacity=4, backend=OracleBackend, small_jobs=0)
     with pytest.raises(ValueError):
         eng.submit(Job(0, "a", 5, 10, 1))
     with pytest.raises(ValueError):
@@ -118,7 +118,7 @@ def test_parking_under_budget_pressure(designs):
     (checkpoints), runs, and the parked jobs resume; results equal alone runs."""
     t = [0.0]
     eng = Engine(designs, capacity=4, slice_cycles=5, backend=CheckpointingOracleBackend, clock=lambda: t[0],
-                 park_after=0.5)
+                 park_after=0.5, small_jobs=0)
     eng.budget = max(eng.context_bytes(d) for d in designs.values()) + 1
     jobs = [Job(0, "a", 2, 60, 21, arrival=0.0), Job(1, "a", 1, 45, 22, arrival=0.0),
             Job(2, "b", 3, 20, 23, arrival=0.1)]
@@ -132,6 +132,21 @@ def test_parking_under_budget_pressure(designs):
     assert len(eng.done) == 3 and eng.parked >= 1
     assert jobs[0].parks >= 1
     assert max(j.parks for j in jobs) <= 3  # time slices of park_after, no ping-pong every step
+    for j in jobs:
+        reg, mem = run_alone(designs[j.design], j, backend=OracleBackend)
+        for i in range(j.instances):
+            np.testing.assert_array_equal(j.reg[i], reg[i], err_msg=f"job {j.job_id} instance {i}")
+            np.testing.assert_array_equal(j.mem[i], mem[i])
+
+
+def test_small_jobs_use_their_own_contexts(designs):
+    """Jobs of at most small_jobs instances run in a design's second context
+    (the streamed kernel on the device); results are unchanged."""
+    jobs = [Job(0, "a", 1, 30, 41), Job(1, "a", 6, 22, 42), Job(2, "b", 2, 17, 43), Job(3, "a", 2, 9, 44)]
+    eng = Engine(designs, capacity=8, slice_cycles=6, backend=OracleBackend, small_jobs=2, small_capacity=4)
+    rep = eng.run(jobs)
+    assert rep["jobs"] == 4
+    assert {k for k in eng.ctx} == {("a", "small"), ("a", "batch"), ("b", "small")}
     for j in jobs:
         reg, mem = run_alone(designs[j.design], j, backend=OracleBackend)
         for i in range(j.instances):

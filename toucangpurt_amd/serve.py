@@ -8,9 +8,18 @@ running it alone in its own context (tgs_set_instances(n) +
 tgs_set_stimulus(inputs, seed)).
 
 The engine keeps one batched context per design resident in HBM (program
-image plus instance-minor state for a fixed number of instance slots) and
-runs in slices: every step makes one persistent launch per design with
-active slots, each active slot advancing min(remaining, slice) cycles.
+image plus instance-minor state for a fixed number of instance slots) and,
+optionally, a second context per design on the streamed single-instance
+kernel for jobs of at most `small_jobs` instances. A batched launch-cycle
+costs about the same for 1 or 192 instances (one wave per 64-instance group
+walks a partition's tasks in series; about 1 ms on 200K LUT4 partitioned by
+256), while the streamed kernel spreads one instance's partition over a
+workgroup (64 us per cycle on the same design): on a lightly loaded GPU a
+small job finishes much sooner on it. Under load the batched context, which
+shares every program read among all its instances, gives the higher
+throughput, so `small_jobs` is 0 (off) by default. The engine runs in
+slices: every step makes one run per context with active slots, each active
+slot advancing min(remaining, slice) cycles.
 Requests that arrive while others run are admitted at the next step into
 slots that finished jobs freed, so jobs of one design overlap in the same
 launch and jobs of different designs alternate launch by launch. Admission
@@ -36,6 +45,7 @@ import numpy as np
 from . import _native
 
 KERNEL_BATCH = _native.KERNEL_BATCH
+KERNEL_STREAM = _native.KERNEL_STREAM
 POOL_REG, POOL_MEM, POOL_EXCHANGE = _native.POOL_REG, _native.POOL_MEM, _native.POOL_EXCHANGE
 
 
@@ -65,7 +75,7 @@ class Job:
 
 
 def gpu_backend(design, kernel=KERNEL_BATCH):
-    """A batched simulator context for a generated design (gen.Design)."""
+    """A simulator context for a generated design (gen.Design) on `kernel`."""
     from . import rt
     sim = rt.Simulator(0)
     if sim.init(0, design.design, design.symbols, False, False) != 0:
@@ -75,10 +85,12 @@ def gpu_backend(design, kernel=KERNEL_BATCH):
 
 
 class _Ctx:
-    """One resident design: a batched context with `capacity` instance slots."""
+    """One resident context of a design: `capacity` instance slots of one kind
+    ("batch": the batched kernel; "small": the streamed kernel)."""
 
     def __init__(self, key, design, sim, capacity, nbytes):
         self.key, self.design, self.sim, self.capacity, self.nbytes = key, design, sim, capacity, nbytes
+        self.design_key, self.kind = key
         self.free = list(range(capacity))
         self.jobs = []
         self.last_used = 0.0
@@ -96,10 +108,12 @@ class Engine:
     """
 
     def __init__(self, designs, budget=250 << 30, capacity=128, slice_cycles=2000, backend=gpu_backend,
-                 clock=time.perf_counter, park_after=1.0):
+                 clock=time.perf_counter, park_after=1.0, small_jobs=0, small_capacity=64):
         self.designs = designs
         self.budget = budget
         self.park_after = park_after
+        self.small_jobs = small_jobs
+        self.small_capacity = small_capacity
         self.capacity = capacity
         self.slice = slice_cycles
         self.backend = backend
@@ -121,9 +135,13 @@ class Engine:
         st = design.stats
         return int(st["reg_pool"]) + int(st["mem_pool"]) + int(st["exchange_pool"]) + 64
 
-    def context_bytes(self, design):
+    def context_bytes(self, design, capacity=None):
         """Program image (about the serialized size) plus all slots' state."""
-        return int(design.stats["design_bytes"]) + self.capacity * self.state_bytes(design)
+        cap = self.capacity if capacity is None else capacity
+        return int(design.stats["design_bytes"]) + cap * self.state_bytes(design)
+
+    def kind_of(self, job):
+        return "small" if self.small_jobs and job.instances <= self.small_jobs else "batch"
 
     def _make_room(self, need, now):
         while self.used + need > self.budget:
@@ -136,20 +154,22 @@ class Engine:
             del self.ctx[victim.key]
         return True
 
-    def _context(self, key, now):
+    def _context(self, name, kind, now):
+        key = (name, kind)
         c = self.ctx.get(key)
         if c is not None:
             return c
-        d = self.designs[key]
-        nbytes = self.context_bytes(d)
+        d = self.designs[name]
+        cap = self.small_capacity if kind == "small" else self.capacity
+        nbytes = self.context_bytes(d, cap)
         if nbytes > self.budget:
-            raise ValueError(f"design {key} alone needs {nbytes} B, more than the budget {self.budget} B")
+            raise ValueError(f"design {name} alone needs {nbytes} B, more than the budget {self.budget} B")
         if not self._make_room(nbytes, now):
             return None
-        sim = self.backend(d)
-        sim.set_instances(self.capacity)
+        sim = self.backend(d, KERNEL_STREAM if kind == "small" else KERNEL_BATCH)
+        sim.set_instances(cap)
         sim.set_stimulus(d.inputs, 0)
-        c = _Ctx(key, d, sim, self.capacity, nbytes)
+        c = _Ctx(key, d, sim, cap, nbytes)
         self.ctx[key] = c
         self.used += nbytes
         return c
@@ -183,16 +203,17 @@ class Engine:
             if job.arrival > now:
                 waiting.append(job)
                 continue
-            c = self._context(job.design, now)
+            kind = self.kind_of(job)
+            c = self._context(job.design, kind, now)
             since = job.arrival if job.waiting_since is None else job.waiting_since
             if c is None and now - since > self.park_after:
                 # only contexts whose jobs have all run at least park_after since
                 # they started or resumed (no ping-pong between two designs)
-                busy = [x for x in self.ctx.values() if x.jobs and x.key != job.design and
+                busy = [x for x in self.ctx.values() if x.jobs and x.design_key != job.design and
                         all(now - j.running_since >= self.park_after for j in x.jobs)]
                 if busy:
                     parked += self._park(min(busy, key=lambda x: x.last_used), now)
-                    c = self._context(job.design, now)
+                    c = self._context(job.design, kind, now)
             if c is None or len(c.free) < job.instances:
                 waiting.append(job)
                 continue
