@@ -86,6 +86,24 @@ def middle_levels():
     return Case("middle_levels", p.serialize(), r, m, 1, {16 + k: v for k, v in enumerate(want)})
 
 
+def last_fanout():
+    """Last level (SimEval.cu:422-447) sending one lane's value to several slots, next to an
+    immediate and a lane past the level's size; a second micropart of the level shares the
+    wave (lanes 32-63)."""
+    regs = [5, 9, 3, 12]
+    p = Prog(64, lut=TABLE)
+    P = p.add(0, Part(128))
+    P.regread = [(k, 20 + k) for k in range(4)]
+    P.level(lut_mp(top=[(L1, 20, 21, 22), (0, 0, 0, 23)],
+                   last=[(32, 64), (32, 65), (33, 66), (32, 67), (9, 68), (40, 69)]),
+            lut_mp(top=[(0, 0, 0, 21)], last=[(32, 70), (32, 71)]))
+    P.regwrite = (16, 64, 8)
+    a0 = T(L1 + (5 << 8 | 9 << 4 | 3))
+    want = [a0, a0, 12, a0, 9, 0, 9, 9]
+    r, m = _pools(64, 0, dict(enumerate(regs)))
+    return Case("last_fanout", p.serialize(), r, m, 1, {16 + k: v for k, v in enumerate(want)})
+
+
 def vector_read():
     """VecRead (SimEval.cu:449-481): off = (i0<<12|i1<<8|i2<<4|i3) + offset; inside the
     vector read vp/const, else outRange (< 16 immediate)."""
@@ -320,5 +338,5 @@ def lfsr16(n=100, seed=0xACE1):
 
 
 def all_cases():
-    return [top_operands(), middle_levels(), vector_read(), vector_ops(), memory(), word_copies(),
+    return [top_operands(), middle_levels(), last_fanout(), vector_read(), vector_ops(), memory(), word_copies(),
             print_stop(False), print_stop(True), counter64(), xorshift_in_logic(), lfsr16()]

@@ -39,8 +39,9 @@ def design(name):
 @pytest.mark.skipif(not os.path.exists(EXE), reason="stream_check not built")
 @pytest.mark.parametrize("dname", list(DESIGNS))
 @pytest.mark.parametrize("nc,groups,slots", [(2, 1, 1), (4, 3, 1), (4, 1000, 1), (2, 7, 1), (8, 3, 2), (8, 1000, 2),
-                                             (4, 2, 2), (8, 5, 4)])
+                                             (4, 2, 2), (8, 5, 4), (8, 3, -2), (8, 2, -4)])
 def test_stream_image_matches_oracle(tmp_path, dname, nc, groups, slots):
+    """slots < 0: the builder picks one or -slots per region, as the runtime does."""
     d = design(dname)
     (tmp_path / "d.bin").write_bytes(d.design)
     (tmp_path / "in.u32").write_bytes(np.asarray(d.inputs, np.uint32).tobytes())

@@ -106,12 +106,20 @@ enum StreamStep : uint32_t {
 };
 constexpr uint32_t kKindNone = 7;      // entry kind of a wave with no task in the step
 constexpr uint32_t kSplit = 1u << 7;   // entry flag: the waves are split over the slots
+constexpr uint32_t kLastDirect = 1u << 23;  // T_LUT2 entry flag (n = nMid uses bits 8-15): direct last level
 constexpr uint32_t kRow = 64;
 // Task rows, lane l of a row belongs to lane l of the wave:
 //   T_LUT2   (n = nMid, rows 3 + nMid): row 0 top word {lut | op0 << 16},
 //            row 1 top word {op1 | op2 << 16}, row 2 last {result | shuffleId
 //            << 16} (0xFFFFFFFF = no op), row 3 + m middle level m (packed
 //            on-disk op, 0 = no op); lanes 0-31 micropart A, 32-63 B.
+//            With kLastDirect, row 2 is indexed by the lane that holds the
+//            value instead: {slot | 0 << 16} writes the lane's own final value
+//            to slot, {slot | (1 + c) << 16} writes the constant c (an
+//            immediate shuffle id, or 0 for a lane past the last level's
+//            size), 0xFFFFFFFF nothing. The builder uses it when no lane's
+//            value goes to more than one slot; it saves the last level's
+//            shuffle.
 //   T_VECREAD (n ops, 5 rows), T_VECOP (3 rows), T_MEMREAD (4 rows): row k
 //            lane i = word k of op i in the wave64 layout above.
 // Partition block: one section of 1 + 2 * nJ rows per compute wave (row 0 =

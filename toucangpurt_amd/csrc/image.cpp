@@ -691,10 +691,53 @@ uint64_t lpt(const std::vector<uint64_t> &cost, uint32_t bins, std::vector<uint3
   return mx;
 }
 
+// Last level of a LUT2 task indexed by the lane that holds each value
+// (the kLastDirect row): lane (shuffleId mod 32) of the micropart's half
+// writes its own final value to the op's slot, so no shuffle is needed.
+// An immediate shuffle id, or a lane past the last level's size (whose
+// value is 0: its rows are zero words, which evaluate to 0), becomes a
+// constant write placed on a lane with no value to write. Returns false,
+// leaving `row` alone, when some lane's value goes to more than one slot or
+// two ops of the level write one slot (their order then stays the op order).
+bool directLastRow(const CGMicroPartInfo *A, const CGMicroPartInfo *B, uint32_t *row) {
+  uint32_t out[64];
+  std::fill(out, out + 64, 0xFFFFFFFFu);
+  std::vector<std::pair<uint32_t, uint32_t>> consts;  // (slot, value)
+  std::vector<uint32_t> slots;
+  for (const CGMicroPartInfo *m : {A, B})
+    if (m)
+      for (const auto &o : m->lastLevel) slots.push_back(o.result);
+  std::sort(slots.begin(), slots.end());
+  if (std::adjacent_find(slots.begin(), slots.end()) != slots.end()) return false;
+  for (uint32_t h = 0; h < 2; ++h) {
+    const CGMicroPartInfo *m = h ? B : A;
+    if (!m) continue;
+    const size_t nPrev = m->middleLevels.empty() ? m->topLevel.size() : m->middleLevels.back().size();
+    for (const auto &o : m->lastLevel) {
+      const uint32_t s = o.shuffleId & 31u;
+      if (o.shuffleId < 32 || s >= nPrev) {
+        consts.emplace_back(o.result, o.shuffleId < 32 ? o.shuffleId : 0u);
+        continue;
+      }
+      uint32_t &w = out[h * 32 + s];
+      if (w != 0xFFFFFFFFu) return false;
+      w = o.result;
+    }
+  }
+  uint32_t l = 0;
+  for (const auto &c : consts) {
+    while (l < 64 && out[l] != 0xFFFFFFFFu) ++l;
+    if (l == 64) return false;
+    out[l] = c.first | (c.second + 1) << 16;
+  }
+  std::copy(out, out + 64, row);
+  return true;
+}
+
 }  // namespace
 
 StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32_t nC, uint32_t groups,
-                             uint32_t slots) {
+                             uint32_t slots, bool forceSlots) {
   StreamImage img;
   img.nC = nC = std::max<uint32_t>(nC, 1);
   img.groups = groups = std::max<uint32_t>(groups, 1);
@@ -714,12 +757,13 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
     blockRows[L].resize(d.parts.size());
     waveTasks[L].resize(d.parts.size());
   }
-  std::vector<uint64_t> partCost(d.parts.size());
+  std::vector<uint64_t> partCost[2];  // per layout: the cost the partition assignment balances
+  for (auto &v : partCost) v.assign(d.parts.size(), 0);
   for (size_t pi = 0; pi < d.parts.size(); ++pi) {
     const auto &p = d.parts[pi];
     img.maxVp = std::max(img.maxVp, p.valuePoolSize);
     for (uint32_t L = 0; L < nLayouts; ++L) waveTasks[L][pi].resize(p.exec_mParts.size());
-    uint64_t cost = 0;
+    uint64_t cost[2] = {0, 0};
     for (size_t li = 0; li < p.exec_mParts.size(); ++li) {
       std::vector<StreamTask> tasks;
       std::vector<const CGMicroPartInfo *> luts;
@@ -763,7 +807,8 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
         };
         put(A, 0);
         if (B) put(B, 32);
-        tasks.push_back({off, T_LUT2 << 4 | nMid << 8 | rows << 24, 4 + 2ull * nMid});
+        const uint32_t direct = directLastRow(A, B, &W[off + 2 * kRow]) ? kLastDirect : 0u;
+        tasks.push_back({off, T_LUT2 << 4 | nMid << 8 | direct | rows << 24, 4 + 2ull * nMid});
       }
       auto vecTasks = [&](auto &ops, uint32_t kind, uint32_t rows, auto fill, auto costOf) {
         for (size_t i = 0; i < ops.size(); i += 64) {
@@ -809,7 +854,7 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
       for (uint32_t L = 0; L < nLayouts; ++L) {
         std::vector<uint32_t> binOf;
         const uint64_t span = lpt(tc, wavesOf[L], binOf);
-        if (L == nLayouts - 1) cost += 2 + span;  // priced as the partition mostly runs
+        cost[L] += 2 + span;
         auto &lv = waveTasks[L][pi][li];
         lv.resize(wavesOf[L]);
         for (size_t t = 0; t < tasks.size(); ++t) lv[binOf[t]].push_back(tasks[t]);
@@ -876,21 +921,82 @@ StreamImage buildStreamImage(const SimDesignInfo &d, const StimMap &stim, uint32
         for (const auto &o : p.ops_last_stop) W[q++] = o.en;
       }
     }
-    partCost[pi] = cost + 8 + 4 * nJ0 + (p.ops_last_memWrite.empty() ? 0 : 8);
+    for (uint32_t L = 0; L < nLayouts; ++L) partCost[L][pi] = cost[L] + 8 + 4 * nJ0 + (p.ops_last_memWrite.empty() ? 0 : 8);
   }
   if (img.consts.empty()) img.consts.push_back(0);
+  // Estimated time of a partition's steps in each layout, in LUT exec-level
+  // steps (measured on the streamed kernel: a LUT level ~1 us, partition
+  // begin ~2.4 us, end ~0.75 us, a level slice holding word ops ~1.9 us).
+  constexpr double kBeginSteps = 2.4, kEndSteps = 0.75, kWordSteps = 1.9, kBarrierSteps = 2.0;
+  // Per-step cost of a round whose partitions run side by side (measured on
+  // the 10M-LUT4 design: 1.12 for two slots, 1.18 for four).
+  const double splitFactor = slots == 2 ? 1.12 : 1.18;
+  std::vector<double> estOf[2];
+  for (uint32_t L = 0; L < nLayouts; ++L) {
+    estOf[L].assign(d.parts.size(), 0.0);
+    for (size_t pi = 0; pi < d.parts.size(); ++pi) {
+      double e = kBeginSteps + kEndSteps;
+      for (const auto &lv : waveTasks[L][pi]) {
+        size_t nSub = 1;
+        for (const auto &mine : lv) nSub = std::max(nSub, mine.size());
+        for (size_t q = 0; q < nSub; ++q) {
+          bool word = false;
+          for (const auto &mine : lv) word |= q < mine.size() && ((mine[q].y >> 4) & 7u) != T_LUT2;
+          e += word ? kWordSteps : 1.0;
+        }
+      }
+      estOf[L][pi] = e;
+    }
+  }
   // partitions -> (slot, workgroup) bins per region, slot-major so that every
-  // workgroup's first slot fills before any second slot; then the step streams
+  // workgroup's first slot fills before any second slot; each region takes
+  // the slot count (one or `slots`) with the shorter estimated critical path
+  // (regions end in a grid barrier, so each costs its slowest workgroup)
   const size_t bins = size_t(groups) * slots;
   auto binOfSlot = [&](uint32_t g, uint32_t s) { return size_t(s) * groups + g; };
   std::vector<std::vector<uint32_t>> per(bins * 2);
+  img.estCycle = 2 * kBarrierSteps;
   for (int r = 0; r < 2; ++r) {
     const auto &ids = d.regionPartitionIds[r];
-    std::vector<uint64_t> c(ids.size());
-    for (size_t k = 0; k < ids.size(); ++k) c[k] = partCost[ids[k]];
-    std::vector<uint32_t> binOf;
-    img.maxGroupCost += lpt(c, uint32_t(bins), binOf);
-    for (size_t k = 0; k < ids.size(); ++k) per[size_t(binOf[k]) * 2 + r].push_back(ids[k]);
+    struct Option {
+      uint32_t slots;
+      std::vector<uint32_t> binOf;
+      uint64_t maxCost;
+      double time;
+    };
+    std::vector<Option> opts;
+    for (uint32_t s : {slots, 1u}) {
+      if (!opts.empty() && (s == opts[0].slots || forceSlots)) continue;
+      Option o{s, {}, 0, 0.0};
+      std::vector<uint64_t> c(ids.size());
+      for (size_t k = 0; k < ids.size(); ++k) c[k] = partCost[s > 1 ? 1 : 0][ids[k]];
+      o.maxCost = lpt(c, groups * s, o.binOf);
+      std::vector<std::vector<uint32_t>> lists(size_t(groups) * s);
+      for (size_t k = 0; k < ids.size(); ++k) lists[o.binOf[k]].push_back(ids[k]);
+      for (uint32_t g = 0; g < groups; ++g) {
+        double t = 0;
+        for (size_t k = 0;; ++k) {
+          double mx = 0;
+          uint32_t occupied = 0, only = 0;
+          for (uint32_t q = 0; q < s; ++q) {
+            const auto &lst = lists[binOfSlot(g, q)];
+            if (k >= lst.size()) continue;
+            ++occupied;
+            only = lst[k];
+            mx = std::max(mx, estOf[nLayouts - 1][lst[k]]);
+          }
+          if (!occupied) break;
+          t += occupied > 1 ? splitFactor * mx : estOf[0][only];
+        }
+        o.time = std::max(o.time, t);
+      }
+      opts.push_back(std::move(o));
+    }
+    const Option &best = opts.size() > 1 && opts[1].time <= opts[0].time ? opts[1] : opts[0];
+    img.regionSlots[r] = best.slots;
+    img.maxGroupCost += best.maxCost;
+    img.estCycle += best.time;
+    for (size_t k = 0; k < ids.size(); ++k) per[size_t(best.binOf[k]) * 2 + r].push_back(ids[k]);
   }
   const uint32_t rec = 1 + 2 * nC;
   // region steps: the same entry for every wave

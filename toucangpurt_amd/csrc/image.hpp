@@ -71,12 +71,16 @@ struct StreamImage {
   std::vector<uint32_t> stepBase, stepCount;  // per group: first record, records per cycle
   std::vector<uint8_t> consts;
   uint32_t maxVp = 0, n0 = 0, nTotal = 0, groups = 0, nC = 0, slots = 1, maxSteps = 0;
+  uint32_t regionSlots[2] = {1, 1};  // slots used in each region (1 or `slots`)
   uint64_t maxGroupCost = 0, tasks = 0;
+  double estCycle = 0;  // estimated cycle time, in LUT exec-level steps
 };
-// `slots` partitions run side by side in a workgroup, on nC / slots compute
-// waves each (the partition-begin, level and partition-end steps of the
-// partitions assigned to the slots are merged).
+// Up to `slots` partitions run side by side in a workgroup, on nC / slots
+// compute waves each (the partition-begin, level and partition-end steps of
+// the partitions assigned to the slots are merged). With `forceSlots` every
+// region is laid out for `slots`; otherwise each region uses `slots` or one
+// slot, whichever gives it the shorter estimated critical path.
 StreamImage buildStreamImage(const toucanGPUSim::SimDesignInfo &d, const StimMap &stim, uint32_t nC,
-                             uint32_t groups, uint32_t slots);
+                             uint32_t groups, uint32_t slots, bool forceSlots = false);
 
 }  // namespace tgs

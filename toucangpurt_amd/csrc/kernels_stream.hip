@@ -56,7 +56,7 @@ __device__ __forceinline__ void lutPairS(const Pf &p, const uint32_t *__restrict
                                          const uint8_t *lut) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t hb = lane & 32u;
-  const uint32_t nMid = (p.ey >> 8) & 0xFFFFu;
+  const uint32_t nMid = (p.ey >> 8) & 0xFFu;
   const uint32_t x = p.r0, y = p.r1, wl = p.r2;
   uint32_t val;
   {
@@ -79,9 +79,16 @@ __device__ __forceinline__ void lutPairS(const Pf &p, const uint32_t *__restrict
   if (nMid > 1) mid(p.r4);
   if (nMid > 2) mid(p.r5);
   for (uint32_t m = 3; m < nMid; ++m) mid(W[p.ex + (3 + m) * kRow + lane]);
-  const uint32_t sid = (wl >> 16) & 0xFFu;
-  uint32_t r = __shfl(val, int((sid & 31u) | hb));
-  if (sid < 32) r = sid;
+  uint32_t r;
+  if (__builtin_amdgcn_readfirstlane(p.ey) & kLastDirect) {
+    // row 2 is indexed by the lane holding the value: no shuffle
+    const uint32_t k = wl >> 16;
+    r = k == 0 ? val : k - 1;
+  } else {
+    const uint32_t sid = (wl >> 16) & 0xFFu;
+    r = __shfl(val, int((sid & 31u) | hb));
+    if (sid < 32) r = sid;
+  }
   if (wl != 0xFFFFFFFFu) vp[wl & 0xFFFFu] = uint8_t(r);
 }
 

@@ -331,21 +331,21 @@ int buildStream(tgs_ctx *c) {
     nc = widest > 4 ? 8 : widest > 2 ? 4 : 2;
   }
   c->streamNCUsed = nc;
-  // Launch shape: partition slots per workgroup (1, 2 or 4 side by side) and
-  // workgroups (one or two per CU). Each candidate's image gives its
-  // critical path in steps (the longest workgroup step stream of a cycle);
-  // a step costs more when the waves split over slots or two workgroups
-  // share a CU (LDS traffic contends). Per-step factors measured on the
-  // 10M-LUT4 design: 1, 1.12 and 1.18 for 1, 2 and 4 slots; x1.4 for two
-  // workgroups per CU. The cheapest candidate that fits runs.
-  auto slotFactor = [](uint32_t sl) { return sl == 1 ? 1.0 : sl == 2 ? 1.12 : 1.18; };
+  // Launch shape: partition slots per workgroup (up to 1, 2 or 4 side by
+  // side, chosen per region by the image builder) and workgroups (one or two
+  // per CU). Each candidate's image estimates its cycle time (per region the
+  // slowest workgroup's steps, weighted by step type and by the cost of
+  // split steps); a step costs about 1.4x when two workgroups share a CU (the
+  // dependent LDS chains of both contend, measured on the 10M-LUT4 design).
+  // The cheapest candidate that fits runs.
   constexpr double kTwoPerCUStep = 1.4;
   std::vector<uint32_t> slotList{1};
   if (maxRegion > cus)  // some workgroup holds several partitions of a region
     for (uint32_t sl : {2u, 4u})
       if (nc % sl == 0 && nc / sl >= 2) slotList.push_back(sl);
-  if (const char *e = std::getenv("TGS_STREAM_SLOTS")) {
-    const uint32_t sl = uint32_t(std::max(1, std::min(std::atoi(e), int(nc))));
+  const char *forced = std::getenv("TGS_STREAM_SLOTS");  // every region laid out for this many slots
+  if (forced) {
+    const uint32_t sl = uint32_t(std::max(1, std::min(std::atoi(forced), int(nc))));
     slotList.assign(1, nc % sl ? 1u : sl);
   }
   std::vector<uint32_t> groupList;
@@ -362,7 +362,7 @@ int buildStream(tgs_ctx *c) {
   std::string why;
   for (uint32_t sl : slotList)
     for (uint32_t g : groupList) {
-      StreamImage img = buildStreamImage(c->design, c->stim, nc, g, sl);
+      StreamImage img = buildStreamImage(c->design, c->stim, nc, g, sl, forced != nullptr);
       const uint32_t vpBytes = up16(std::max<uint32_t>(img.maxVp, 16));
       const uint32_t l = streamLdsBytes(nc, sl, vpBytes, img.maxSteps);
       if (l + streamStaticLdsBytes(nc) > limit) {
@@ -383,7 +383,7 @@ int buildStream(tgs_ctx *c) {
         why = buf;
         continue;
       }
-      const double score = double(img.maxSteps) * slotFactor(sl) * (g > cus ? kTwoPerCUStep : 1.0);
+      const double score = img.estCycle * (g > cus ? kTwoPerCUStep : 1.0);
       if (!have || score < best) {
         have = true;
         best = score;

@@ -7,6 +7,7 @@
 // oracle. It is not linked into the runtime and never runs on the device path.
 //
 // Usage: stream_check <design.bin> <inputs.u32> <nc> <groups> <cycles> <seed> <out.bin> [init.bin] [slots]
+// (slots < 0: up to -slots, chosen per region by the builder as the runtime does)
 // init.bin, if given, holds the initial register pool followed by the memory pool.
 #include <algorithm>
 #include <cstdio>
@@ -68,7 +69,8 @@ struct Interp {
 
   void lutPair(const uint32_t *rec, uint32_t w) {
     const uint32_t ey = rec[2 + 2 * w], ex = rec[1 + 2 * w];
-    const uint32_t nMid = (ey >> 8) & 0xFFFFu;
+    const uint32_t nMid = (ey >> 8) & 0xFFu;
+    if ((ey >> 16) & 0x7Fu) die("LUT task entry bits 16-22 must be clear");
     uint32_t val[64], rows[64][kStreamRows];
     if (3 + nMid != (ey >> 24) && (ey >> 24) != 255) die("LUT task row count");
     for (uint32_t l = 0; l < 64; ++l)
@@ -93,9 +95,16 @@ struct Interp {
     }
     uint32_t out[64];
     bool wr[64];
+    const bool direct = (ey & kLastDirect) != 0;
     for (uint32_t l = 0; l < 64; ++l) {
       const uint32_t wl = rows[l][2], sid = (wl >> 16) & 0xFFu;
-      out[l] = sid < 32 ? sid : val[(sid & 31u) | (l & 32u)];
+      if (direct) {
+        const uint32_t k = wl >> 16;
+        if (wl != 0xFFFFFFFFu && k > 32) die("direct last-level constant out of range");
+        out[l] = k == 0 ? val[l] : k - 1;
+      } else {
+        out[l] = sid < 32 ? sid : val[(sid & 31u) | (l & 32u)];
+      }
       wr[l] = wl != 0xFFFFFFFFu;
     }
     for (uint32_t l = 0; l < 64; ++l)
@@ -279,8 +288,11 @@ int main(int argc, char **argv) {
   std::memcpy(ids.data(), inb.data(), ids.size() * 4);
   StimMap stim;
   if (buildStimMap(d, ids.data(), ids.size(), &stim, &err) != TGS_OK) die("stimulus: " + err);
-  const uint32_t slots = argc > 9 ? std::atoi(argv[9]) : 1;
-  StreamImage im = buildStreamImage(d, stim, nc, groups, slots);
+  const uint32_t slots = argc > 9 ? std::abs(std::atoi(argv[9])) : 1;
+  // slots given: every region laid out for them (a negative count lets the
+  // builder choose one or -slots per region, as the runtime does)
+  const bool force = argc <= 9 || std::atoi(argv[9]) > 0;
+  StreamImage im = buildStreamImage(d, stim, nc, groups, slots, force);
   if (im.slots != slots || nc % slots) die("slots must divide the compute waves");
   if (im.stepBase.size() != im.groups || im.stepCount.size() != im.groups) die("step tables");
   if (im.words.size() % kRow) die("image is not a whole number of rows");
