@@ -234,14 +234,15 @@ def run_batched(args):
     n = args.instances
     rng = np.random.default_rng(args.seed)
     cycles = rng.integers(args.min_cycles, args.max_cycles + 1, size=n).astype(np.uint32)
-    # autotune: partition size x workgroup size, on a short uniform run
+    # autotune the partition size on a short uniform run (the runtime's
+    # default workgroup shape: two 64-instance groups per workgroup)
     best = None
     tuning = []
     designs = {}
     for part in (256, 512, 1024, 2048, 4096):
         d = gen.generate("lut200k", seed=args.seed, part_luts=part)
         designs[part] = d
-        for block in (256, 512, 1024):
+        for block in (128,):
             sim = rt.Simulator(0)
             assert sim.init(0, d.design, d.symbols, False, False) == 0
             try:
