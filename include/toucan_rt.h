@@ -134,6 +134,18 @@ int tgs_run_batch(tgs_ctx *ctx, const uint32_t *cycles, uint32_t n, int *any_sto
  * one nibble per id, no per-nibble memcpy. */
 int tgs_read_regs(tgs_ctx *ctx, uint32_t instance, const uint32_t *ids, uint32_t n, uint8_t *out);
 int tgs_write_regs(tgs_ctx *ctx, uint32_t instance, const uint32_t *ids, uint32_t n, const uint8_t *in);
+/* Testbench step of every instance in one call: the reference's
+ * setRegister -> eval -> readRegister loop (src/ToucanSim.cu:206-289) makes a
+ * synchronous copy per nibble for each register and a synchronizing launch
+ * per cycle. Writes register write_ids[k] of instance i to
+ * write_vals[i * n_write + k] (ids distinct), runs `cycles` cycles of every
+ * instance (0: none), then reads register read_ids[k] of instance i into
+ * read_vals[i * n_read + k]. The ids and values travel through one pinned
+ * mailbox: one host-to-device copy and one scatter kernel before the run, one
+ * gather kernel and one device-to-host copy after it, one synchronization.
+ * *any_stopped (may be null): some instance's sticky stop flag is set. */
+int tgs_step(tgs_ctx *ctx, const uint32_t *write_ids, uint32_t n_write, const uint8_t *write_vals, uint32_t cycles,
+             const uint32_t *read_ids, uint32_t n_read, uint8_t *read_vals, int *any_stopped);
 /* Whole-pool access (checkpoint/restore of the architectural state). */
 int tgs_read_pool(tgs_ctx *ctx, uint32_t instance, int pool, uint64_t offset, uint64_t n, uint8_t *out);
 int tgs_write_pool(tgs_ctx *ctx, uint32_t instance, int pool, uint64_t offset, uint64_t n, const uint8_t *in);

@@ -100,5 +100,11 @@ hipError_t launchPermute(const uint8_t *in, uint8_t *out, uint64_t rows, uint32_
                          hipStream_t stream);
 hipError_t launchPermute64(const uint64_t *in, uint64_t *out, uint64_t rows, uint32_t instPad, const uint32_t *src,
                            hipStream_t stream);
+// Testbench mailbox: register nibbles ids[0..n) of nSlots instance slots to
+// (toPool) or from a dense buffer vals[instance * n + k]; nibble r of slot s
+// lives at pool[r * instPad + s], and slot s holds instance instOf[s]
+// (instOf null: slot s holds instance s).
+hipError_t launchRegMailbox(uint8_t *pool, uint32_t instPad, const uint32_t *instOf, uint32_t nSlots,
+                            const uint32_t *ids, uint32_t n, uint8_t *vals, bool toPool, hipStream_t stream);
 
 }  // namespace tgs

@@ -446,6 +446,32 @@ __global__ void slotColumn(uint8_t *pool, uint32_t instPad, uint32_t slot, uint6
   }
 }
 
+// Consecutive threads take consecutive slots of one register row, so the
+// pool side of an instance-minor layout is coalesced.
+__global__ void regMailbox(uint8_t *pool, uint32_t instPad, const uint32_t *instOf, uint32_t nSlots,
+                           const uint32_t *ids, uint32_t n, uint8_t *vals, int toPool) {
+  const uint64_t total = uint64_t(nSlots) * n;
+  for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t k = uint32_t(t / nSlots), s = uint32_t(t - uint64_t(k) * nSlots);
+    const uint32_t i = instOf ? instOf[s] : s;
+    uint8_t *p = pool + size_t(ids[k]) * instPad + s;
+    uint8_t *v = vals + size_t(i) * n + k;
+    if (toPool)
+      *p = *v;
+    else
+      *v = *p;
+  }
+}
+
+hipError_t launchRegMailbox(uint8_t *pool, uint32_t instPad, const uint32_t *instOf, uint32_t nSlots,
+                            const uint32_t *ids, uint32_t n, uint8_t *vals, bool toPool, hipStream_t stream) {
+  const uint64_t total = uint64_t(nSlots) * n;
+  if (total == 0) return hipSuccess;
+  const uint32_t grid = uint32_t(std::min<uint64_t>((total + 255) / 256, 1024));
+  regMailbox<<<dim3(grid), dim3(256), 0, stream>>>(pool, instPad, instOf, nSlots, ids, n, vals, toPool ? 1 : 0);
+  return hipGetLastError();
+}
+
 hipError_t launchSlotColumn(uint8_t *pool, uint32_t instPad, uint32_t slot, uint64_t rows, uint8_t *dense, bool toPool,
                             hipStream_t stream) {
   if (rows == 0) return hipSuccess;

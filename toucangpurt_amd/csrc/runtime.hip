@@ -163,6 +163,11 @@ struct tgs_ctx {
 
   tgs_profile prof{};
   SnapRing snap;
+  bool timingPending = false;  // the last launch's events are not accounted yet
+  // testbench mailbox (tgs_step): pinned host staging and its device twin
+  void *boxHost = nullptr;
+  size_t boxBytes = 0;
+  DBuf boxDev;
   // signal probes: request order -> CSR position (CSR = grouped by partition)
   std::vector<uint32_t> probeCsrOf;
   DBuf prOff, prSlot;
@@ -448,7 +453,18 @@ uint32_t maxCyclesPerLaunch(uint32_t groups) {
   return uint32_t(std::min<uint64_t>(m, 1u << 30));
 }
 
-int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles) {
+// Launch `cycles` cycles of one instance. With `wait` each launch is waited
+// for and timed at once; without it the last launch's events stay pending
+// (c->timingPending) and finishTiming() accounts them after the caller's own
+// synchronization, so a testbench step costs one round trip.
+int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
+  if (c->timingPending && cycles > 0) {  // a step left its launch untimed: account it before reusing the events
+    c->timingPending = false;
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->prof.kernel_ms += ms;
+  }
   while (cycles > 0) {
     int rc;
     if (c->kernel == TGS_KERNEL_PLAIN) {
@@ -530,12 +546,16 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles) {
       in.cycles += n;
       c->prof.cycles += n;
     }
+    c->prof.launches += 1;
+    c->prof.kernel = c->kernel;
+    if (!wait && cycles == 0) {
+      c->timingPending = true;
+      break;
+    }
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->prof.kernel_ms += ms;
-    c->prof.launches += 1;
-    c->prof.kernel = c->kernel;
     if (c->kernel == TGS_KERNEL_WAVE64 || c->kernel == TGS_KERNEL_STREAM) {
       uint64_t t[2];
       HIPCHK(c, hipMemcpy(t, in.ctl.as<uint32_t>() + C_BARTICKS, 16, hipMemcpyDeviceToHost));
@@ -985,6 +1005,7 @@ void tgs_destroy(tgs_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->snap.release();
+  if (c->boxHost) (void)hipHostFree(c->boxHost);
   c->inst.clear();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1256,6 +1277,108 @@ int tgs_write_regs(tgs_ctx *c, uint32_t instance, const uint32_t *ids, uint32_t 
   if ((rc = poolIO(c, instance, TGS_POOL_REG, 0, pool.size(), pool.data(), false))) return rc;
   for (uint32_t i = 0; i < n; ++i) pool[ids[i]] = vals[i];
   return poolIO(c, instance, TGS_POOL_REG, 0, pool.size(), pool.data(), true);
+}
+
+int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_t *writeVals, uint32_t cycles,
+             const uint32_t *readIds, uint32_t nRead, uint8_t *readVals, int *anyStopped) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if ((nWrite && (!writeIds || !writeVals)) || (nRead && (!readIds || !readVals))) return TGS_E_ARG;
+  const uint32_t regs = c->design.regPoolSize, ni = instanceCount(c);
+  for (uint32_t k = 0; k < nWrite; ++k)
+    if (writeIds[k] >= regs) return c->fail(TGS_E_ARG, "register id %u out of range", writeIds[k]);
+  for (uint32_t k = 0; k < nRead; ++k)
+    if (readIds[k] >= regs) return c->fail(TGS_E_ARG, "register id %u out of range", readIds[k]);
+  {
+    std::vector<uint32_t> w(writeIds, writeIds + nWrite);
+    std::sort(w.begin(), w.end());
+    if (std::adjacent_find(w.begin(), w.end()) != w.end())
+      return c->fail(TGS_E_ARG, "a register id is written twice in one step");
+  }
+  if ((rc = setDevice(c))) return rc;
+  // mailbox: [write ids][read ids][write values, per instance][read values,
+  // per instance][stop word per instance][barrier ticks of the last launch]
+  const size_t oRead = 4ull * nWrite, oWv = oRead + 4ull * nRead, oRv = oWv + size_t(ni) * nWrite;
+  const size_t oStop = (oRv + size_t(ni) * nRead + 7) & ~size_t(7), oTicks = oStop + ((4ull * ni + 7) & ~size_t(7));
+  const size_t bytes = oTicks + 16;
+  if (bytes > c->boxBytes) {
+    if (c->boxHost) HIPCHK(c, hipHostFree(c->boxHost));
+    c->boxHost = nullptr;
+    c->boxBytes = 0;
+    HIPCHK(c, hipHostMalloc(&c->boxHost, bytes, hipHostMallocDefault));
+    HIPCHK(c, c->boxDev.alloc(bytes));
+    c->boxBytes = bytes;
+  }
+  uint8_t *h = static_cast<uint8_t *>(c->boxHost), *d = c->boxDev.as<uint8_t>();
+  std::memcpy(h, writeIds, 4ull * nWrite);
+  std::memcpy(h + oRead, readIds, 4ull * nRead);
+  std::memcpy(h + oWv, writeVals, size_t(ni) * nWrite);
+  auto box = [&](const uint32_t *ids, uint32_t n, uint8_t *vals, bool toPool) -> int {
+    if (c->batched) {
+      HIPCHK(c, launchRegMailbox(c->bs.reg.as<uint8_t>(), c->bs.instPad, c->bs.instOfD.as<uint32_t>(), c->bs.n, ids,
+                                 n, vals, toPool, c->stream));
+    } else {
+      for (uint32_t i = 0; i < ni; ++i)
+        HIPCHK(c, launchRegMailbox(c->inst[i]->reg.as<uint8_t>(), 1, nullptr, 1, ids, n, vals + size_t(i) * n, toPool,
+                                   c->stream));
+    }
+    return TGS_OK;
+  };
+  if (nWrite) {
+    HIPCHK(c, hipMemcpyAsync(d, h, oRv, hipMemcpyHostToDevice, c->stream));
+    if ((rc = box(reinterpret_cast<const uint32_t *>(d), nWrite, d + oWv, true))) return rc;
+  } else if (nRead) {
+    HIPCHK(c, hipMemcpyAsync(d + oRead, h + oRead, 4ull * nRead, hipMemcpyHostToDevice, c->stream));
+  }
+  int any = 0;
+  if (c->batched) {
+    if (cycles) {
+      std::vector<uint32_t> all(ni, cycles);
+      if ((rc = tgs_run_batch(c, all.data(), ni, &any))) return rc;
+    } else {
+      for (uint32_t i = 0; i < ni && !any; ++i)
+        if ((rc = tgs_get_stop(c, i, &any))) return rc;
+    }
+  } else {
+    // the last launch is not waited for: its timing, the stop words and its
+    // barrier ticks come back with the read values
+    for (uint32_t i = 0; i < ni; ++i) {
+      if (cycles && (rc = runInstance(c, *c->inst[i], cycles, i + 1 < ni))) return rc;
+      HIPCHK(c, hipMemcpyAsync(d + oStop + 4ull * i, c->inst[i]->ctl.as<uint32_t>() + C_STOP, 4,
+                               hipMemcpyDeviceToDevice, c->stream));
+    }
+    if (c->timingPending)
+      HIPCHK(c, hipMemcpyAsync(d + oTicks, c->inst[ni - 1]->ctl.as<uint32_t>() + C_BARTICKS, 16,
+                               hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (nRead) {
+    if ((rc = box(reinterpret_cast<const uint32_t *>(d + oRead), nRead, d + oRv, false))) return rc;
+  }
+  // one copy back: read values, stop words and ticks
+  const size_t back0 = oRv, back1 = c->batched ? oRv + size_t(ni) * nRead : bytes;
+  if (back1 > back0) HIPCHK(c, hipMemcpyAsync(h + back0, d + back0, back1 - back0, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->timingPending) {
+    c->timingPending = false;
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->prof.kernel_ms += ms;
+    if (c->kernel == TGS_KERNEL_WAVE64 || c->kernel == TGS_KERNEL_STREAM) {
+      uint64_t t[2];
+      std::memcpy(t, h + oTicks, 16);
+      c->prof.barrier_ticks += t[0];
+      c->prof.total_ticks += t[1];
+    }
+  }
+  if (!c->batched)
+    for (uint32_t i = 0; i < ni; ++i) {
+      uint32_t w = 0;
+      std::memcpy(&w, h + oStop + 4ull * i, 4);
+      any |= w != 0;
+    }
+  if (nRead) std::memcpy(readVals, h + oRv, size_t(ni) * nRead);
+  if (anyStopped) *anyStopped = any;
+  return TGS_OK;
 }
 
 int tgs_get_stop(tgs_ctx *c, uint32_t instance, int *stopped) {

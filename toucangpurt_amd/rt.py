@@ -49,6 +49,7 @@ class Simulator:
             raise SimError(rc, "no usable GPU (cooperative launch required)")
         self.symbols = None
         self.cycle_cnt = 0
+        self._n_inst = 1
         self._reset_probes()
         self._design = None
         self._symbols_blob = None
@@ -94,6 +95,7 @@ class Simulator:
         self._symbols_blob = symbols
         self.symbols = parse_symbols(symbols) if symbols else None
         self.cycle_cnt = 0
+        self._n_inst = 1
         self._reset_probes()
         return 0
 
@@ -234,6 +236,10 @@ class Simulator:
 
     def set_instances(self, n):
         self._check(self._lib.tgs_set_instances(self._h, int(n)))
+        self._n_inst = int(n)
+
+    def instances(self):
+        return self._n_inst
 
     def set_stimulus(self, input_ids, seed):
         ids = np.ascontiguousarray(input_ids, dtype=np.uint32)
@@ -244,6 +250,23 @@ class Simulator:
         st = ctypes.c_int()
         self._check(self._lib.tgs_run_batch(self._h, c.ctypes.data_as(_u32p), c.size, ctypes.byref(st)))
         return bool(st.value)
+
+    def step(self, write_ids, write_vals, cycles, read_ids):
+        """One testbench step of every instance (tgs_step): write register
+        nibbles write_ids (write_vals shaped [instances, len(write_ids)]),
+        run `cycles` cycles, read register nibbles read_ids. Returns
+        (values [instances, len(read_ids)], any instance stopped)."""
+        wi = np.ascontiguousarray(write_ids, dtype=np.uint32).reshape(-1)
+        ri = np.ascontiguousarray(read_ids, dtype=np.uint32).reshape(-1)
+        n = self.instances()
+        wv = np.ascontiguousarray(write_vals, dtype=np.uint8).reshape(n, wi.size)
+        out = np.zeros((n, ri.size), dtype=np.uint8)
+        st = ctypes.c_int()
+        self._check(self._lib.tgs_step(self._h, wi.ctypes.data_as(_u32p), wi.size, wv.ctypes.data_as(_u8p),
+                                       int(cycles), ri.ctypes.data_as(_u32p), ri.size, out.ctypes.data_as(_u8p),
+                                       ctypes.byref(st)))
+        self.cycle_cnt += int(cycles)
+        return out, bool(st.value)
 
     def pool_size(self, pool):
         n = ctypes.c_uint64()
