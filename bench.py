@@ -49,8 +49,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["single", "batched", "serve", "both", "all"], default="all",
-                    help="all = single + batched + serving legs")
+    ap.add_argument("--mode", choices=["single", "batched", "serve", "testbench", "both", "all"], default="all",
+                    help="all = single + batched + serving + testbench legs")
+    ap.add_argument("--tb-steps", type=int, default=300, help="testbench steps timed per path")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals, jobs/s")
     ap.add_argument("--serve-designs", type=int, default=8)
@@ -375,6 +376,83 @@ def run_serve(args):
     return rep
 
 
+def run_testbench(args):
+    """Testbench leg: setRegister -> eval -> readRegister per step (4 registers
+    written, 4 read, one cycle) through the reference-style per-register calls
+    against tgs_step, on the 20K-LUT4 design (one instance on the streamed
+    kernel, 1,024 instances on the batched kernel)."""
+    import numpy as np
+    from toucangpurt_amd import KERNEL_BATCH, KERNEL_STREAM, gen, rt
+
+    def regs_of(sim):
+        names = [n for n in sorted(sim.symbols["reg"]) if len(sim.symbols["reg"][n]) <= 16][:8]
+        ids = [np.array([l[0] for n in ns for l in sim.symbols["reg"][n]], np.uint32) for ns in (names[:4], names[4:])]
+        return names[:4], names[4:], ids[0], ids[1]
+
+    out = {"design": "lut20k: 20,000 LUT4, depth 32", "registers_written": 4, "registers_read": 4, "cycles_per_step": 1}
+    d = gen.generate("lut20k", seed=args.seed)
+    sim = rt.Simulator(0)
+    assert sim.init(0, d.design, d.symbols, False, False) == 0
+    sim.set_kernel(KERNEL_STREAM)
+    win, rout, wids, rids = regs_of(sim)
+    vals = np.zeros((1, wids.size), np.uint8)
+
+    def slow_step(i):
+        for n in win:
+            sim.setRegister(n, i & 1)
+        sim.eval()
+        return [sim.readRegister(n) for n in rout]
+
+    for i in range(20):
+        slow_step(i)
+        sim.step(wids, vals, 1, rids)
+    n = args.tb_steps
+    t0 = time.perf_counter()
+    for i in range(n):
+        slow_step(i)
+    slow = (time.perf_counter() - t0) / n
+    t0 = time.perf_counter()
+    for i in range(n):
+        vals[:] = i & 1
+        sim.step(wids, vals, 1, rids)
+    fast = (time.perf_counter() - t0) / n
+    sim.cleanup()
+    out["single"] = {"instances": 1, "kernel": "stream", "per_register_calls_us": slow * 1e6, "tgs_step_us": fast * 1e6}
+    log(f"testbench: 1 instance {slow * 1e6:.0f} us per step with per-register calls, {fast * 1e6:.0f} us with tgs_step")
+    db = gen.generate("lut20k", seed=args.seed, part_luts=512)
+    sim = rt.Simulator(0)
+    assert sim.init(0, db.design, db.symbols, False, False) == 0
+    sim.set_kernel(KERNEL_BATCH)
+    ni = 1024
+    sim.set_instances(ni)
+    sim.set_stimulus(db.inputs, args.seed)
+    win, rout, wids, rids = regs_of(sim)
+    vals = np.zeros((ni, wids.size), np.uint8)
+    for _ in range(5):
+        sim.step(wids, vals, 1, rids)
+    m = max(args.tb_steps // 10, 5)
+    t0 = time.perf_counter()
+    for i in range(m):
+        vals[:] = i & 1
+        sim.step(wids, vals, 1, rids)
+    fast = (time.perf_counter() - t0) / m
+    t0 = time.perf_counter()
+    for k in range(ni):
+        for n in win:
+            sim.setRegister(n, 1, k)
+    sim.run_batch(np.ones(ni, np.uint32))
+    for k in range(ni):
+        for n in rout:
+            sim.readRegister(n, k)
+    slow = time.perf_counter() - t0
+    sim.cleanup()
+    out["batched"] = {"instances": ni, "kernel": "batched", "design": "lut20k with partitions of 512 LUT4",
+                      "per_register_calls_us": slow * 1e6, "tgs_step_us": fast * 1e6}
+    log(f"testbench: {ni} instances {slow * 1e3:.1f} ms per step with per-register calls, "
+        f"{fast * 1e6:.0f} us with tgs_step")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -384,6 +462,15 @@ def main():
     single = run_single(args) if args.mode in ("single", "both", "all") else None
     batched = run_batched(args) if args.mode in ("batched", "both", "all") else None
     serving = run_serve(args) if args.mode in ("serve", "all") else None
+    testbench = run_testbench(args) if args.mode in ("testbench", "all") else None
+    if single is None and batched is None and serving is None:
+        out = {"metric": "testbench step latency (write 4 registers, 1 cycle, read 4 registers)",
+               "value": testbench["single"]["tgs_step_us"], "unit": "us per step", "n_gpus": 1,
+               "steps": args.tb_steps, "warmup": 20, "ms_per_step": testbench["single"]["tgs_step_us"] / 1e3,
+               "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+               "data": "synthetic", "config": {"workload": testbench["design"]}, "testbench_step": testbench}
+        print(json.dumps(out))
+        return
     if single is None and batched is None:
         out = {"metric": "serving p50/p99 job latency and instance-cycles/s", "value": serving["cycles_per_s"],
                "unit": "instance-cycles/s", "n_gpus": 1, "steps": serving["steps"], "warmup": 0,
@@ -401,7 +488,7 @@ def main():
                "speedup_vs_plain": (single["value"] / single["plain"]["value"]) if single.get("plain") else None,
                "cpu_baseline": single.get("cpu"), "cpu_baseline_1thread": single.get("cpu_1thread"),
                "parity_vs_oracle": single.get("parity"), "parity_cycles": single.get("parity_cycles"),
-               "batched": batched, "serving": serving}
+               "batched": batched, "serving": serving, "testbench_step": testbench}
     else:
         out = {"metric": "batched stimulus-cycles/sec @200K-LUT design x instances", "value": batched["value"],
                "unit": "stimulus-cycles/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -409,7 +496,8 @@ def main():
                "vs_baseline": None, "dtype": "u8", "data": "synthetic",
                "config": {"workload": batched["workload"], "part_luts": batched["part_luts"]},
                "roofline": batched["roofline"], "cpu_baseline": batched.get("cpu"),
-               "speedup_vs_plain": batched.get("speedup_vs_plain"), "batched": batched, "serving": serving}
+               "speedup_vs_plain": batched.get("speedup_vs_plain"), "batched": batched, "serving": serving,
+               "testbench_step": testbench}
     print(json.dumps(out))
 
 
