@@ -1,12 +1,12 @@
 """Testbench stepping through tgs_step (write register nibbles -> run cycles ->
 read register nibbles, every instance in one call) against the oracle run
-step by step with the same writes, for one instance, several instances on
-the streamed kernel and a batched context; plus its argument checks."""
+step by step with the same writes, on every kernel family (one or several
+single-instance contexts, a batched context); plus its argument checks."""
 import numpy as np
 import pytest
 
 from simharness import OracleRun, assert_pools_equal, gpu_pools, gpu_sim
-from toucangpurt_amd import KERNEL_BATCH, KERNEL_STREAM, gen
+from toucangpurt_amd import KERNEL_BATCH, KERNEL_PLAIN, KERNEL_STREAM, KERNEL_WAVE64, gen
 from toucangpurt_amd.rt import SimError
 
 pytestmark = pytest.mark.gpu
@@ -29,7 +29,8 @@ def _ids(d, rng):
     return w, r
 
 
-@pytest.mark.parametrize("kernel,n", [(KERNEL_STREAM, 1), (KERNEL_STREAM, 3), (KERNEL_BATCH, 70)])
+@pytest.mark.parametrize("kernel,n", [(KERNEL_STREAM, 1), (KERNEL_STREAM, 3), (KERNEL_WAVE64, 1), (KERNEL_PLAIN, 2),
+                                      (KERNEL_BATCH, 70)])
 def test_step_matches_oracle(gpu_available, kernel, n):
     d = design()
     rng = np.random.default_rng(11 + n)
