@@ -179,6 +179,72 @@ def vector_ops():
     return Case("vector_ops", p.serialize(), r, m, 1, want)
 
 
+def _vec_value(nibs, n):
+    """Operand of SimEval.cu:491-503 as the reference forms it: OR of byte k << 4k over
+    __int128, then sign_extend_128 from bit 4n-1 when 0 < n < 32 (:222-228)."""
+    v = 0
+    for k in range(n):
+        v |= int(nibs[k]) << (4 * k)
+    v &= (1 << 128) - 1
+    if 0 < n < 32 and (v >> (4 * n - 1)) & 1:
+        v |= ((1 << 128) - 1) & ~((1 << (4 * n)) - 1)
+    return v - (1 << 128) if v >> 127 else v
+
+
+def vector_ops_narrow():
+    """VecArithOrLogic with operands of at most 16 nibbles (the ops of one level share a
+    wave): sign extension from bits 63, 31 and 3, compares that depend on bits above 63
+    (a top byte above 15 ORed in, as the reference's byte OR does), length 0 (value 0, no
+    nibbles written), an unknown op. Expected values follow the reference's __int128
+    arithmetic (SimEval.cu:482-548) restated here."""
+    ops_vals = {
+        "xa": _nibbles(0x8000000000000001, 16), "ya": _nibbles(0x7FFFFFFFFFFFFFFF, 16),
+        "xb": _nibbles(0x3123456789ABCDEF, 15) + [0x23], "yb": _nibbles(0x3123456789ABCDEF, 16),
+        "x8": _nibbles(0x12345678, 7) + [0x31], "y8": _nibbles(0x12345678, 8),
+        "n8": [0x8], "n7": [0x7],
+    }
+    regs, base, slot = {}, {}, 0
+    for name, nibs in ops_vals.items():
+        base[name] = 16 + slot
+        for k, v in enumerate(nibs):
+            regs[slot + k] = v
+        slot += len(nibs)
+    ADD, SUB, MUL, EQ, LT, LE = 0, 1, 2, 3, 4, 5
+    spec = [("xa", "ya", 16, ADD), ("xa", "ya", 16, SUB), ("xa", "ya", 16, MUL), ("xa", "ya", 16, LT),
+            ("xa", "ya", 16, LE), ("xa", "ya", 16, EQ), ("ya", "xa", 16, LT), ("xb", "yb", 16, EQ),
+            ("yb", "xb", 16, LT), ("xb", "yb", 16, LT), ("xb", "yb", 16, LE), ("xb", "yb", 16, SUB),
+            ("x8", "y8", 8, EQ), ("y8", "x8", 8, LT), ("x8", "y8", 8, ADD), ("x8", "y8", 8, MUL),
+            ("n8", "n7", 1, LT), ("n8", "n7", 1, ADD), ("xa", "ya", 0, EQ), ("xa", "ya", 0, LT),
+            ("xa", "ya", 16, 9)]
+    while (sum(n if op <= MUL else 1 for _, _, n, op in spec)) % 4:  # whole words for the register copy
+        spec.append(("xb", "xb", 16, EQ))
+    ops, want_slots, res = [], {}, 200
+    for a, b, n, op in spec:
+        ops.append((base[a], base[b], n, op, 0, res))
+        va, vb = _vec_value(ops_vals[a], n), _vec_value(ops_vals[b], n)
+        if op <= MUL:
+            r = (va + vb if op == ADD else va - vb if op == SUB else va * vb) % (1 << 128)
+            want_slots[res] = [(r >> (4 * k)) & 15 for k in range(n)]
+            res += n
+        else:
+            want_slots[res] = [int(va == vb) if op == EQ else int(va < vb) if op == LT
+                               else int(va <= vb) if op == LE else 0]
+            res += 1
+    ops.append((base["xa"], base["ya"], 0, ADD, 0, 500))  # length 0: writes nothing
+    count = res - 200
+    p = Prog(512, lut=TABLE)
+    P = p.add(0, Part(640))
+    P.regread = [(k, 16 + k) for k in range(slot)]
+    P.level(vecop_mp(ops))
+    P.regwrite = (256, 200, count)
+    want = {}
+    for sl, nibs in want_slots.items():
+        for k, v in enumerate(nibs):
+            want[256 + sl - 200 + k] = v
+    r, m = _pools(512, 0, regs)
+    return Case("vector_ops_narrow", p.serialize(), r, m, 1, want)
+
+
 def memory():
     """MemRead (SimEval.cu:550-575): disabled reads write nothing; address from 8 slots,
     x4 when hasMultipleWriter. MemWrite (:670-693): same address rule, dat < 16 immediate."""
@@ -338,5 +404,6 @@ def lfsr16(n=100, seed=0xACE1):
 
 
 def all_cases():
-    return [top_operands(), middle_levels(), last_fanout(), vector_read(), vector_ops(), memory(), word_copies(),
+    return [top_operands(), middle_levels(), last_fanout(), vector_read(), vector_ops(), vector_ops_narrow(), memory(),
+            word_copies(),
             print_stop(False), print_stop(True), counter64(), xorshift_in_logic(), lfsr16()]

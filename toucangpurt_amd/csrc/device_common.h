@@ -121,10 +121,45 @@ __device__ __forceinline__ unsigned __int128 vpNibbles(const uint8_t *vp, uint32
   return ((unsigned __int128)z3 << 96) | ((unsigned __int128)z2 << 64) | ((unsigned __int128)z1 << 32) | z0;
 }
 
+// vpNibbles for len <= 16: the low 64 bits of the same value, and in *high
+// bits 64-67, which only an out-of-range byte 15 (value above 15) reaches.
+__device__ __forceinline__ uint64_t vpNibbles64(const uint8_t *vp, uint32_t base, uint32_t len, uint32_t *high) {
+  const uint32_t *w = reinterpret_cast<const uint32_t *>(vp) + (base >> 2);
+  const uint32_t skip = base & 3u;
+  uint32_t x[5];
+#pragma unroll
+  for (uint32_t i = 0; i < 5; ++i) x[i] = w[i];
+  uint32_t c[4];
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i) {
+    uint32_t y = __builtin_amdgcn_alignbyte(x[i + 1], x[i], skip);
+    const int rem = int(len) - int(4 * i);
+    y &= rem >= 4 ? ~0u : rem <= 0 ? 0u : (1u << (8 * rem)) - 1u;
+    const uint32_t even = (y & 0xFFu) | ((y >> 8) & 0xFF00u);
+    const uint32_t odd = ((y >> 8) & 0xFFu) | ((y >> 16) & 0xFF00u);
+    c[i] = even | (odd << 4);
+  }
+  *high = c[3] >> 16;
+  const uint32_t z0 = c[0] | (c[1] << 16);
+  const uint32_t z1 = (c[1] >> 16) | c[2] | (c[3] << 16);
+  return (uint64_t(z1) << 32) | z0;
+}
+
 // Result of a VecArithOrLogic ADD/SUB/MUL: nibbles 0..len-1 of r to value
 // slots res..res+len-1 (SimEval.cu:527-534).
 __device__ __forceinline__ void vpPutNibbles(uint8_t *vp, uint32_t res, uint32_t len, unsigned __int128 r) {
   for (uint32_t k = 0; k < len; k += 8) {  // eight nibbles per 32-bit word of r
+    const uint32_t w = uint32_t(r);
+    r >>= 32;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+      if (k + j < len) vp[uint16_t(res + k + j)] = uint8_t((w >> (4 * j)) & 15u);
+  }
+}
+
+// The same for results that fit 64 bits (len <= 16).
+__device__ __forceinline__ void vpPutNibbles(uint8_t *vp, uint32_t res, uint32_t len, uint64_t r) {
+  for (uint32_t k = 0; k < len; k += 8) {
     const uint32_t w = uint32_t(r);
     r >>= 32;
 #pragma unroll

@@ -109,9 +109,39 @@ __device__ __forceinline__ void vecReadS(const Pf &p, uint8_t *vp, const uint8_t
 // VecArithOrLogic (SimEval.cu:482-548), one op per lane.
 __device__ __forceinline__ void vecOpS(const Pf &p, uint8_t *vp, const uint8_t *consts) {
   const uint32_t lane = threadIdx.x & 63u;
-  if (lane >= ((p.ey >> 8) & 0xFFFFu)) return;
+  const bool act = lane < ((p.ey >> 8) & 0xFFFFu);
   const uint32_t b1 = p.r0 & 0xFFFFu, b2 = p.r0 >> 16;
   const uint32_t len = p.r1 & 0xFFFFu, name = (p.r1 >> 16) & 0xFFu, cf = p.r1 >> 24, res = p.r2;
+  if (__ballot(act && (len > 16 || cf != 0)) == 0) {
+    // every op of the wave reads at most 16 nibbles from the value pool: the
+    // 128-bit value is (high, low) with low its low 64 bits; ADD/SUB/MUL
+    // results only keep low bits, compares use both halves
+    if (!act) return;
+    uint32_t e1, e2;
+    uint64_t l1 = vpNibbles64(vp, b1, len, &e1), l2 = vpNibbles64(vp, b2, len, &e2);
+    int64_t h1 = int64_t(e1), h2 = int64_t(e2);
+    if (len > 0) {  // sign_extend_128 (SimEval.cu:222-228) from bit 4 len - 1 < 64
+      const uint32_t bit = 4 * len - 1;
+      const uint64_t ext = bit == 63 ? 0 : ~((uint64_t(1) << (bit + 1)) - 1);
+      if ((l1 >> bit) & 1) {
+        l1 |= ext;
+        h1 = -1;
+      }
+      if ((l2 >> bit) & 1) {
+        l2 |= ext;
+        h2 = -1;
+      }
+    }
+    if (name <= 2) {
+      const uint64_t r = name == 0 ? l1 + l2 : name == 1 ? l1 - l2 : l1 * l2;
+      vpPutNibbles(vp, res, len, r);
+    } else {
+      const bool eq = h1 == h2 && l1 == l2, lt = h1 < h2 || (h1 == h2 && l1 < l2);
+      vp[res] = uint8_t(name == 3 ? eq : name == 4 ? lt : name == 5 ? (lt || eq) : 0);
+    }
+    return;
+  }
+  if (!act) return;
   unsigned __int128 v1, v2;
   if (cf == 0) {
     v1 = vpNibbles(vp, b1, len);
