@@ -87,6 +87,12 @@ int tgs_initial_state(const uint8_t *design, size_t design_size, const uint8_t *
 
 /* setupGPU (src/ToucanSim.cu:16-100): select device, check cooperative launch. */
 int tgs_create(int device, tgs_ctx **out);
+/* The same on the caller's stream (a hipStream_t of `device`, passed as a
+ * pointer): every copy and launch of the context is ordered on it, so the
+ * simulator interleaves with the caller's own work (the reference runs on the
+ * default stream, src/ToucanSim.cu:206-267). The stream stays the caller's:
+ * tgs_destroy does not destroy it, and it must outlive the context. */
+int tgs_create_on_stream(int device, void *stream, tgs_ctx **out);
 /* cleanup (src/ToucanSim.cu:302-304, a no-op there): frees all device memory. */
 void tgs_destroy(tgs_ctx *ctx);
 /* Text of the last error on this context (empty if none). */

@@ -152,7 +152,8 @@ def gen_lib():
 
 # Exported symbols of libtoucan_rt.so, as declared in include/toucan_rt.h.
 RT_SYMBOLS = [
-    "tgs_program_stats_from_bytes", "tgs_initial_state", "tgs_create", "tgs_destroy", "tgs_last_error", "tgs_load",
+    "tgs_program_stats_from_bytes", "tgs_initial_state", "tgs_create", "tgs_create_on_stream", "tgs_destroy",
+    "tgs_last_error", "tgs_load",
     "tgs_load_files", "tgs_set_kernel", "tgs_set_launch", "tgs_set_print", "tgs_set_instances", "tgs_set_stimulus",
     "tgs_eval", "tgs_eval_free_running", "tgs_run_batch", "tgs_read_regs", "tgs_write_regs", "tgs_step",
     "tgs_read_pool", "tgs_write_pool", "tgs_pool_size", "tgs_reset_instance", "tgs_snapshot_begin",
@@ -177,6 +178,7 @@ def rt_lib():
         lib.tgs_initial_state.argtypes = [_u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32,
                                           _u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t]
         lib.tgs_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        lib.tgs_create_on_stream.argtypes = [ctypes.c_int, vp, ctypes.POINTER(vp)]
         lib.tgs_destroy.argtypes = [vp]
         lib.tgs_destroy.restype = None
         lib.tgs_last_error.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]

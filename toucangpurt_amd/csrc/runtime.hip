@@ -119,6 +119,7 @@ uint64_t seedFor(uint64_t seed, uint32_t instance) {
 struct tgs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  bool ownStream = true;  // false: the caller's stream (tgs_create_on_stream), not destroyed here
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipDeviceProp_t prop{};
   std::string err;
@@ -973,7 +974,7 @@ int tgs_initial_state(const uint8_t *design, size_t designSize, const uint8_t *s
   return TGS_OK;
 }
 
-int tgs_create(int device, tgs_ctx **out) {
+static int createCtx(int device, hipStream_t userStream, tgs_ctx **out) {
   if (!out) return TGS_E_ARG;
   *out = nullptr;
   int n = 0;
@@ -988,7 +989,13 @@ int tgs_create(int device, tgs_ctx **out) {
   if (hipGetDeviceProperties(&c->prop, device) != hipSuccess) return TGS_E_NODEV;
   // setupGPU (ToucanSim.cu:40-43): the cycle kernels need cooperative launch.
   if (!c->prop.cooperativeLaunch) return TGS_E_NODEV;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return TGS_E_HIP;
+  if (userStream) {
+    c->stream = userStream;
+    c->ownStream = false;
+  } else if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    c->stream = nullptr;
+    return TGS_E_HIP;
+  }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return TGS_E_HIP;
   const char *blk = std::getenv("TGS_W64_BLOCK");
   if (blk) c->w64Block = std::max(64u, std::min(1024u, uint32_t(std::atoi(blk)) / 64 * 64));
@@ -1000,6 +1007,13 @@ int tgs_create(int device, tgs_ctx **out) {
   return TGS_OK;
 }
 
+int tgs_create(int device, tgs_ctx **out) { return createCtx(device, nullptr, out); }
+
+int tgs_create_on_stream(int device, void *stream, tgs_ctx **out) {
+  if (!stream) return TGS_E_ARG;
+  return createCtx(device, static_cast<hipStream_t>(stream), out);
+}
+
 void tgs_destroy(tgs_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
@@ -1009,7 +1023,7 @@ void tgs_destroy(tgs_ctx *c) {
   c->inst.clear();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && c->ownStream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 

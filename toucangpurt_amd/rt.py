@@ -41,10 +41,16 @@ def program_stats(design: bytes) -> dict:
 class Simulator:
     """One design resident on one GPU, with any number of stimulus instances."""
 
-    def __init__(self, gpu_id=0):
+    def __init__(self, gpu_id=0, stream=None):
+        """stream: a hipStream_t of gpu_id (as an int or c_void_p) to run on;
+        the caller keeps it alive and destroys it (tgs_create_on_stream)."""
         self._lib = _native.rt_lib()
         self._h = ctypes.c_void_p()
-        rc = self._lib.tgs_create(gpu_id, ctypes.byref(self._h))
+        if stream is not None:
+            rc = self._lib.tgs_create_on_stream(gpu_id, ctypes.c_void_p(int(getattr(stream, "value", stream) or 0)),
+                                                ctypes.byref(self._h))
+        else:
+            rc = self._lib.tgs_create(gpu_id, ctypes.byref(self._h))
         if rc != TGS_OK:
             raise SimError(rc, "no usable GPU (cooperative launch required)")
         self.symbols = None
