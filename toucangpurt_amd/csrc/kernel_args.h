@@ -103,8 +103,11 @@ hipError_t launchPermute64(const uint64_t *in, uint64_t *out, uint64_t rows, uin
 // Testbench mailbox: register nibbles ids[0..n) of nSlots instance slots to
 // (toPool) or from a dense buffer vals[instance * n + k]; nibble r of slot s
 // lives at pool[r * instPad + s], and slot s holds instance instOf[s]
-// (instOf null: slot s holds instance s).
+// (instOf null: slot s holds instance s). With ctlWords > 0, thread 0 also
+// copies ctlWords words from ctlSrc to ctlDst (a single instance's control
+// words read back with its registers).
 hipError_t launchRegMailbox(uint8_t *pool, uint32_t instPad, const uint32_t *instOf, uint32_t nSlots,
-                            const uint32_t *ids, uint32_t n, uint8_t *vals, bool toPool, hipStream_t stream);
+                            const uint32_t *ids, uint32_t n, uint8_t *vals, bool toPool, hipStream_t stream,
+                            const uint32_t *ctlSrc = nullptr, uint32_t *ctlDst = nullptr, uint32_t ctlWords = 0);
 
 }  // namespace tgs

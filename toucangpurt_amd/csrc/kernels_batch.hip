@@ -449,7 +449,9 @@ __global__ void slotColumn(uint8_t *pool, uint32_t instPad, uint32_t slot, uint6
 // Consecutive threads take consecutive slots of one register row, so the
 // pool side of an instance-minor layout is coalesced.
 __global__ void regMailbox(uint8_t *pool, uint32_t instPad, const uint32_t *instOf, uint32_t nSlots,
-                           const uint32_t *ids, uint32_t n, uint8_t *vals, int toPool) {
+                           const uint32_t *ids, uint32_t n, uint8_t *vals, int toPool, const uint32_t *ctlSrc,
+                           uint32_t *ctlDst, uint32_t ctlWords) {
+  if (blockIdx.x == 0 && threadIdx.x < ctlWords) ctlDst[threadIdx.x] = ctlSrc[threadIdx.x];
   const uint64_t total = uint64_t(nSlots) * n;
   for (uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total; t += uint64_t(gridDim.x) * blockDim.x) {
     const uint32_t k = uint32_t(t / nSlots), s = uint32_t(t - uint64_t(k) * nSlots);
@@ -464,11 +466,14 @@ __global__ void regMailbox(uint8_t *pool, uint32_t instPad, const uint32_t *inst
 }
 
 hipError_t launchRegMailbox(uint8_t *pool, uint32_t instPad, const uint32_t *instOf, uint32_t nSlots,
-                            const uint32_t *ids, uint32_t n, uint8_t *vals, bool toPool, hipStream_t stream) {
+                            const uint32_t *ids, uint32_t n, uint8_t *vals, bool toPool, hipStream_t stream,
+                            const uint32_t *ctlSrc, uint32_t *ctlDst, uint32_t ctlWords) {
   const uint64_t total = uint64_t(nSlots) * n;
-  if (total == 0) return hipSuccess;
-  const uint32_t grid = uint32_t(std::min<uint64_t>((total + 255) / 256, 1024));
-  regMailbox<<<dim3(grid), dim3(256), 0, stream>>>(pool, instPad, instOf, nSlots, ids, n, vals, toPool ? 1 : 0);
+  if (ctlWords > 256) return hipErrorInvalidValue;
+  if (total == 0 && ctlWords == 0) return hipSuccess;
+  const uint32_t grid = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>((total + 255) / 256, 1024)));
+  regMailbox<<<dim3(grid), dim3(256), 0, stream>>>(pool, instPad, instOf, nSlots, ids, n, vals, toPool ? 1 : 0,
+                                                   ctlSrc, ctlDst, ctlWords);
   return hipGetLastError();
 }
 

@@ -1311,10 +1311,12 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
   }
   if ((rc = setDevice(c))) return rc;
   // mailbox: [write ids][read ids][write values, per instance][read values,
-  // per instance][stop word per instance][barrier ticks of the last launch]
+  // per instance][control words 0..kCtlBack-1 per instance: stop flag,
+  // barrier ticks of the last launch]
+  constexpr uint32_t kCtlBack = C_TOTTICKS + 2;
   const size_t oRead = 4ull * nWrite, oWv = oRead + 4ull * nRead, oRv = oWv + size_t(ni) * nWrite;
-  const size_t oStop = (oRv + size_t(ni) * nRead + 7) & ~size_t(7), oTicks = oStop + ((4ull * ni + 7) & ~size_t(7));
-  const size_t bytes = oTicks + 16;
+  const size_t oCtl = (oRv + size_t(ni) * nRead + 7) & ~size_t(7);
+  const size_t bytes = oCtl + 4ull * kCtlBack * ni;
   if (bytes > c->boxBytes) {
     if (c->boxHost) HIPCHK(c, hipHostFree(c->boxHost));
     c->boxHost = nullptr;
@@ -1354,19 +1356,20 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
         if ((rc = tgs_get_stop(c, i, &any))) return rc;
     }
   } else {
-    // the last launch is not waited for: its timing, the stop words and its
-    // barrier ticks come back with the read values
-    for (uint32_t i = 0; i < ni; ++i) {
+    // the last launch is not waited for: its timing, the stop flags and its
+    // barrier ticks come back with the read values (the gather copies each
+    // instance's control words)
+    for (uint32_t i = 0; i < ni; ++i)
       if (cycles && (rc = runInstance(c, *c->inst[i], cycles, i + 1 < ni))) return rc;
-      HIPCHK(c, hipMemcpyAsync(d + oStop + 4ull * i, c->inst[i]->ctl.as<uint32_t>() + C_STOP, 4,
-                               hipMemcpyDeviceToDevice, c->stream));
-    }
-    if (c->timingPending)
-      HIPCHK(c, hipMemcpyAsync(d + oTicks, c->inst[ni - 1]->ctl.as<uint32_t>() + C_BARTICKS, 16,
-                               hipMemcpyDeviceToDevice, c->stream));
   }
-  if (nRead) {
-    if ((rc = box(reinterpret_cast<const uint32_t *>(d + oRead), nRead, d + oRv, false))) return rc;
+  if (c->batched) {
+    if (nRead && (rc = box(reinterpret_cast<const uint32_t *>(d + oRead), nRead, d + oRv, false))) return rc;
+  } else {
+    for (uint32_t i = 0; i < ni; ++i)
+      HIPCHK(c, launchRegMailbox(c->inst[i]->reg.as<uint8_t>(), 1, nullptr, 1,
+                                 reinterpret_cast<const uint32_t *>(d + oRead), nRead, d + oRv + size_t(i) * nRead,
+                                 false, c->stream, c->inst[i]->ctl.as<uint32_t>(),
+                                 reinterpret_cast<uint32_t *>(d + oCtl) + size_t(kCtlBack) * i, kCtlBack));
   }
   // one copy back: read values, stop words and ticks
   const size_t back0 = oRv, back1 = c->batched ? oRv + size_t(ni) * nRead : bytes;
@@ -1379,7 +1382,7 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
     c->prof.kernel_ms += ms;
     if (c->kernel == TGS_KERNEL_WAVE64 || c->kernel == TGS_KERNEL_STREAM) {
       uint64_t t[2];
-      std::memcpy(t, h + oTicks, 16);
+      std::memcpy(t, h + oCtl + 4ull * (size_t(kCtlBack) * (ni - 1) + C_BARTICKS), 16);
       c->prof.barrier_ticks += t[0];
       c->prof.total_ticks += t[1];
     }
@@ -1387,7 +1390,7 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
   if (!c->batched)
     for (uint32_t i = 0; i < ni; ++i) {
       uint32_t w = 0;
-      std::memcpy(&w, h + oStop + 4ull * i, 4);
+      std::memcpy(&w, h + oCtl + 4ull * (size_t(kCtlBack) * i + C_STOP), 4);
       any |= w != 0;
     }
   if (nRead) std::memcpy(readVals, h + oRv, size_t(ni) * nRead);
