@@ -13,6 +13,7 @@
 #include <string>
 #include <tuple>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "SimConfigs.h"
@@ -37,6 +38,14 @@ class ToucanSimulator {
   uint64_t readSignal(const std::string &signalName);
   void cleanup();
   void printProfile();
+
+  // Extension: one testbench step in one runtime call (tgs_step): set the
+  // registers `writes` (name, value; as setRegister), run `cycles` cycles,
+  // read the registers `reads` into `out` (as readRegister, same order).
+  // Returns the sticky stop flag like eval(). With a VCD dump open it runs
+  // setRegister / eval / readRegister so every cycle is dumped.
+  bool step(const std::vector<std::pair<std::string, uint64_t>> &writes, uint32_t cycles,
+            const std::vector<std::string> &reads, std::vector<uint64_t> &out);
 
   // Extensions: kernel family (TGS_KERNEL_*), stimulus, last runtime error.
   int setKernel(int kernel);

@@ -37,6 +37,8 @@ def test_cpp_simulator_class(gpu_available, tmp_path, kernel):
             _, name, val = line.split()
             sig[name] = int(val)
     assert "cycles 10" in out.stdout
+    # ToucanSimulator::step equals setRegister / eval / readRegister (0+1+2+3 cycles)
+    assert "step ok 6" in out.stdout
     syms = parse_symbols(d.symbols)["reg"]
     orc = OracleRun(d.design, d.symbols, True)
     reg = orc.st.pool(0)

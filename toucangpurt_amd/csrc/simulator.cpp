@@ -162,6 +162,54 @@ uint64_t ToucanSimulator::readRegister(const std::string &signalName) {
   return r;
 }
 
+bool ToucanSimulator::step(const std::vector<std::pair<std::string, uint64_t>> &writes, uint32_t cycles,
+                           const std::vector<std::string> &reads, std::vector<uint64_t> &out) {
+  out.assign(reads.size(), 0);
+  if (dumpVCD) {  // every cycle goes through eval() so the dump sees it
+    for (auto &w : writes) setRegister(w.first, w.second);
+    int stopped = 0;
+    for (uint32_t k = 0; k < cycles; ++k) stopped = eval() ? 1 : 0;
+    if (cycles == 0 && tgs_get_stop(ctx, 0, &stopped) != TGS_OK) stopped = 0;
+    for (size_t k = 0; k < reads.size(); ++k) out[k] = readRegister(reads[k]);
+    return stopped != 0;
+  }
+  std::vector<uint32_t> wids, rids, rlen;
+  std::vector<uint8_t> wvals;
+  for (auto &w : writes) {
+    assert(symbols.regDebugInfo.count(w.first));
+    auto &locs = symbols.regDebugInfo[w.first];
+    assert(locs.size() <= 16 && "Register wider than 64 bits are not supported by this function");
+    uint64_t v = w.second;
+    const size_t base = wvals.size();
+    wvals.resize(base + locs.size());
+    for (auto &l : locs) wids.push_back(std::get<0>(l));
+    for (size_t k = locs.size(); k-- > 0;) {
+      wvals[base + k] = uint8_t(v & 0xF);
+      v >>= 4;
+    }
+    assert(v == 0 && "Given value is wider than register");
+  }
+  for (auto &r : reads) {
+    assert(symbols.regDebugInfo.count(r));
+    auto &locs = symbols.regDebugInfo[r];
+    assert(locs.size() <= 16 && "Register wider than 64 bits are not supported by this function");
+    for (auto &l : locs) rids.push_back(std::get<0>(l));
+    rlen.push_back(uint32_t(locs.size()));
+  }
+  std::vector<uint8_t> rvals(rids.size());
+  int stopped = 0;
+  if (tgs_step(ctx, wids.data(), uint32_t(wids.size()), wvals.data(), cycles, rids.data(), uint32_t(rids.size()),
+               rvals.data(), &stopped) != TGS_OK) {
+    std::cerr << "step failed: " << lastError() << std::endl;
+    std::abort();
+  }
+  if (cycles) flushPrints();
+  cycle_cnt += cycles;
+  for (size_t k = 0, at = 0; k < reads.size(); at += rlen[k], ++k)
+    for (uint32_t j = 0; j < rlen[k]; ++j) out[k] = (out[k] << 4) | rvals[at + j];
+  return stopped != 0;
+}
+
 // Probe list management: every fragment of a probed signal is one
 // (partition, value slot) probe; probeFirst maps a signal to its first probe.
 uint32_t ToucanSimulator::addSignalProbes(const std::vector<std::string> &names) {
