@@ -52,3 +52,26 @@ def test_rounds_match_oracle(gpu_available, block, groups, slots):
         sim.cleanup()
     finally:
         del os.environ["TGS_STREAM_SLOTS"]
+
+
+@pytest.mark.parametrize("block,groups", [(704, 3), (704, 7), (448, 4)])
+def test_rounds_auto_slots_match_oracle(gpu_available, block, groups):
+    """Slots chosen by the runtime (per region, by the estimated critical path)."""
+    d = design()
+    print(f"case stream rounds auto slots block={block} groups={groups}", flush=True)
+    assert "TGS_STREAM_SLOTS" not in os.environ
+    sim = rt.Simulator(0)
+    assert sim.init(0, d.design, d.symbols, False, True) == 0
+    sim.set_kernel(KERNEL_STREAM)
+    sim.set_launch(KERNEL_STREAM, block, groups)
+    sim.set_stimulus(d.inputs, 29)
+    sim.eval_free_running(9)
+    prof = sim.profile()
+    assert prof["grid"] == groups and prof["block"] == block
+    orc = OracleRun(d.design, inputs=d.inputs, seed=29)
+    orc.run(9, enable_print=True)
+    reg, mem, _ = orc.pools()
+    np.testing.assert_array_equal(sim.read_pool(POOL_REG), reg)
+    np.testing.assert_array_equal(sim.read_pool(POOL_MEM), mem)
+    assert sim.print_log() == sorted(orc.st.print_log())
+    sim.cleanup()

@@ -350,15 +350,21 @@ int buildStream(tgs_ctx *c) {
   // dependent LDS chains of both contend, measured on the 10M-LUT4 design).
   // The cheapest candidate that fits runs.
   constexpr double kTwoPerCUStep = 1.4;
-  std::vector<uint32_t> slotList{1};
-  if (maxRegion > cus)  // some workgroup holds several partitions of a region
-    for (uint32_t sl : {2u, 4u})
-      if (nc % sl == 0 && nc / sl >= 2) slotList.push_back(sl);
+  // slot counts worth building for `g` workgroups: more than one only when
+  // some workgroup holds several partitions of a region
   const char *forced = std::getenv("TGS_STREAM_SLOTS");  // every region laid out for this many slots
-  if (forced) {
-    const uint32_t sl = uint32_t(std::max(1, std::min(std::atoi(forced), int(nc))));
-    slotList.assign(1, nc % sl ? 1u : sl);
-  }
+  auto slotsFor = [&](uint32_t g) {
+    std::vector<uint32_t> l{1};
+    if (forced) {
+      const uint32_t sl = uint32_t(std::max(1, std::min(std::atoi(forced), int(nc))));
+      l.assign(1, nc % sl ? 1u : sl);
+      return l;
+    }
+    if (maxRegion > g)
+      for (uint32_t sl : {2u, 4u})
+        if (nc % sl == 0 && nc / sl >= 2) l.push_back(sl);
+    return l;
+  };
   std::vector<uint32_t> groupList;
   if (c->streamGroups) {
     groupList.push_back(c->streamGroups);
@@ -371,8 +377,8 @@ int buildStream(tgs_ctx *c) {
   bool have = false;
   uint32_t groups = 0, slots = 1, lds = 0;
   std::string why;
-  for (uint32_t sl : slotList)
-    for (uint32_t g : groupList) {
+  for (uint32_t g : groupList)
+    for (uint32_t sl : slotsFor(g)) {
       StreamImage img = buildStreamImage(c->design, c->stim, nc, g, sl, forced != nullptr);
       const uint32_t vpBytes = up16(std::max<uint32_t>(img.maxVp, 16));
       const uint32_t l = streamLdsBytes(nc, sl, vpBytes, img.maxSteps);
