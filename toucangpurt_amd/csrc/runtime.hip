@@ -1324,6 +1324,7 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
   const size_t oCtl = (oRv + size_t(ni) * nRead + 7) & ~size_t(7);
   const size_t bytes = oCtl + 4ull * kCtlBack * ni;
   if (bytes > c->boxBytes) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // no copy of a failed earlier step may still use the old box
     if (c->boxHost) HIPCHK(c, hipHostFree(c->boxHost));
     c->boxHost = nullptr;
     c->boxBytes = 0;
@@ -1357,9 +1358,11 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
     if (cycles) {
       std::vector<uint32_t> all(ni, cycles);
       if ((rc = tgs_run_batch(c, all.data(), ni, &any))) return rc;
-    } else {
-      for (uint32_t i = 0; i < ni && !any; ++i)
-        if ((rc = tgs_get_stop(c, i, &any))) return rc;
+    } else {  // every slot's sticky stop flag in one copy
+      std::vector<uint64_t> st(c->bs.instPad);
+      HIPCHK(c, hipMemcpyAsync(st.data(), c->bs.stop.p, 8 * st.size(), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (auto v : st) any |= v != 0;
     }
   } else {
     // the last launch is not waited for: its timing, the stop flags and its
