@@ -109,3 +109,27 @@ def _u32p():
 def _u8p():
     import ctypes
     return ctypes.POINTER(ctypes.c_uint8)
+
+
+@pytest.mark.parametrize("stops", [2, 0])
+@pytest.mark.parametrize("kernel,n", [(KERNEL_STREAM, 1), (KERNEL_BATCH, 5)])
+def test_step_reports_stop(gpu_available, kernel, n, stops):
+    """The stop flag a step returns is the oracle's: set by a design whose stop
+    op fires, never set by one without stop ops."""
+    d = gen.generate(None, n_luts=2000, depth=6, n_regs=100, n_inputs=20, n_vecops=50, n_prints=2, n_stops=stops,
+                     stop_rarity=3, part_luts=400, seed=13)
+    print(f"case step stop kernel={kernel} n={n} stops={stops}", flush=True)
+    sim = gpu_sim(d.design, d.symbols, False, None, kernel=kernel)
+    sim.set_instances(n)
+    sim.set_stimulus(d.inputs, 3)
+    orcs = [OracleRun(d.design, d.symbols, False, d.inputs, seed=3, instance=k) for k in range(n)]
+    seen = False
+    for cycles in (0, 1, 2, 4, 8):
+        _, stopped = sim.step([], np.zeros((n, 0), np.uint8), cycles, [])
+        for orc in orcs:
+            orc.run(cycles)
+        want = any(bool(orc.st.stopped) for orc in orcs)
+        assert stopped == want, f"after {cycles} more cycles: step says {stopped}, oracle {want}"
+        seen |= want
+    assert seen == (stops > 0)
+    sim.cleanup()
