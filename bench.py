@@ -75,6 +75,8 @@ def parse():
     ap.add_argument("--min-cycles", type=int, default=1_000)
     ap.add_argument("--max-cycles", type=int, default=50_000)
     ap.add_argument("--tune-cycles", type=int, default=100)
+    ap.add_argument("--part-luts", type=int, default=0,
+                    help="batched leg: partition size in LUT4s (0: the autotuner picks)")
     ap.add_argument("--slice", type=int, default=5_000, help="cycles per instance per batched launch")
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
@@ -89,6 +91,19 @@ def cpu_threads():
     (OMP_NUM_THREADS where the host sets it), else the CPU count."""
     env = os.environ.get("OMP_NUM_THREADS")
     return max(1, int(env)) if env and env.isdigit() else max(1, os.cpu_count() or 1)
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), for the CPU baseline."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
 
 
 def oracle_rate(st, seconds, threads=1):
@@ -120,6 +135,10 @@ def plain_rate(d, cycles, seed):
             "kernel_ms": prof["kernel_ms"], "grid": prof["grid"], "block": prof["block"]}
 
 
+# How the generator draws LUT fan-in (stated with every workload).
+FANIN = ("each LUT4 input drawn uniformly from the earlier levels of the LUT's own 256-LUT4 cluster, "
+         "level 0 being the cluster's 32 register/input (region 0) or exchange (region 1) sources")
+
 KERNEL_NAMES = {0: "plain", 1: "wave64", 3: "stream"}
 # device function names of the kernels, as rocprofv3 reports them (profile file names)
 KERNEL_SYMBOLS = {0: "plainCycles", 1: "w64Cycles", 3: "streamCycles"}
@@ -142,7 +161,7 @@ def measured_traffic(kernel_name):
 
 def run_single(args):
     import numpy as np
-    from toucangpurt_amd import KERNEL_STREAM, KERNEL_WAVE64, POOL_REG, gen, rt
+    from toucangpurt_amd import KERNEL_STREAM, KERNEL_WAVE64, POOL_EXCHANGE, POOL_MEM, POOL_REG, gen, rt
     d = gen.generate("lut1m", seed=args.seed)
     log(f"single: design generated {d.stats['n_parts']} partitions")
     # pick the faster of the persistent kernels on a short run
@@ -187,7 +206,7 @@ def run_single(args):
     out = {
         "value": cycles / dt, "dt": dt, "cycles": cycles,
         "config": {"workload": "lut1m: 1,000,000 LUT4 + 20,000 word ops, depth 64, 50,000 regs, "
-                               "16 SRAM 4096x64b, 128 inputs, 100,000 cycles",
+                               "16 SRAM 4096x64b, 128 inputs, 100,000 cycles; " + FANIN,
                    "cycles": cycles, "cycles_per_step": args.chunk, "kernel": KERNEL_NAMES[kernel],
                    "grid": prof["grid"], "block": prof["block"], "lds_bytes": prof["lds_bytes"],
                    "program_source": SOURCES[prof["resident"]]},
@@ -210,10 +229,11 @@ def run_single(args):
         n1, dt1 = oracle_rate(st, args.cpu_seconds / 2, 1)
         nt = cpu_threads()
         n2, dt2 = oracle_rate(st, args.cpu_seconds, nt)
-        out["cpu"] = {"value": n2 / dt2, "unit": "simulated cycles/s", "cores": nt, "kind": "port",
+        out["cpu"] = {"value": n2 / dt2, "unit": "simulated cycles/s", "cores": nt, "kind": "port", "cpu": cpu_model(),
                       "sample": f"{n2} cycles of the 1M-LUT4 design with the same stimulus (after {n1 + 4} "
                                 f"others), C oracle, partitions of each region over {nt} threads"}
         out["cpu_1thread"] = {"value": n1 / dt1, "unit": "simulated cycles/s", "cores": 1, "kind": "port",
+                              "cpu": cpu_model(),
                               "sample": f"{n1} cycles of the same run, C oracle, 1 thread"}
         total = n1 + n2 + 4
         chk = rt.Simulator(0)
@@ -221,7 +241,8 @@ def run_single(args):
         chk.set_kernel(kernel)
         chk.set_stimulus(d.inputs, args.seed)
         chk.eval_free_running(total)
-        out["parity"] = bool(np.array_equal(chk.read_pool(POOL_REG), st.pool(0)))
+        out["parity"] = all(bool(np.array_equal(chk.read_pool(pool), st.pool(k)))
+                            for k, pool in enumerate((POOL_REG, POOL_MEM, POOL_EXCHANGE)))
         out["parity_cycles"] = total
         chk.cleanup()
         log(f"single: cpu {out['cpu']['value']:.1f} cycles/s, parity {out['parity']}")
@@ -230,7 +251,7 @@ def run_single(args):
 
 def run_batched(args):
     import numpy as np
-    from toucangpurt_amd import KERNEL_BATCH, POOL_REG, gen, rt
+    from toucangpurt_amd import KERNEL_BATCH, POOL_EXCHANGE, POOL_MEM, POOL_REG, gen, rt
     from toucangpurt_amd.rt import SimError
     n = args.instances
     rng = np.random.default_rng(args.seed)
@@ -240,7 +261,7 @@ def run_batched(args):
     best = None
     tuning = []
     designs = {}
-    for part in (256, 512, 1024, 2048, 4096):
+    for part in ((args.part_luts,) if args.part_luts else (256, 512, 1024, 2048, 4096)):
         d = gen.generate("lut200k", seed=args.seed, part_luts=part)
         designs[part] = d
         for block in (128,):
@@ -295,7 +316,7 @@ def run_batched(args):
            "kernel_ms": prof["kernel_ms"], "part_luts": part, "block": block, "grid": prof["grid"],
            "lds_bytes": prof["lds_bytes"], "tuning": tuning,
            "workload": "lut200k: 200,000 LUT4, depth 48, 8 SRAM 1024x32b, 1,024 instances, "
-                       f"cycles uniform in [{args.min_cycles}, {args.max_cycles}]",
+                       f"cycles uniform in [{args.min_cycles}, {args.max_cycles}]; " + FANIN,
            "bytes_per_cycle_shared": stats["b_prog"], "bytes_per_cycle_per_instance": stats["b_state"]}
     # roofline: program bytes once per launch-cycle, state bytes per live instance-cycle
     algo = stats["b_prog"] * launch_cycles + stats["b_state"] * total
@@ -304,7 +325,7 @@ def run_batched(args):
                        "frac": achieved / HBM_PEAK_GBS}
     # spot parity: the instance with the fewest cycles against a fresh single-instance run
     k = int(np.argmin(cycles))
-    got = sim.read_pool(POOL_REG, k)
+    got = [sim.read_pool(pool, k) for pool in (POOL_REG, POOL_MEM, POOL_EXCHANGE)]
     sim.cleanup()
     if not args.no_plain:
         out["plain"] = plain_rate(d, args.plain_cycles, args.seed)
@@ -320,10 +341,11 @@ def run_batched(args):
         st.run(int(cycles[k]), threads=nt)
         cdt = time.perf_counter() - t0
         out["cpu"] = {"value": int(cycles[k]) / cdt, "unit": "stimulus-cycles/s", "cores": nt, "kind": "port",
+                      "cpu": cpu_model(),
                       "sample": f"instance {k}: {int(cycles[k])} cycles of the 200K-LUT4 design, C oracle, "
                                 f"partitions of each region over {nt} threads"}
         out["parity_instance"] = k
-        out["parity"] = bool(np.array_equal(got, st.pool(0)))
+        out["parity"] = all(bool(np.array_equal(g, st.pool(i))) for i, g in enumerate(got))
         log(f"batched: cpu {out['cpu']['value']:.1f} cycles/s, parity {out['parity']}")
     return out
 

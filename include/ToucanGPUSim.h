@@ -5,6 +5,13 @@
 // is the MI355X runtime behind include/toucan_rt.h; the object owns one
 // runtime context instead of the reference's process globals, so several
 // simulators may live in one process.
+//
+// Errors never end the process (the reference asserts on unknown names and
+// exits on CUDA errors, include/SimEval.h:11-19, src/ToucanSim.cu:271-289):
+// a failing call prints the reason on stderr, returns false / 0 /
+// UINT64_MAX, and leaves the status in lastStatus() (a TGS_E_* code of
+// include/toucan_rt.h, 0 after a successful call) and the text in
+// lastError().
 #pragma once
 
 #include <cstddef>
@@ -16,6 +23,7 @@
 #include <utility>
 #include <vector>
 
+#include "GPUConsts.h"
 #include "SimConfigs.h"
 #include "ToucanGPUGenDataTypes.h"
 
@@ -28,8 +36,20 @@ class ToucanSimulator {
   SimDesignInfo design;
   SimDebugInfo symbols;
 
+  // With ENABLE_SIM_PROFILE defined where this header is included (as in
+  // the reference's SimConfigs.h), init also turns on the per-partition
+  // profile of cycles PROFILE_START_CYCLE .. + PROFILE_COLLECT_CYCLE of each
+  // free-running launch (src/SimEval.cu:894-983) that printProfile reports.
   int init(const int gpu_id, const std::string designBinFilename, const std::string symbolBinFilename,
-           bool shouldRandomize, bool enablePrint);
+           bool shouldRandomize, bool enablePrint) {
+#ifdef ENABLE_SIM_PROFILE
+    return initWithProfile(gpu_id, designBinFilename, symbolBinFilename, shouldRandomize, enablePrint, true);
+#else
+    return initWithProfile(gpu_id, designBinFilename, symbolBinFilename, shouldRandomize, enablePrint, false);
+#endif
+  }
+  int initWithProfile(const int gpu_id, const std::string &designBinFilename, const std::string &symbolBinFilename,
+                      bool shouldRandomize, bool enablePrint, bool profileParts);
   void enableVCDDump(const std::string vcdFilename);
   bool eval();
   bool eval_free_running(uint32_t max_cycles);
@@ -37,6 +57,9 @@ class ToucanSimulator {
   uint64_t readRegister(const std::string &signalName);
   uint64_t readSignal(const std::string &signalName);
   void cleanup();
+  // src/ToucanSim.cu:306-325: one line per partition (useful share, useful
+  // and total ticks) when the per-partition profile was collected, after a
+  // line with the launch shape and the measured kernel time.
   void printProfile();
 
   // Extension: one testbench step in one runtime call (tgs_step): set the
@@ -51,6 +74,7 @@ class ToucanSimulator {
   int setKernel(int kernel);
   int setStimulus(const std::vector<uint32_t> &inputIds, uint64_t seed);
   std::string lastError() const;
+  int lastStatus() const { return status; }
   tgs_ctx *context() const { return ctx; }
 
   ~ToucanSimulator() { cleanup(); }
@@ -59,6 +83,11 @@ class ToucanSimulator {
 
  private:
   tgs_ctx *ctx = nullptr;
+  int status = 0;
+  bool loadedOk = false;  // the last init succeeded
+  std::string errText;
+  bool fail(int code, const std::string &what);
+  bool ready(const char *call);
   bool dumpVCD = false;
   std::ofstream *vcdOfs = nullptr;
   std::vector<std::string> vcdNames;

@@ -67,8 +67,9 @@ typedef struct tgs_profile {
   uint32_t kernel;        /* kernel family of the last launch */
   uint32_t resident;      /* program source: 0 global, 1 staged in LDS per visit, 2 LDS-resident, 3 streamed */
   uint64_t prog_bytes;    /* device program image bytes */
-  uint64_t barrier_ticks; /* wave64: summed ticks spent waiting at grid barriers (rank 0 block) */
-  uint64_t total_ticks;   /* wave64: summed ticks of the cycle loop (rank 0 block) */
+  uint64_t barrier_ticks; /* wave64 / stream: real-time ticks spent waiting at grid barriers, summed over
+                             every workgroup of every launch */
+  uint64_t total_ticks;   /* wave64 / stream: real-time ticks of the cycle loop, summed the same way */
 } tgs_profile;
 
 /* ---- program inspection (host only, no device needed) ---- */
@@ -212,15 +213,40 @@ int tgs_get_cycles(tgs_ctx *ctx, uint32_t instance, uint64_t *cycles);
 int tgs_get_stop_cycle(tgs_ctx *ctx, uint32_t instance, uint64_t *cycle);
 
 /* Print log of fired print ops: (cycle, message id) pairs in firing order of
- * cycles. Returns the number of entries available via *count. The reference
- * printf()s from the device (src/SimEval.cu:696-704). */
+ * cycles, every event since the instance was loaded, reset or restored or
+ * since the last tgs_clear_print_log. *count gets the number of entries
+ * held; up to `cap` are copied. The reference printf()s from the device
+ * (src/SimEval.cu:696-704). The device log holds 65,536 events per launch;
+ * the runtime moves it to host memory after every launch that may have
+ * printed, so a long run loses nothing unless one launch fires more. */
 int tgs_get_print_log(tgs_ctx *ctx, uint32_t instance, uint64_t *cycles, uint32_t *msgs, uint64_t cap,
                       uint64_t *count);
+/* Drop the instance's held print events (the caller has consumed them).
+ * *dropped (may be null) gets the number of events lost since the last
+ * clear because a launch fired more than the device log holds. */
+int tgs_clear_print_log(tgs_ctx *ctx, uint32_t instance, uint64_t *dropped);
 
-/* copy_profile_data (src/SimEval.cu:1100-1104) and printProfile
- * (src/ToucanSim.cu:306-325): launch shape and measured kernel time. */
+/* printProfile (src/ToucanSim.cu:306-325): launch shape and measured kernel
+ * time; barrier_ticks / total_ticks cover every workgroup of every launch. */
 int tgs_get_profile(tgs_ctx *ctx, tgs_profile *out);
 int tgs_reset_profile(tgs_ctx *ctx);
+
+/* Per-partition profile (ENABLE_SIM_PROFILE of src/SimEval.cu:894-983).
+ * When enabled, the single-instance kernels time launch-local cycles
+ * start .. start + count - 1 of every launch with the shader clock
+ * (s_memtime, the counterpart of clock64): for each partition, the ticks
+ * from the start of its region on its workgroup (after the grid barrier or
+ * the launch start) to the end of its evaluation ("useful"), and to the end
+ * of the region's grid barrier ("total"). The batched kernel is not timed.
+ * The reference's defaults are PROFILE_START_CYCLE = PROFILE_COLLECT_CYCLE
+ * = 100 (include/SimConfigs.h:3-6). Disabling it costs the kernels nothing. */
+int tgs_set_part_profile(tgs_ctx *ctx, int enable, uint32_t start_cycle, uint32_t collect_cycles);
+/* copy_profile_data (src/SimEval.cu:1100-1104): per-partition averages over
+ * the cycles collected by the last launch that reached the window (the
+ * reference's running means restart with every launch); `count` entries
+ * (partition order of the design). *collected (may be null) gets the number
+ * of cycles averaged, 0 when no launch has reached the window. */
+int tgs_get_part_profile(tgs_ctx *ctx, int64_t *useful, int64_t *total, size_t count, uint32_t *collected);
 
 /* Program statistics of the loaded design. */
 int tgs_get_program_stats(tgs_ctx *ctx, tgs_program_stats *out);

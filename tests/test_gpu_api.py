@@ -16,7 +16,7 @@ pytestmark = [pytest.mark.gpu, pytest.mark.subprocess_gpu]
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-@pytest.mark.parametrize("kernel", [0, 1, 2], ids=["plain", "wave64", "batch"])
+@pytest.mark.parametrize("kernel", [0, 1, 2, 3], ids=["plain", "wave64", "batch", "stream"])
 def test_cpp_simulator_class(gpu_available, tmp_path, kernel):
     d = gen.generate(None, n_luts=3000, depth=6, n_regs=120, n_inputs=16, n_vecops=120, n_srams=2, sram_depth=256,
                      sram_width=32, part_luts=600, seed=31)

@@ -159,8 +159,8 @@ RT_SYMBOLS = [
     "tgs_read_pool", "tgs_write_pool", "tgs_pool_size", "tgs_reset_instance", "tgs_snapshot_begin",
     "tgs_snapshot_wait", "tgs_set_probes", "tgs_read_probes", "tgs_checkpoint_size", "tgs_checkpoint_save", "tgs_checkpoint_restore",
     "tgs_get_stop", "tgs_get_cycles",
-    "tgs_get_stop_cycle", "tgs_get_print_log", "tgs_get_profile", "tgs_reset_profile",
-    "tgs_get_program_stats",
+    "tgs_get_stop_cycle", "tgs_get_print_log", "tgs_clear_print_log", "tgs_get_profile", "tgs_reset_profile",
+    "tgs_set_part_profile", "tgs_get_part_profile", "tgs_get_program_stats",
 ]
 
 
@@ -211,6 +211,10 @@ def rt_lib():
         lib.tgs_get_cycles.argtypes = [vp, ctypes.c_uint32, _u64p]
         lib.tgs_get_stop_cycle.argtypes = [vp, ctypes.c_uint32, _u64p]
         lib.tgs_get_print_log.argtypes = [vp, ctypes.c_uint32, _u64p, _u32p, ctypes.c_uint64, _u64p]
+        lib.tgs_clear_print_log.argtypes = [vp, ctypes.c_uint32, _u64p]
+        lib.tgs_set_part_profile.argtypes = [vp, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32]
+        lib.tgs_get_part_profile.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64),
+                                             ctypes.c_size_t, _u32p]
         lib.tgs_get_profile.argtypes = [vp, ctypes.POINTER(Profile)]
         lib.tgs_reset_profile.argtypes = [vp]
         lib.tgs_get_program_stats.argtypes = [vp, ctypes.POINTER(ProgramStats)]

@@ -10,13 +10,15 @@ namespace tgs {
 // Control block of one instance (device memory, u32 words).
 enum Ctl : uint32_t {
   C_STOP = 0,        // sticky stop flag (SimEval.cu:194 shouldStop)
-  C_BARRIER = 1,     // grid-barrier arrival counter, zeroed before each launch
-  C_LOGCOUNT = 2,    // print-log entries appended (may exceed capacity)
+  C_LOGCOUNT = 2,    // u64: print-log entries appended since the host last drained the log
   C_STOPCYC = 4,     // u64: first cycle a stop op fired (UINT64_MAX if none)
-  C_BARTICKS = 6,    // u64: rank-0 ticks spent in grid barriers (last launch)
-  C_TOTTICKS = 8,    // u64: rank-0 ticks of the cycle loop (last launch)
+  C_BARTICKS = 6,    // u64: real-time ticks spent in grid barriers, summed over workgroups (last launch)
+  C_TOTTICKS = 8,    // u64: real-time ticks of the cycle loop, summed over workgroups (last launch)
+  C_BARRIER = 10,    // grid-barrier arrival counter
   C_WORDS = 16
 };
+// Words the host zeroes before every launch of a persistent kernel.
+constexpr uint32_t kCtlLaunchFirst = C_BARTICKS, kCtlLaunchWords = C_BARRIER + 1 - C_BARTICKS;
 
 constexpr uint32_t kMaxDraws = 64;  // up to 4096 stimulus inputs
 
@@ -40,7 +42,30 @@ struct CommonArgs {
   // slots whose end-of-visit values go to probeVal (CSR order)
   const uint32_t *probeOff, *probeSlot;
   uint8_t *probeVal;
+  // per-partition profile (null when off): launch-local cycles
+  // [profStart, profStart + profCount) add, per partition, the shader-clock
+  // ticks from its region's start on its workgroup to its end (profUseful)
+  // and, per workgroup and region, the ticks to the end of the region's grid
+  // barrier (profWg[2 * workgroup + region]; profOwner names the workgroup)
+  uint64_t *profUseful, *profWg;
+  uint32_t *profOwner;  // per partition: the workgroup that ran it
+  uint32_t profStart, profCount;
 };
+
+// Whether launch-local cycle `cyc` is inside the profile window.
+__device__ __forceinline__ bool profCycle(const CommonArgs &a, uint32_t cyc) {
+  return a.profUseful != nullptr && cyc >= a.profStart && cyc - a.profStart < a.profCount;
+}
+
+// Shader clock (the counterpart of clock64).
+__device__ __forceinline__ uint64_t shaderClock() { return __builtin_amdgcn_s_memtime(); }
+
+// End of workgroup `wg`'s barrier-measured loop: add its barrier-wait and
+// loop ticks to the launch totals.
+__device__ __forceinline__ void addBarrierTicks(const CommonArgs &a, uint64_t barTicks, uint64_t loopTicks) {
+  atomicAdd(reinterpret_cast<unsigned long long *>(&a.ctl[C_BARTICKS]), (unsigned long long)barTicks);
+  atomicAdd(reinterpret_cast<unsigned long long *>(&a.ctl[C_TOTTICKS]), (unsigned long long)loopTicks);
+}
 
 // xorshift64, Marsaglia (2003), shifts (13, 7, 17).
 __device__ __forceinline__ uint64_t xorshift64(uint64_t x) {
@@ -65,7 +90,7 @@ __device__ __forceinline__ uint32_t stimBit(const uint64_t *draws, uint32_t inpu
 
 // Print op (SimEval.cu:696-704): append (cycle, message) to the instance log.
 __device__ __forceinline__ void logPrint(const CommonArgs &a, uint64_t cycle, uint32_t msg) {
-  uint32_t k = atomicAdd(&a.ctl[C_LOGCOUNT], 1u);
+  const uint64_t k = atomicAdd(reinterpret_cast<unsigned long long *>(&a.ctl[C_LOGCOUNT]), 1ull);
   if (k < a.logCap) {
     a.logCycle[k] = cycle;
     a.logMsg[k] = msg;

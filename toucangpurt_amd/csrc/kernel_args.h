@@ -57,7 +57,7 @@ struct BatchArgs {
   uint32_t nIn;
   uint64_t *stop;                 // per slot
   uint64_t *stopCycle;            // per slot
-  uint32_t *logCount;
+  uint64_t *logCount;             // entries appended since the host last drained the log
   uint64_t *logCycle;
   uint64_t *logMsg;               // msg | instance << 32
   const uint32_t *instOf;         // slot -> instance

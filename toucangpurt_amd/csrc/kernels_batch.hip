@@ -307,7 +307,7 @@ __device__ __forceinline__ void waveItem(const BatchArgs &a, uint32_t part, cons
   if (liveLane && a.printEnable)
     for (uint32_t k = 0; k < npr; ++k)
       if (opnd(cx, L[k] & 0xFFFFu) != 0) {
-        uint32_t q = atomicAdd(a.logCount, 1u);
+        const uint64_t q = atomicAdd(reinterpret_cast<unsigned long long *>(a.logCount), 1ull);
         if (q < a.logCap) {
           a.logCycle[q] = cycle;
           a.logMsg[q] = uint64_t(L[k] >> 16) | (uint64_t(a.instOf[inst]) << 32);

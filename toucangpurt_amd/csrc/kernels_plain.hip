@@ -204,16 +204,36 @@ __global__ __launch_bounds__(1024) void plainCycles(PlainArgs a) {
   __shared__ uint32_t queue;
   cg::grid_group grid = cg::this_grid();
   uint64_t xs = *a.c.xs;
+  const bool lead = threadIdx.x == 0;
   for (uint32_t c = 0; c < a.c.nCycles; ++c) {
     const uint64_t cycle = a.c.cycleBase + c;
+    const bool pc = profCycle(a.c, c) && lead;  // ENABLE_SIM_PROFILE (SimEval.cu:894-983)
     if (threadIdx.x == 0) stimulusDraws(xs, a.c.nIn, draws);
     __syncthreads();
     if (blockIdx.x == 0)
       for (uint32_t i = threadIdx.x; i < a.c.nIn; i += blockDim.x) a.c.reg[a.c.inIds[i]] = uint8_t(stimBit(draws, i));
-    for (uint32_t p = blockIdx.x; p < a.n0; p += gridDim.x) plainPartition(a, p, plainVp, draws, cycle, &queue);
+    uint64_t t0 = pc ? shaderClock() : 0;
+    for (uint32_t p = blockIdx.x; p < a.n0; p += gridDim.x) {
+      plainPartition(a, p, plainVp, draws, cycle, &queue);
+      if (pc) {
+        a.c.profUseful[p] += shaderClock() - t0;
+        a.c.profOwner[p] = blockIdx.x;
+      }
+    }
     grid.sync();
-    for (uint32_t p = a.n0 + blockIdx.x; p < a.nTotal; p += gridDim.x) plainPartition(a, p, plainVp, draws, cycle, &queue);
+    if (pc) {
+      a.c.profWg[2 * blockIdx.x] += shaderClock() - t0;
+      t0 = shaderClock();
+    }
+    for (uint32_t p = a.n0 + blockIdx.x; p < a.nTotal; p += gridDim.x) {
+      plainPartition(a, p, plainVp, draws, cycle, &queue);
+      if (pc) {
+        a.c.profUseful[p] += shaderClock() - t0;
+        a.c.profOwner[p] = blockIdx.x;
+      }
+    }
     grid.sync();
+    if (pc) a.c.profWg[2 * blockIdx.x + 1] += shaderClock() - t0;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.c.xs = xs;
 }

@@ -407,8 +407,12 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
   const uint32_t slotWaves = NC / a.slots;
   uint64_t xs = *c.xs;
   uint64_t barTicks = 0;
-  const bool prof = g == 0;
   const uint64_t tStart = __builtin_amdgcn_s_memrealtime();
+  // per-partition profile (SimEval.cu:894-983): region start on this
+  // workgroup, the region's index inside the cycle
+  uint64_t regionT0 = 0;
+  uint32_t region = 0;
+  bool pc = false;
   uint32_t bar = a.barrierBase;
   __syncthreads();  // start barrier: steps 0 and 1 are in the ring
   const uint32_t *mySlot = ring + wave * kStreamRows * 64 + lane;
@@ -434,6 +438,9 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
       __syncthreads();
       if (g == 0)
         for (uint32_t k = tid; k < c.nIn; k += nTc) c.reg[c.inIds[k]] = uint8_t(stimBit(draws, k));
+      pc = profCycle(c, cyc);
+      region = 0;
+      if (pc) regionT0 = shaderClock();
     }
     readStep(nxt, j + 1);
     const uint32_t type = cur.ey & 15u, kind = (cur.ey >> 4) & 7u;
@@ -454,8 +461,18 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     } else if (type == S_PEND) {
       partEnd(cur, c, W, vp, c.cycleBase + cyc, ts, nTs);
       __syncthreads();
+      if (pc && ts == 0 && kind != kKindNone) {
+        const uint32_t part = __builtin_amdgcn_readlane(cur.r0, S_PART_ID);
+        c.profUseful[part] += shaderClock() - regionT0;
+        c.profOwner[part] = g;
+      }
     } else {
-      barTicks += gridBarrier(&c.ctl[C_BARRIER], (++bar) * gridDim.x, prof);
+      barTicks += gridBarrier(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+      if (pc) {
+        if (tid == 0) c.profWg[2 * g + region] += shaderClock() - regionT0;
+        regionT0 = shaderClock();
+        region ^= 1u;
+      }
     }
     cur = nxt;
     ++j;
@@ -464,10 +481,9 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
       ++cyc;
     }
   }
-  if (g == 0 && tid == 0) {
-    *c.xs = xs;
-    *reinterpret_cast<uint64_t *>(&c.ctl[C_BARTICKS]) = barTicks;
-    *reinterpret_cast<uint64_t *>(&c.ctl[C_TOTTICKS]) = __builtin_amdgcn_s_memrealtime() - tStart;
+  if (tid == 0) {
+    if (g == 0) *c.xs = xs;
+    addBarrierTicks(c, barTicks, __builtin_amdgcn_s_memrealtime() - tStart);
   }
 }
 

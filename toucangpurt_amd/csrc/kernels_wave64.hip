@@ -263,24 +263,42 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
   __syncthreads();
   uint64_t xs = *a.c.xs;
   uint64_t barTicks = 0;
-  const bool prof = g == 0;
   const uint64_t tStart = __builtin_amdgcn_s_memrealtime();
   uint32_t bar = a.barrierBase;
+  // per-partition profile (SimEval.cu:894-983): ticks from the region's
+  // start on this workgroup to the end of each partition, and to the end of
+  // the region's grid barrier
+  auto partDone = [&](uint32_t part, uint64_t t0, bool on) {
+    if (on && tid == 0) {
+      a.c.profUseful[part] += shaderClock() - t0;
+      a.c.profOwner[part] = g;
+    }
+  };
   for (uint32_t c = 0; c < a.c.nCycles; ++c) {
     const uint64_t cycle = a.c.cycleBase + c;
+    const bool pc = profCycle(a.c, c);
     if (tid == 0) stimulusDraws(xs, a.c.nIn, draws);
     __syncthreads();
     if (g == 0)
       for (uint32_t k = tid; k < a.c.nIn; k += nt) a.c.reg[a.c.inIds[k]] = uint8_t(stimBit(draws, k));
-    for (uint32_t s = s0; s < s1; ++s) w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
-    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, prof);
-    for (uint32_t s = s1; s < s2; ++s) w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
-    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, prof);
+    uint64_t t0 = pc ? shaderClock() : 0;
+    for (uint32_t s = s0; s < s1; ++s) {
+      w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
+      partDone(a.schedPart[s], t0, pc);
+    }
+    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+    if (pc && tid == 0) a.c.profWg[2 * g] += shaderClock() - t0;
+    t0 = pc ? shaderClock() : 0;
+    for (uint32_t s = s1; s < s2; ++s) {
+      w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
+      partDone(a.schedPart[s], t0, pc);
+    }
+    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+    if (pc && tid == 0) a.c.profWg[2 * g + 1] += shaderClock() - t0;
   }
-  if (g == 0 && tid == 0) {
-    *a.c.xs = xs;
-    *reinterpret_cast<uint64_t *>(&a.c.ctl[C_BARTICKS]) = barTicks;
-    *reinterpret_cast<uint64_t *>(&a.c.ctl[C_TOTTICKS]) = __builtin_amdgcn_s_memrealtime() - tStart;
+  if (tid == 0) {
+    if (g == 0) *a.c.xs = xs;
+    addBarrierTicks(a.c, barTicks, __builtin_amdgcn_s_memrealtime() - tStart);
   }
 }
 

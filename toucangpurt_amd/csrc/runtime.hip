@@ -29,7 +29,12 @@ using toucanGPUSim::SimDesignInfo;
 
 namespace {
 
+// Device print-log entries per instance (single-instance layouts) and the
+// floor of the shared log of the batched layout. Launches are cut short so
+// that one launch can never fire more print events than the log holds; the
+// host drains the log into HostLog after every launch that may have printed.
 constexpr uint32_t kLogCap = 1u << 16;
+constexpr uint64_t kBatchLogMax = 1ull << 24;
 
 // Device buffer owned by a context.
 struct DBuf {
@@ -90,10 +95,23 @@ struct Instance {
 // k * instPad + s. Slots are instances reordered by decreasing cycle count.
 struct BatchState {
   uint32_t n = 0, instPad = 0;
+  uint64_t logCap = 0;  // entries of the shared print log
   DBuf reg, mem, exg, xs, draws, stop, stopCycle, logCount, logCycle, logMsg, barrier, probe;
   DBuf runCycles, cycleBase, groupsActive, instOfD, perm, scratch;
   std::vector<uint32_t> slotOf, instOf;  // instance -> slot, slot -> instance
   std::vector<uint64_t> cycles;          // per instance
+};
+
+// Print events of one instance moved off the device (tgs_get_print_log).
+struct HostLog {
+  std::vector<uint64_t> cyc;
+  std::vector<uint32_t> msg;
+  uint64_t dropped = 0;  // events a launch fired past the device log's capacity
+  void clear() {
+    cyc.clear();
+    msg.clear();
+    dropped = 0;
+  }
 };
 
 // Host copy of one instance's architectural state (layout conversions).
@@ -163,6 +181,15 @@ struct tgs_ctx {
   uint32_t streamNC = 0, streamNCUsed = 4, streamSlots = 1, streamGroups = 0, streamGrid = 0, streamLds = 0;
 
   tgs_profile prof{};
+  // per-partition profile (tgs_set_part_profile) of instance 0
+  bool partProf = false;
+  uint32_t profStart = 100, profCount = 100, profCollected = 0;
+  DBuf profUseful, profWg, profOwner;
+  // print events moved off the device, per instance; logDirty: a launch may
+  // have printed since the last drain
+  std::vector<HostLog> logs;
+  bool logDirty = false;
+  bool stepFailed = false;  // the last tgs_step returned an error after queueing work
   SnapRing snap;
   bool timingPending = false;  // the last launch's events are not accounted yet
   // testbench mailbox (tgs_step): pinned host staging and its device twin
@@ -451,7 +478,76 @@ CommonArgs commonArgs(tgs_ctx *c, Instance &in, uint32_t cycles) {
     a.probeSlot = c->prSlot.as<uint32_t>();
     a.probeVal = in.probe.as<uint8_t>();
   }
+  if (c->partProf && !c->inst.empty() && &in == c->inst[0].get()) {
+    a.profUseful = c->profUseful.as<uint64_t>();
+    a.profWg = c->profWg.as<uint64_t>();
+    a.profOwner = c->profOwner.as<uint32_t>();
+    a.profStart = c->profStart;
+    a.profCount = c->profCount;
+  }
   return a;
+}
+
+// Largest launch (in cycles) whose print events always fit the device log:
+// every print op fires at most once per cycle and instance.
+uint32_t printCycleCap(const tgs_ctx *c, uint64_t logCap, uint64_t instances) {
+  if (!c->printEnable || c->stats.n_print == 0) return UINT32_MAX;
+  return uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(UINT32_MAX, logCap / (c->stats.n_print * instances))));
+}
+
+// Move the device print logs to the instances' host logs and empty them.
+int drainLogs(tgs_ctx *c) {
+  if (!c->logDirty) return TGS_OK;
+  c->logDirty = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const uint32_t n = c->batched ? c->bs.n : uint32_t(c->inst.size());
+  if (c->logs.size() < n) c->logs.resize(n);
+  const uint64_t zero = 0;
+  if (!c->batched) {
+    for (uint32_t i = 0; i < c->inst.size(); ++i) {
+      Instance &in = *c->inst[i];
+      uint64_t count = 0;
+      HIPCHK(c, hipMemcpy(&count, in.ctl.as<uint32_t>() + C_LOGCOUNT, 8, hipMemcpyDeviceToHost));
+      if (count == 0) continue;
+      const uint64_t k = std::min<uint64_t>(count, kLogCap);
+      HostLog &L = c->logs[i];
+      const size_t at = L.cyc.size();
+      L.cyc.resize(at + k);
+      L.msg.resize(at + k);
+      HIPCHK(c, hipMemcpy(L.cyc.data() + at, in.logCycle.p, k * 8, hipMemcpyDeviceToHost));
+      HIPCHK(c, hipMemcpy(L.msg.data() + at, in.logMsg.p, k * 4, hipMemcpyDeviceToHost));
+      L.dropped += count - k;
+      HIPCHK(c, hipMemcpy(in.ctl.as<uint32_t>() + C_LOGCOUNT, &zero, 8, hipMemcpyHostToDevice));
+    }
+    return TGS_OK;
+  }
+  auto &b = c->bs;
+  uint64_t count = 0;
+  HIPCHK(c, hipMemcpy(&count, b.logCount.p, 8, hipMemcpyDeviceToHost));
+  if (count == 0) return TGS_OK;
+  const uint64_t k = std::min<uint64_t>(count, b.logCap);
+  std::vector<uint64_t> cy(k), mg(k);
+  HIPCHK(c, hipMemcpy(cy.data(), b.logCycle.p, k * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(mg.data(), b.logMsg.p, k * 8, hipMemcpyDeviceToHost));
+  for (uint64_t e = 0; e < k; ++e) {
+    const uint32_t inst = uint32_t(mg[e] >> 32);
+    if (inst >= n) continue;
+    c->logs[inst].cyc.push_back(cy[e]);
+    c->logs[inst].msg.push_back(uint32_t(mg[e]));
+  }
+  if (count > k) c->logs[0].dropped += count - k;  // instance unknown: reported on instance 0
+  HIPCHK(c, hipMemcpy(b.logCount.p, &zero, 8, hipMemcpyHostToDevice));
+  return TGS_OK;
+}
+
+// Before a launch of instance 0 that reaches the profile window: clear the
+// sums the kernel adds to.
+int profBegin(tgs_ctx *c, Instance &in, uint32_t n) {
+  if (!c->partProf || c->inst.empty() || &in != c->inst[0].get() || n <= c->profStart) return TGS_OK;
+  HIPCHK(c, hipMemsetAsync(c->profUseful.p, 0, c->profUseful.n, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->profWg.p, 0, c->profWg.n, c->stream));
+  c->profCollected = std::min(c->profCount, n - c->profStart);
+  return TGS_OK;
 }
 
 // Largest cycle count per launch that keeps the barrier counter in range.
@@ -472,11 +568,15 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->prof.kernel_ms += ms;
   }
+  const uint32_t printCap = printCycleCap(c, kLogCap, 1);
+  uint32_t *ctlLaunch = in.ctl.as<uint32_t>() + kCtlLaunchFirst;
   while (cycles > 0) {
-    int rc;
+    int rc = drainLogs(c);  // every launch starts with an empty device log
+    if (rc) return rc;
     if (c->kernel == TGS_KERNEL_PLAIN) {
       if ((rc = buildPlain(c))) return rc;
-      uint32_t n = std::min(cycles, maxCyclesPerLaunch(c->plainGrid));
+      uint32_t n = std::min({cycles, maxCyclesPerLaunch(c->plainGrid), printCap});
+      if ((rc = profBegin(c, in, n))) return rc;
       PlainArgs a{};
       a.c = commonArgs(c, in, n);
       a.c.consts = c->pConsts.as<uint8_t>();
@@ -503,7 +603,8 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
         c->kernel = TGS_KERNEL_WAVE64;  // the default family does not fit: the wave64 kernel runs it
         continue;
       }
-      uint32_t n = std::min(cycles, maxCyclesPerLaunch(c->streamGrid));
+      uint32_t n = std::min({cycles, maxCyclesPerLaunch(c->streamGrid), printCap});
+      if ((rc = profBegin(c, in, n))) return rc;
       StreamArgs a{};
       a.c = commonArgs(c, in, n);
       a.c.consts = c->sConsts.as<uint8_t>();
@@ -514,7 +615,7 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
       a.vpBytes = up16(std::max<uint32_t>(c->simg.maxVp, 16));
       a.slots = c->streamSlots;
       a.barrierBase = 0;
-      HIPCHK(c, hipMemsetAsync(in.ctl.as<uint32_t>() + C_BARRIER, 0, 4, c->stream));
+      HIPCHK(c, hipMemsetAsync(ctlLaunch, 0, 4 * kCtlLaunchWords, c->stream));
       HIPCHK(c, hipEventRecord(c->ev0, c->stream));
       HIPCHK(c, launchStream(a, c->streamNCUsed, c->streamGrid, c->streamLds, c->stream));
       HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -528,7 +629,8 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
       c->prof.cycles += n;
     } else {
       if ((rc = buildW64(c))) return rc;
-      uint32_t n = std::min(cycles, maxCyclesPerLaunch(c->sched.groups));
+      uint32_t n = std::min({cycles, maxCyclesPerLaunch(c->sched.groups), printCap});
+      if ((rc = profBegin(c, in, n))) return rc;
       W64Args a{};
       a.c = commonArgs(c, in, n);
       a.c.consts = c->wConsts.as<uint8_t>();
@@ -540,7 +642,7 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
       a.schedLds = c->wSchedLds.as<uint32_t>();
       a.vpBytes = up16(std::max<uint32_t>(c->w64.maxVp, 16));
       a.barrierBase = 0;
-      HIPCHK(c, hipMemsetAsync(in.ctl.as<uint32_t>() + C_BARRIER, 0, 4, c->stream));
+      HIPCHK(c, hipMemsetAsync(ctlLaunch, 0, 4 * kCtlLaunchWords, c->stream));
       HIPCHK(c, hipEventRecord(c->ev0, c->stream));
       HIPCHK(c, launchWave64(a, c->w64Mode, c->sched.groups, c->w64Block, c->w64Lds, c->stream));
       HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -555,6 +657,7 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
     }
     c->prof.launches += 1;
     c->prof.kernel = c->kernel;
+    if (c->printEnable && c->stats.n_print) c->logDirty = true;
     if (!wait && cycles == 0) {
       c->timingPending = true;
       break;
@@ -657,9 +760,10 @@ int batchUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
   HIPCHK(c, b.draws.alloc(8 * P * 2 * kMaxDraws));
   HIPCHK(c, b.stop.alloc(8 * P));
   HIPCHK(c, b.stopCycle.alloc(8 * P));
-  HIPCHK(c, b.logCount.alloc(4));
-  HIPCHK(c, b.logCycle.alloc(kLogCap * 8));
-  HIPCHK(c, b.logMsg.alloc(kLogCap * 8));
+  b.logCap = std::max<uint64_t>(kLogCap, std::min<uint64_t>(kBatchLogMax, c->stats.n_print * P));
+  HIPCHK(c, b.logCount.alloc(8));
+  HIPCHK(c, b.logCycle.alloc(b.logCap * 8));
+  HIPCHK(c, b.logMsg.alloc(b.logCap * 8));
   HIPCHK(c, b.barrier.alloc(4));
   HIPCHK(c, b.runCycles.alloc(4 * P));
   HIPCHK(c, b.cycleBase.alloc(8 * P));
@@ -700,7 +804,7 @@ int batchUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
   HIPCHK(c, hipMemcpy(b.xs.p, xs.data(), 8 * P, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(b.stop.p, st.data(), 8 * P, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(b.stopCycle.p, sc.data(), 8 * P, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemset(b.logCount.p, 0, 4));
+  HIPCHK(c, hipMemset(b.logCount.p, 0, 8));
   HIPCHK(c, hipMemcpy(b.instOfD.p, b.instOf.data(), 4 * P, hipMemcpyHostToDevice));
   return TGS_OK;
 }
@@ -839,9 +943,11 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
       base[s] = b.cycles[b.instOf[s]];
       maxC = std::max(maxC, remaining[s]);
     }
-  const uint64_t perLaunch = std::min<uint64_t>(1u << 20, 0xE0000000ull / (2ull * c->batchGrid + 2));
+  const uint64_t perLaunch = std::min<uint64_t>({1u << 20, 0xE0000000ull / (2ull * c->batchGrid + 2),
+                                                  printCycleCap(c, b.logCap, P)});
   std::vector<uint32_t> run(P), active;
   while (maxC > 0) {
+    if ((rc = drainLogs(c))) return rc;  // every launch starts with an empty device log
     const uint32_t L = uint32_t(std::min<uint64_t>(maxC, perLaunch));
     for (uint32_t s = 0; s < P; ++s) run[s] = uint32_t(std::min<uint64_t>(remaining[s], L));
     active.assign(L, 0);
@@ -881,11 +987,11 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
     a.nIn = c->stim.ids.size();
     a.stop = b.stop.as<uint64_t>();
     a.stopCycle = b.stopCycle.as<uint64_t>();
-    a.logCount = b.logCount.as<uint32_t>();
+    a.logCount = b.logCount.as<uint64_t>();
     a.logCycle = b.logCycle.as<uint64_t>();
     a.logMsg = b.logMsg.as<uint64_t>();
     a.instOf = b.instOfD.as<uint32_t>();
-    a.logCap = kLogCap;
+    a.logCap = uint32_t(b.logCap);
     a.printEnable = c->printEnable;
     a.barrier = b.barrier.as<uint32_t>();
     if (c->nProbe()) {
@@ -902,6 +1008,7 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
     c->prof.kernel_ms += ms;
     c->prof.launches += 1;
     c->prof.kernel = TGS_KERNEL_BATCH;
+    if (c->printEnable && c->stats.n_print) c->logDirty = true;
     c->prof.grid = c->batchGrid;
     c->prof.block = c->batchBlockEff;
     c->prof.lds_bytes = c->batchLds;
@@ -921,7 +1028,9 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
 int switchLayout(tgs_ctx *c, bool batched) {
   if (c->batched == batched) return TGS_OK;
   std::vector<HostInst> h;
-  int rc = downloadAll(c, h);
+  int rc = drainLogs(c);
+  if (rc) return rc;
+  rc = downloadAll(c, h);
   if (rc) return rc;
   c->batched = batched;
   if (batched) c->inst.clear();
@@ -1078,6 +1187,10 @@ int tgs_load(tgs_ctx *c, const uint8_t *design, size_t designSize, const uint8_t
   c->kernel = c->kernel == TGS_KERNEL_BATCH ? TGS_KERNEL_STREAM : c->kernel;
   c->inst.emplace_back(new Instance());
   if ((rc = resetInstance(c, *c->inst[0], 0, c->stimSeed))) return rc;
+  c->logs.assign(1, HostLog());
+  c->logDirty = false;
+  c->partProf = false;
+  c->profCollected = 0;
   c->loaded = true;
   c->prof = tgs_profile{};
   return TGS_OK;
@@ -1158,7 +1271,10 @@ int tgs_set_instances(tgs_ctx *c, uint32_t n) {
   if (n == 0) return c->fail(TGS_E_ARG, "at least one instance");
   std::vector<HostInst> h;
   if ((rc = initialHost(c, n, h))) return rc;
-  return uploadAll(c, h);
+  if ((rc = uploadAll(c, h))) return rc;
+  c->logs.assign(n, HostLog());
+  c->logDirty = false;
+  return TGS_OK;
 }
 
 int tgs_set_stimulus(tgs_ctx *c, const uint32_t *ids, uint32_t n, uint64_t seed) {
@@ -1316,6 +1432,10 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
       return c->fail(TGS_E_ARG, "a register id is written twice in one step");
   }
   if ((rc = setDevice(c))) return rc;
+  // A step that failed may have left copies or kernels queued that still
+  // read the mailbox: wait for them before the mailbox is rewritten.
+  if (c->stepFailed) HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stepFailed = true;  // until this step returns successfully
   // mailbox: [write ids][read ids][write values, per instance][read values,
   // per instance][control words 0..kCtlBack-1 per instance: stop flag,
   // barrier ticks of the last launch]
@@ -1404,6 +1524,7 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
     }
   if (nRead) std::memcpy(readVals, h + oRv, size_t(ni) * nRead);
   if (anyStopped) *anyStopped = any;
+  c->stepFailed = false;
   return TGS_OK;
 }
 
@@ -1423,6 +1544,8 @@ int tgs_reset_instance(tgs_ctx *c, uint32_t instance, uint64_t seed, uint32_t in
   if (rc) return rc;
   if (instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "instance %u out of range", instance);
   if ((rc = setDevice(c))) return rc;
+  if ((rc = drainLogs(c))) return rc;
+  c->logs[instance].clear();  // the new run's log starts empty
   if (!c->batched) return resetInstance(c, *c->inst[instance], index, seed);
   auto &b = c->bs;
   const auto &d = c->design;
@@ -1526,9 +1649,8 @@ int tgs_set_probes(tgs_ctx *c, const uint32_t *parts, const uint32_t *slots, uin
     if (slots[i] >= d.parts[parts[i]].valuePoolSize)
       return c->fail(TGS_E_ARG, "probe %u: slot %u outside partition %u's value pool", i, slots[i], parts[i]);
   }
-  // carry every instance's state across the re-allocation of its probe buffer
-  std::vector<HostInst> h;
-  if ((rc = downloadAll(c, h))) return rc;
+  if ((rc = setDevice(c))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // no launch still reads the old list
   std::vector<uint32_t> off(nParts + 1, 0), fill(nParts, 0), slot(n), csrOf(n);
   for (uint32_t i = 0; i < n; ++i) off[parts[i] + 1]++;
   for (uint32_t q = 0; q < nParts; ++q) off[q + 1] += off[q];
@@ -1540,9 +1662,23 @@ int tgs_set_probes(tgs_ctx *c, const uint32_t *parts, const uint32_t *slots, uin
   c->probeCsrOf = csrOf;
   if ((rc = upload(c, c->prOff, off.data(), 4ull * off.size()))) return rc;
   if ((rc = upload(c, c->prSlot, slot.data(), 4ull * n))) return rc;
+  // new, zeroed probe buffers; the rest of every instance's state (pools,
+  // control words, print log) stays where it is
+  if (c->batched) {
+    auto &b = c->bs;
+    const auto &d = c->design;
+    HIPCHK(c, b.probe.alloc(std::max<uint64_t>(n, 1) * b.instPad));
+    HIPCHK(c, hipMemsetAsync(b.probe.p, 0, b.probe.n, c->stream));
+    const uint64_t rows = std::max<uint64_t>({regRows(d), memRows(d), exgRows(d), uint64_t(n)}) + 8;
+    if (b.scratch.n < rows * b.instPad) HIPCHK(c, b.scratch.alloc(rows * b.instPad));
+  } else {
+    for (auto &in : c->inst) {
+      HIPCHK(c, in->probe.alloc(std::max<uint32_t>(n, 1)));
+      HIPCHK(c, hipMemsetAsync(in->probe.p, 0, in->probe.n, c->stream));
+    }
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (auto &x : h) x.probe.assign(n, 0);
-  return uploadAll(c, h);
+  return TGS_OK;
 }
 
 int tgs_read_probes(tgs_ctx *c, uint32_t instance, uint8_t *out, uint32_t n) {
@@ -1646,6 +1782,8 @@ int tgs_checkpoint_restore(tgs_ctx *c, uint32_t instance, const uint8_t *buf, ui
     return c->fail(TGS_E_ARG, "checkpoint was taken on another design");
   if (bytes < ckptBytes(d)) return c->fail(TGS_E_FORMAT, "checkpoint truncated");
   if ((rc = setDevice(c))) return rc;
+  if ((rc = drainLogs(c))) return rc;
+  c->logs[instance].clear();  // print events are output of the replaced run, not state
   uint8_t *p = const_cast<uint8_t *>(buf) + sizeof h;
   if ((rc = poolIO(c, instance, TGS_POOL_REG, 0, h.regBytes, p, true))) return rc;
   if ((rc = poolIO(c, instance, TGS_POOL_MEM, 0, h.memBytes, p + h.regBytes, true))) return rc;
@@ -1696,34 +1834,22 @@ int tgs_get_print_log(tgs_ctx *c, uint32_t instance, uint64_t *cycles, uint32_t 
   int rc = checkLoaded(c);
   if (rc) return rc;
   if (!count || instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "bad instance or pointer");
-  if (!c->batched) {
-    Instance *in = c->inst[instance].get();
-    uint32_t n = 0;
-    HIPCHK(c, hipMemcpy(&n, in->ctl.as<uint32_t>() + C_LOGCOUNT, 4, hipMemcpyDeviceToHost));
-    *count = std::min<uint64_t>(n, kLogCap);
-    uint64_t k = std::min<uint64_t>(cap, *count);
-    if (k && cycles) HIPCHK(c, hipMemcpy(cycles, in->logCycle.p, k * 8, hipMemcpyDeviceToHost));
-    if (k && msgs) HIPCHK(c, hipMemcpy(msgs, in->logMsg.p, k * 4, hipMemcpyDeviceToHost));
-    return TGS_OK;
-  }
-  uint32_t n = 0;
-  HIPCHK(c, hipMemcpy(&n, c->bs.logCount.p, 4, hipMemcpyDeviceToHost));
-  const uint64_t total = std::min<uint64_t>(n, kLogCap);
-  std::vector<uint64_t> cy(total), mg(total);
-  if (total) {
-    HIPCHK(c, hipMemcpy(cy.data(), c->bs.logCycle.p, total * 8, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(mg.data(), c->bs.logMsg.p, total * 8, hipMemcpyDeviceToHost));
-  }
-  uint64_t k = 0;
-  for (uint64_t i = 0; i < total; ++i) {
-    if ((mg[i] >> 32) != instance) continue;
-    if (k < cap) {
-      if (cycles) cycles[k] = cy[i];
-      if (msgs) msgs[k] = uint32_t(mg[i]);
-    }
-    ++k;
-  }
-  *count = k;
+  if ((rc = drainLogs(c))) return rc;
+  const HostLog &L = c->logs[instance];
+  *count = L.cyc.size();
+  const uint64_t k = std::min<uint64_t>(cap, *count);
+  if (k && cycles) std::memcpy(cycles, L.cyc.data(), k * 8);
+  if (k && msgs) std::memcpy(msgs, L.msg.data(), k * 4);
+  return TGS_OK;
+}
+
+int tgs_clear_print_log(tgs_ctx *c, uint32_t instance, uint64_t *dropped) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if (instance >= instanceCount(c)) return c->fail(TGS_E_ARG, "instance %u out of range", instance);
+  if ((rc = drainLogs(c))) return rc;
+  if (dropped) *dropped = c->logs[instance].dropped;
+  c->logs[instance].clear();
   return TGS_OK;
 }
 
@@ -1736,6 +1862,54 @@ int tgs_get_profile(tgs_ctx *c, tgs_profile *out) {
 int tgs_reset_profile(tgs_ctx *c) {
   if (!c) return TGS_E_ARG;
   c->prof = tgs_profile{};
+  return TGS_OK;
+}
+
+int tgs_set_part_profile(tgs_ctx *c, int enable, uint32_t start, uint32_t count) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  c->partProf = false;
+  c->profCollected = 0;
+  if (!enable) return TGS_OK;
+  if (count == 0) return c->fail(TGS_E_ARG, "the profile window needs at least one cycle");
+  const size_t parts = c->design.parts.size();
+  // workgroups of any single-instance launch: at most two per CU (stream) or
+  // one per partition of a region (plain, wave64)
+  const size_t groups = std::max<size_t>(parts, 4ull * c->prop.multiProcessorCount);
+  HIPCHK(c, c->profUseful.alloc(8 * std::max<size_t>(parts, 1)));
+  HIPCHK(c, c->profOwner.alloc(4 * std::max<size_t>(parts, 1)));
+  HIPCHK(c, c->profWg.alloc(16 * groups));
+  HIPCHK(c, hipMemset(c->profOwner.p, 0, c->profOwner.n));
+  c->profStart = start;
+  c->profCount = count;
+  c->partProf = true;
+  return TGS_OK;
+}
+
+int tgs_get_part_profile(tgs_ctx *c, int64_t *useful, int64_t *total, size_t count, uint32_t *collected) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  const size_t parts = c->design.parts.size();
+  if (count > parts || (count && (!useful || !total)))
+    return c->fail(TGS_E_ARG, "at most %zu partitions", parts);
+  if (collected) *collected = c->partProf ? c->profCollected : 0;
+  if (!c->partProf || c->profCollected == 0) {
+    for (size_t p = 0; p < count; ++p) useful[p] = total[p] = 0;
+    return TGS_OK;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<uint64_t> u(parts), wg(c->profWg.n / 8);
+  std::vector<uint32_t> owner(parts);
+  HIPCHK(c, hipMemcpy(u.data(), c->profUseful.p, 8 * parts, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(owner.data(), c->profOwner.p, 4 * parts, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(wg.data(), c->profWg.p, c->profWg.n, hipMemcpyDeviceToHost));
+  const size_t n0 = c->design.regionPartitionIds[0].size();
+  const uint64_t k = c->profCollected;
+  for (size_t p = 0; p < count; ++p) {
+    const size_t at = 2ull * owner[p] + (p < n0 ? 0 : 1);
+    useful[p] = int64_t(u[p] / k);
+    total[p] = at < wg.size() ? int64_t(wg[at] / k) : 0;
+  }
   return TGS_OK;
 }
 

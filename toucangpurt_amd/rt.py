@@ -347,6 +347,30 @@ class Simulator:
                                                 msg.ctypes.data_as(_u32p), k, ctypes.byref(n)))
         return sorted(zip(cyc[:k].tolist(), msg[:k].tolist()))
 
+    def clear_print_log(self, instance=0):
+        """Drop the instance's held print events (consumed); returns how many
+        events were lost since the last clear (0 unless a launch overflowed)."""
+        d = ctypes.c_uint64()
+        self._check(self._lib.tgs_clear_print_log(self._h, int(instance), ctypes.byref(d)))
+        return d.value
+
+    def set_part_profile(self, enable=True, start=100, count=100):
+        """Per-partition useful/total clock ticks of launch-local cycles
+        start .. start + count - 1 (ENABLE_SIM_PROFILE, src/SimEval.cu:894-983)."""
+        self._check(self._lib.tgs_set_part_profile(self._h, 1 if enable else 0, int(start), int(count)))
+
+    def part_profile(self):
+        """(useful, total, cycles averaged): per-partition average ticks of the
+        last launch that reached the profile window (copy_profile_data)."""
+        n = sum(self.program_stats()["n_parts"])
+        u = np.zeros(max(n, 1), np.int64)
+        t = np.zeros(max(n, 1), np.int64)
+        k = ctypes.c_uint32()
+        self._check(self._lib.tgs_get_part_profile(self._h, u.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                   t.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), n,
+                                                   ctypes.byref(k)))
+        return u[:n], t[:n], k.value
+
     def profile(self):
         p = _native.Profile()
         self._check(self._lib.tgs_get_profile(self._h, ctypes.byref(p)))
