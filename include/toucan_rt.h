@@ -137,6 +137,20 @@ int tgs_eval_free_running(tgs_ctx *ctx, uint32_t cycles, int *stopped);
 /* Batched run: instance k advances cycles[k] cycles (uneven counts). */
 int tgs_run_batch(tgs_ctx *ctx, const uint32_t *cycles, uint32_t n, int *any_stopped);
 
+/* No counterpart in the reference: tgs_run_batch for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as tgs_run_batch on a batched context; the
+ * per-launch scratch of the batched kernel (slot cycle counts and bases, the
+ * active-group table, the grid-barrier counter) lives in a static array of
+ * 311,552 bytes per GPU, held until the process ends, instead of context
+ * allocations, and no hipMalloc / hipFree or other allocating call is made.
+ * At most 4,096 instances (TGS_E_LIMIT beyond); the batched program image
+ * must be built, which tgs_set_kernel(TGS_KERNEL_BATCH), tgs_set_instances,
+ * tgs_set_stimulus and tgs_set_launch do (TGS_E_STATE otherwise). Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (from
+ * several threads or contexts) race and must be serialized by the caller. */
+int tgs_run_batch_noalloc(tgs_ctx *ctx, const uint32_t *cycles, uint32_t n, int *any_stopped);
+
 /* read_reg_from_gpu / write_reg_to_gpu (src/SimEval.cu:1058-1081), batched:
  * one nibble per id, no per-nibble memcpy. */
 int tgs_read_regs(tgs_ctx *ctx, uint32_t instance, const uint32_t *ids, uint32_t n, uint8_t *out);

@@ -94,6 +94,9 @@ hipError_t streamSetLds(uint32_t lds);
 hipError_t launchBatch(const BatchArgs &args, uint32_t grid, uint32_t block, uint32_t lds, hipStream_t stream);
 hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t block, uint32_t lds);
 hipError_t batchSetLds(uint32_t lds);
+// out[0 .. n) = value (a kernel, so callers that record their work into a
+// HIP graph get no memset node).
+hipError_t launchFill32(uint32_t *out, uint64_t n, uint32_t value, hipStream_t stream);
 hipError_t launchSlotColumn(uint8_t *pool, uint32_t instPad, uint32_t slot, uint64_t rows, uint8_t *dense, bool toPool,
                             hipStream_t stream);
 hipError_t launchPermute(const uint8_t *in, uint8_t *out, uint64_t rows, uint32_t instPad, const uint32_t *src,

@@ -477,6 +477,18 @@ hipError_t launchRegMailbox(uint8_t *pool, uint32_t instPad, const uint32_t *ins
   return hipGetLastError();
 }
 
+__global__ void fill32(uint32_t *out, uint64_t n, uint32_t value) {
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+    out[i] = value;
+}
+
+hipError_t launchFill32(uint32_t *out, uint64_t n, uint32_t value, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint32_t grid = uint32_t(std::min<uint64_t>((n + 255) / 256, 1024));
+  fill32<<<dim3(grid), dim3(256), 0, stream>>>(out, n, value);
+  return hipGetLastError();
+}
+
 hipError_t launchSlotColumn(uint8_t *pool, uint32_t instPad, uint32_t slot, uint64_t rows, uint8_t *dense, bool toPool,
                             hipStream_t stream) {
   if (rows == 0) return hipSuccess;

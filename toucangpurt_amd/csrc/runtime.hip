@@ -14,6 +14,7 @@
 #include <fstream>
 #include <iterator>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -26,6 +27,18 @@
 using namespace tgs;
 using toucanGPUSim::SimDebugInfo;
 using toucanGPUSim::SimDesignInfo;
+
+// Scratch memory of tgs_run_batch_noalloc: one static array per GPU, held
+// until the process ends. Layout: per-slot cycle counts (u32) and cycle bases
+// (u64) of up to kScratchSlots instance slots, the grid-barrier counter, then
+// the active-group table of up to kScratchCycles launch-cycles. In host code
+// `scratch` names a host-side stand-in; its device address on each GPU comes
+// from hipGetSymbolAddress (scratchFor) and only that address is used.
+constexpr uint32_t kScratchSlots = 4096, kScratchCycles = 65536;
+constexpr size_t kScratchRunOff = 0, kScratchBaseOff = 4ull * kScratchSlots,
+                 kScratchBarrierOff = kScratchBaseOff + 8ull * kScratchSlots, kScratchActiveOff = kScratchBarrierOff + 256,
+                 kScratchBytes = kScratchActiveOff + 4ull * kScratchCycles;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
 
 namespace {
 
@@ -190,6 +203,10 @@ struct tgs_ctx {
   std::vector<HostLog> logs;
   bool logDirty = false;
   bool stepFailed = false;  // the last tgs_step returned an error after queueing work
+  // host bookkeeping of batched runs (kept so steady-state runs allocate nothing)
+  std::vector<uint64_t> bhRemaining, bhBase, bhStop;
+  std::vector<uint32_t> bhRun, bhActive;
+  void *scratchDev = nullptr;  // this device's static scratch array (tgs_run_batch_noalloc)
   SnapRing snap;
   bool timingPending = false;  // the last launch's events are not accounted yet
   // testbench mailbox (tgs_step): pinned host staging and its device twin
@@ -929,13 +946,30 @@ int batchOrder(tgs_ctx *c, const uint32_t *cycles) {
   return TGS_OK;
 }
 
-int runBatched(tgs_ctx *c, const uint32_t *cycles) {
+// Device memory a batched run needs beyond the instance state: per-slot
+// cycle counts and bases of the launch, the per-launch-cycle active-group
+// table and the grid-barrier counter. `fixedCycles` = 0: the context's own
+// buffers, the table allocated per launch; otherwise preallocated memory
+// whose table holds at most fixedCycles entries (launches are cut to fit).
+struct BatchScratch {
+  uint32_t *runCycles = nullptr;
+  uint64_t *cycleBase = nullptr;
+  uint32_t *groupsActive = nullptr;
+  uint32_t *barrier = nullptr;
+  uint64_t fixedCycles = 0;
+};
+
+int runBatched(tgs_ctx *c, const uint32_t *cycles, const BatchScratch *fixed = nullptr) {
   int rc;
   if ((rc = ensureBatchImage(c))) return rc;
   auto &b = c->bs;
   const uint32_t P = b.instPad;
   if ((rc = batchOrder(c, cycles))) return rc;
-  std::vector<uint64_t> remaining(P, 0), base(P, 0);
+  // host bookkeeping kept in the context: no host allocation once sized
+  auto &remaining = c->bhRemaining, &base = c->bhBase;
+  auto &run = c->bhRun, &active = c->bhActive;
+  remaining.assign(P, 0);
+  base.assign(P, 0);
   uint64_t maxC = 0;
   for (uint32_t s = 0; s < P; ++s)
     if (b.instOf[s] != 0xFFFFFFFFu) {
@@ -944,8 +978,9 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
       maxC = std::max(maxC, remaining[s]);
     }
   const uint64_t perLaunch = std::min<uint64_t>({1u << 20, 0xE0000000ull / (2ull * c->batchGrid + 2),
-                                                  printCycleCap(c, b.logCap, P)});
-  std::vector<uint32_t> run(P), active;
+                                                  printCycleCap(c, b.logCap, P),
+                                                  fixed ? fixed->fixedCycles : uint64_t(UINT32_MAX)});
+  run.assign(P, 0);
   while (maxC > 0) {
     if ((rc = drainLogs(c))) return rc;  // every launch starts with an empty device log
     const uint32_t L = uint32_t(std::min<uint64_t>(maxC, perLaunch));
@@ -956,11 +991,21 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
       while (g < P / 64 && run[g * 64] > cy) ++g;  // slots sorted by decreasing cycles
       active[cy] = g;
     }
-    HIPCHK(c, b.groupsActive.alloc(4ull * L));
-    HIPCHK(c, hipMemcpyAsync(b.runCycles.p, run.data(), 4ull * P, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b.cycleBase.p, base.data(), 8ull * P, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(b.groupsActive.p, active.data(), 4ull * L, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemsetAsync(b.barrier.p, 0, 4, c->stream));
+    BatchScratch sc;
+    if (fixed) {
+      sc = *fixed;
+      HIPCHK(c, launchFill32(sc.barrier, 1, 0, c->stream));  // a kernel, not a memset node
+    } else {
+      HIPCHK(c, b.groupsActive.alloc(4ull * L));
+      sc.runCycles = b.runCycles.as<uint32_t>();
+      sc.cycleBase = b.cycleBase.as<uint64_t>();
+      sc.groupsActive = b.groupsActive.as<uint32_t>();
+      sc.barrier = b.barrier.as<uint32_t>();
+      HIPCHK(c, hipMemsetAsync(sc.barrier, 0, 4, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(sc.runCycles, run.data(), 4ull * P, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sc.cycleBase, base.data(), 8ull * P, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(sc.groupsActive, active.data(), 4ull * L, hipMemcpyHostToDevice, c->stream));
     BatchArgs a{};
     a.lut = c->lut.as<uint8_t>();
     a.consts = c->wConsts.as<uint8_t>();
@@ -977,9 +1022,9 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
     a.n0 = c->w64.n0;
     a.nTotal = c->w64.nTotal;
     a.vpBytes = up16(std::max<uint32_t>(c->w64.maxVp, 16));
-    a.runCycles = b.runCycles.as<uint32_t>();
-    a.cycleBase = b.cycleBase.as<uint64_t>();
-    a.groupsActive = b.groupsActive.as<uint32_t>();
+    a.runCycles = sc.runCycles;
+    a.cycleBase = sc.cycleBase;
+    a.groupsActive = sc.groupsActive;
     a.nCycles = L;
     a.xs = b.xs.as<uint64_t>();
     a.draws = b.draws.as<uint64_t>();
@@ -993,7 +1038,7 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles) {
     a.instOf = b.instOfD.as<uint32_t>();
     a.logCap = uint32_t(b.logCap);
     a.printEnable = c->printEnable;
-    a.barrier = b.barrier.as<uint32_t>();
+    a.barrier = sc.barrier;
     if (c->nProbe()) {
       a.probeOff = c->prOff.as<uint32_t>();
       a.probeSlot = c->prSlot.as<uint32_t>();
@@ -1089,6 +1134,23 @@ int tgs_initial_state(const uint8_t *design, size_t designSize, const uint8_t *s
   return TGS_OK;
 }
 
+// This device's address of `scratch`, resolved once per device with the
+// device current (the array exists on every GPU the library runs on).
+static hipError_t scratchFor(int device, void **out) {
+  static std::mutex mu;
+  static std::vector<void *> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (size_t(device) >= addr.size()) addr.resize(size_t(device) + 1, nullptr);
+  if (!addr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *out = addr[device];
+  return hipSuccess;
+}
+
 static int createCtx(int device, hipStream_t userStream, tgs_ctx **out) {
   if (!out) return TGS_E_ARG;
   *out = nullptr;
@@ -1112,6 +1174,10 @@ static int createCtx(int device, hipStream_t userStream, tgs_ctx **out) {
     return TGS_E_HIP;
   }
   if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return TGS_E_HIP;
+  if (scratchFor(device, &c->scratchDev) != hipSuccess) {
+    (void)hipGetLastError();
+    return TGS_E_HIP;
+  }
   const char *blk = std::getenv("TGS_W64_BLOCK");
   if (blk) c->w64Block = std::max(64u, std::min(1024u, uint32_t(std::atoi(blk)) / 64 * 64));
   const char *grp = std::getenv("TGS_W64_GROUPS");
@@ -1219,6 +1285,7 @@ int tgs_set_kernel(tgs_ctx *c, int kernel) {
     int rc = setDevice(c);
     if (rc) return rc;
     if ((rc = switchLayout(c, kernel == TGS_KERNEL_BATCH))) return rc;
+    if (kernel == TGS_KERNEL_BATCH && (rc = ensureBatchImage(c))) return rc;
   }
   c->kernel = kernel;
   c->kernelChosen = true;
@@ -1242,6 +1309,10 @@ int tgs_set_launch(tgs_ctx *c, int kernel, uint32_t block, uint32_t groups) {
       if (block) c->batchBlock = block;
       c->batchGridOverride = groups;
       c->batchReady = false;
+      if (c->loaded && c->batched) {  // the batched image is rebuilt now, not at the next run
+        int rc = setDevice(c);
+        if (rc || (rc = ensureBatchImage(c))) return rc;
+      }
       break;
     case TGS_KERNEL_STREAM:
       c->streamNC = 0;  // automatic
@@ -1274,7 +1345,7 @@ int tgs_set_instances(tgs_ctx *c, uint32_t n) {
   if ((rc = uploadAll(c, h))) return rc;
   c->logs.assign(n, HostLog());
   c->logDirty = false;
-  return TGS_OK;
+  return c->batched ? ensureBatchImage(c) : TGS_OK;
 }
 
 int tgs_set_stimulus(tgs_ctx *c, const uint32_t *ids, uint32_t n, uint64_t seed) {
@@ -1301,7 +1372,7 @@ int tgs_set_stimulus(tgs_ctx *c, const uint32_t *ids, uint32_t n, uint64_t seed)
       HIPCHK(c, hipMemcpy(c->inst[i]->xs.p, &z, 8, hipMemcpyHostToDevice));
     }
   }
-  return TGS_OK;
+  return c->batched ? ensureBatchImage(c) : TGS_OK;
 }
 
 int tgs_run_batch(tgs_ctx *c, const uint32_t *cycles, uint32_t n, int *anyStopped) {
@@ -1322,6 +1393,33 @@ int tgs_run_batch(tgs_ctx *c, const uint32_t *cycles, uint32_t n, int *anyStoppe
       any |= s;
     }
   }
+  if (anyStopped) *anyStopped = any;
+  return TGS_OK;
+}
+
+int tgs_run_batch_noalloc(tgs_ctx *c, const uint32_t *cycles, uint32_t n, int *anyStopped) {
+  int rc = checkLoaded(c);
+  if (rc) return rc;
+  if (!c->batched) return c->fail(TGS_E_STATE, "tgs_run_batch_noalloc runs batched contexts (tgs_set_kernel batched)");
+  if (!cycles || n != instanceCount(c)) return c->fail(TGS_E_ARG, "need one cycle count per instance");
+  if (c->bs.instPad > kScratchSlots)
+    return c->fail(TGS_E_LIMIT, "tgs_run_batch_noalloc holds at most %u instances (%u given)", kScratchSlots, n);
+  if (!c->batchReady)
+    return c->fail(TGS_E_STATE, "the batched program image is not built (tgs_set_instances or tgs_set_stimulus "
+                                "after the last tgs_set_launch builds it)");
+  uint8_t *base = static_cast<uint8_t *>(c->scratchDev);
+  BatchScratch sc;
+  sc.runCycles = reinterpret_cast<uint32_t *>(base + kScratchRunOff);
+  sc.cycleBase = reinterpret_cast<uint64_t *>(base + kScratchBaseOff);
+  sc.barrier = reinterpret_cast<uint32_t *>(base + kScratchBarrierOff);
+  sc.groupsActive = reinterpret_cast<uint32_t *>(base + kScratchActiveOff);
+  sc.fixedCycles = kScratchCycles;
+  if ((rc = runBatched(c, cycles, &sc))) return rc;
+  auto &st = c->bhStop;
+  st.resize(c->bs.instPad);
+  HIPCHK(c, hipMemcpy(st.data(), c->bs.stop.p, 8 * st.size(), hipMemcpyDeviceToHost));
+  int any = 0;
+  for (auto v : st) any |= v != 0;
   if (anyStopped) *anyStopped = any;
   return TGS_OK;
 }
