@@ -264,7 +264,7 @@ def run_batched(args):
     for part in ((args.part_luts,) if args.part_luts else (256, 512, 1024, 2048, 4096)):
         d = gen.generate("lut200k", seed=args.seed, part_luts=part)
         designs[part] = d
-        for block in (128,):
+        for block in (0,):
             sim = rt.Simulator(0)
             assert sim.init(0, d.design, d.symbols, False, False) == 0
             try:
@@ -277,10 +277,12 @@ def run_batched(args):
                 sim.run_batch(np.full(n, args.tune_cycles, np.uint32))
                 rate = n * args.tune_cycles / (time.perf_counter() - t0)
             except SimError as e:
-                tuning.append({"part_luts": part, "block": block, "error": str(e)[:80]})
+                tuning.append({"part_luts": part, "error": str(e)[:80]})
                 sim.cleanup()
                 continue
-            tuning.append({"part_luts": part, "block": block, "rate": rate})
+            tp = sim.profile()
+            tuning.append({"part_luts": part, "block": tp["block"], "grid": tp["grid"],
+                           "instances_per_wave": tp["batch_group"], "rate": rate})
             log(f"batched tune part_luts={part} block={block}: {rate:.3e} stimulus-cycles/s")
             if best is None or rate > best[0]:
                 best = (rate, part, block)
@@ -313,8 +315,8 @@ def run_batched(args):
     value = total / dt
     log(f"batched: {value:.3e} stimulus-cycles/s over {total} instance-cycles in {dt:.2f} s")
     out = {"value": value, "unit": "stimulus-cycles/s", "instances": n, "total_cycles": total, "seconds": dt,
-           "kernel_ms": prof["kernel_ms"], "part_luts": part, "block": block, "grid": prof["grid"],
-           "lds_bytes": prof["lds_bytes"], "tuning": tuning,
+           "kernel_ms": prof["kernel_ms"], "part_luts": part, "block": prof["block"], "grid": prof["grid"],
+           "instances_per_wave": prof["batch_group"], "lds_bytes": prof["lds_bytes"], "tuning": tuning,
            "workload": "lut200k: 200,000 LUT4, depth 48, 8 SRAM 1024x32b, 1,024 instances, "
                        f"cycles uniform in [{args.min_cycles}, {args.max_cycles}]; " + FANIN,
            "bytes_per_cycle_shared": stats["b_prog"], "bytes_per_cycle_per_instance": stats["b_state"]}

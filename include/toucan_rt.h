@@ -70,6 +70,8 @@ typedef struct tgs_profile {
   uint64_t barrier_ticks; /* wave64 / stream: real-time ticks spent waiting at grid barriers, summed over
                              every workgroup of every launch */
   uint64_t total_ticks;   /* wave64 / stream: real-time ticks of the cycle loop, summed the same way */
+  uint32_t batch_group;   /* batched: instances per wave (each wave evaluates 64 / batch_group ops per round) */
+  uint32_t reserved;
 } tgs_profile;
 
 /* ---- program inspection (host only, no device needed) ---- */

@@ -119,6 +119,8 @@ class Profile(ctypes.Structure):
         ("prog_bytes", ctypes.c_uint64),
         ("barrier_ticks", ctypes.c_uint64),
         ("total_ticks", ctypes.c_uint64),
+        ("batch_group", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

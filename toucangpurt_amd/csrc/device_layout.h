@@ -57,7 +57,24 @@ enum W64Hdr : uint32_t {
 };
 
 // Task kinds (low byte of a task's first word).
-enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3 };
+enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3, T_LUTROWS = 4 };
+// T_LUTROWS (batched image only): the ops of both micropartitions of a
+// T_LUT2 pair with every operand and destination resolved to a row of the
+// wave's LDS area: value-pool rows [0, V) then level rows: buffer X
+// [0, 64), buffer Y [64, 128) (halves A and B at +0 / +32), and constant
+// rows [128, 160) holding 0..31 (level rows are relative to row V).
+//   w0  kind | nMid << 8
+//   w1  nTop (A + B) | nLast << 16
+//   w2  word offset (from w0) of the middle ops, w3 of the last ops
+//   ceil(nMid / 4) words of level sizes, 8 bits each (A + B ops)
+//   top ops (even offset), 2 words each: lut | r0 << 16,
+//       r1 | r2 << 13 | dst << 26 (r*: absolute rows < 8192, dst a level row)
+//   middle ops per level, 2 words each: lut | r0 << 14 | r1 << 22
+//       (level rows), r2 (a level row) | dst << 8 (an absolute row: a level
+//       row + V, or, on the last middle level, the value-pool slot of the
+//       one last-level op that copies the lane, which is then dropped)
+//   last ops, 1 word each: value-pool slot | source level row << 16
+constexpr uint32_t kLevelRows = 160, kConstRow = 128;
 // T_LUT2: two on-disk 32-lane micropartitions in one wave (lanes 0-31 = A,
 //   lanes 32-63 = B). Words:
 //   w0  kind | nMid << 8

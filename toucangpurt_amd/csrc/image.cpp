@@ -416,7 +416,110 @@ PlainImage buildPlainImage(const SimDesignInfo &d, const StimMap &stim) {
 // ---------------------------------------------------------------------------
 // Wave64 image
 
-Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32_t nWaves) {
+// A T_LUT2 pair (A, B may be null) as a T_LUTROWS task for a value pool of V
+// rows. Lanes at or past the previous level's size read 0 (SimEval.cu:
+// 372-375,406-408, 441-446) and become the constant row of 0; a pass-through
+// op's unused op0/op1 read the constant rows too.
+static std::vector<uint32_t> lutRowsTask(const CGMicroPartInfo *A, const CGMicroPartInfo *B, uint32_t V) {
+  const uint32_t nMid = A->middleLevels.size();
+  const CGMicroPartInfo *H[2] = {A, B};
+  auto prevSize = [](const CGMicroPartInfo *m, uint32_t l) -> uint32_t {
+    return uint32_t(l == 0 ? m->topLevel.size() : m->middleLevels[l - 1].size());
+  };
+  // level row of reference lane `o` (< 32 immediate, else lane o mod 32) read
+  // from buffer base `rb` by half h
+  auto laneRow = [&](uint32_t o, uint32_t h, uint32_t rb, uint32_t nPrev) -> uint32_t {
+    if (o < 32) return kConstRow + o;
+    return (o & 31u) < nPrev ? rb + 32 * h + (o & 31u) : kConstRow;
+  };
+  std::vector<uint32_t> t;
+  uint32_t nt = 0, nl = 0;
+  for (auto *m : H)
+    if (m) {
+      nt += m->topLevel.size();
+      nl += m->lastLevel.size();
+    }
+  t.push_back(T_LUTROWS | nMid << 8);
+  t.push_back(nt | nl << 16);
+  t.push_back(0);
+  t.push_back(0);
+  for (uint32_t l = 0; l < nMid; l += 4) {
+    uint32_t v = 0;
+    for (uint32_t k = 0; k < 4 && l + k < nMid; ++k) {
+      uint32_t n = 0;
+      for (auto *m : H)
+        if (m) n += m->middleLevels[l + k].size();
+      v |= n << (8 * k);
+    }
+    t.push_back(v);
+  }
+  if (t.size() & 1) t.push_back(0);
+  // top level: value-pool operands (slots below 16 are immediates: constant rows)
+  auto topRow = [&](uint32_t s) -> uint32_t { return s < 16 ? V + kConstRow + s : s; };
+  for (uint32_t h = 0; h < 2; ++h) {
+    if (!H[h]) continue;
+    uint32_t k = 0;
+    for (const auto &o : H[h]->topLevel) {
+      const uint32_t r0 = o.lutIndex ? topRow(o.op0) : V + kConstRow, r1 = o.lutIndex ? topRow(o.op1) : V + kConstRow;
+      t.push_back(uint32_t(o.lutIndex) | r0 << 16);
+      t.push_back(r1 | topRow(o.op2) << 13 | (32 * h + k) << 26);
+      ++k;
+    }
+  }
+  t[2] = t.size();
+  // A lane of the last middle level that exactly one last-level op copies
+  // (the usual case) is written straight to that op's value-pool slot, and
+  // the op is dropped; fan-outs and immediates stay last-level ops.
+  std::vector<int32_t> direct[2];  // per half and lane: the one copying last op, -1 none, -2 several
+  for (uint32_t h = 0; h < 2 && nMid; ++h) {
+    if (!H[h]) continue;
+    const uint32_t nPrev = prevSize(H[h], nMid);
+    direct[h].assign(32, -1);
+    for (uint32_t q = 0; q < H[h]->lastLevel.size(); ++q) {
+      const uint32_t sid = H[h]->lastLevel[q].shuffleId;
+      if (sid < 32 || (sid & 31u) >= nPrev) continue;
+      int32_t &e = direct[h][sid & 31u];
+      e = e == -1 ? int32_t(q) : -2;
+    }
+  }
+  for (uint32_t l = 0; l < nMid; ++l) {
+    const uint32_t rb = (l & 1) ? 64 : 0, wb = (l & 1) ? 0 : 64;
+    for (uint32_t h = 0; h < 2; ++h) {
+      if (!H[h]) continue;
+      const uint32_t nPrev = prevSize(H[h], l);
+      uint32_t k = 0;
+      for (const auto &o : H[h]->middleLevels[l]) {
+        const uint32_t lt = o.lutIndex();
+        const uint32_t r0 = lt ? laneRow(o.op0(), h, rb, nPrev) : kConstRow;
+        const uint32_t r1 = lt ? laneRow(o.op1(), h, rb, nPrev) : kConstRow;
+        uint32_t dst = V + wb + 32 * h + k;
+        if (l + 1 == nMid && k < 32 && direct[h][k] >= 0) dst = H[h]->lastLevel[direct[h][k]].result;
+        t.push_back(lt | r0 << 14 | r1 << 22);
+        t.push_back(laneRow(o.op2(), h, rb, nPrev) | dst << 8);
+        ++k;
+      }
+    }
+  }
+  t[3] = t.size();
+  const uint32_t fb = (nMid & 1) ? 64 : 0;
+  uint32_t nlKept = 0;
+  for (uint32_t h = 0; h < 2; ++h) {
+    if (!H[h]) continue;
+    const uint32_t nPrev = prevSize(H[h], nMid);
+    for (uint32_t q = 0; q < H[h]->lastLevel.size(); ++q) {
+      const auto &o = H[h]->lastLevel[q];
+      const uint32_t sid = o.shuffleId;
+      if (nMid && sid >= 32 && (sid & 31u) < nPrev && direct[h][sid & 31u] == int32_t(q)) continue;
+      t.push_back(uint32_t(o.result) | laneRow(sid, h, fb, nPrev) << 16);
+      ++nlKept;
+    }
+  }
+  t[1] = nt | nlKept << 16;
+  if (t.size() & 1) t.push_back(0);
+  return t;
+}
+
+Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32_t nWaves, uint32_t vpRows) {
   Wave64Image img;
   img.n0 = d.regionPartitionIds[0].size();
   img.nTotal = d.parts.size();
@@ -470,6 +573,11 @@ Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32
         if (i + 1 < luts.size() && luts[i + 1]->middleLevels.size() == A->middleLevels.size()) B = luts[i + 1];
         i += B ? 2 : 1;
         uint32_t nMid = A->middleLevels.size();
+        if (vpRows) {
+          st.push_back(lutRowsTask(A, B, vpRows));
+          levelCost += nMid + 2;
+          continue;
+        }
         std::vector<uint32_t> t;
         t.push_back(T_LUT2 | nMid << 8);
         uint32_t ntA = A->topLevel.size(), ntB = B ? B->topLevel.size() : 0;
@@ -495,13 +603,32 @@ Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32
         };
         tops(A);
         tops(B);
+        // Operands that name a lane at or past the previous level's size read
+        // 0 (SimEval.cu:372-375,406-408): they are stored as the immediate 0,
+        // and a pass-through op's unused op0/op1 as immediates too, so the
+        // kernels never need the previous level's size to read an operand.
+        auto prevSize = [](const CGMicroPartInfo *m, uint32_t l) -> uint32_t {
+          return uint32_t(l == 0 ? m->topLevel.size() : m->middleLevels[l - 1].size());
+        };
+        auto lane = [](uint32_t o, uint32_t nPrev) -> uint32_t { return o >= 32 && (o & 31u) >= nPrev ? 0u : o; };
+        auto mids = [&](const CGMicroPartInfo *m, uint32_t l) {
+          if (!m) return;
+          const uint32_t nPrev = prevSize(m, l);
+          for (const auto &o : m->middleLevels[l]) {
+            const uint32_t w = o.getPacked(), lt = w >> 18;
+            const uint32_t o0 = lt ? lane((w >> 12) & 63u, nPrev) : 0u, o1 = lt ? lane((w >> 6) & 63u, nPrev) : 0u;
+            t.push_back(lt << 18 | o0 << 12 | o1 << 6 | lane(w & 63u, nPrev));
+          }
+        };
         for (uint32_t l = 0; l < nMid; ++l) {
-          for (const auto &o : A->middleLevels[l]) t.push_back(o.getPacked());
-          if (B) for (const auto &o : B->middleLevels[l]) t.push_back(o.getPacked());
+          mids(A, l);
+          mids(B, l);
         }
         auto lasts = [&](const CGMicroPartInfo *m) {
           if (!m) return;
-          for (const auto &o : m->lastLevel) t.push_back(uint32_t(o.result) | uint32_t(o.shuffleId) << 16);
+          const uint32_t nPrev = prevSize(m, nMid);
+          for (const auto &o : m->lastLevel)
+            t.push_back(uint32_t(o.result) | lane(uint32_t(o.shuffleId), nPrev) << 16);
         };
         t[2] = uint32_t(t.size());
         lasts(A);
@@ -570,6 +697,7 @@ Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32
     uint32_t ti = 0;
     for (auto &lvStreams : streams)
       for (auto &t : lvStreams) {
+        if (vpRows && (w.size() & 1)) w.push_back(0);  // batched: every task at an even word
         w[taskTab + ti++] = w.size();
         w.insert(w.end(), t.begin(), t.end());
       }

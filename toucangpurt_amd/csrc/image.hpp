@@ -48,7 +48,11 @@ struct Wave64Image {
   uint32_t maxVp = 0;
   uint32_t n0 = 0, nTotal = 0;
 };
-Wave64Image buildWave64Image(const toucanGPUSim::SimDesignInfo &d, const StimMap &stim, uint32_t nWaves);
+// vpRows > 0: the batched kernel's image, LUT pairs as T_LUTROWS tasks for
+// waves whose value pool has vpRows rows (level rows follow them); every
+// task starts at an even word.
+Wave64Image buildWave64Image(const toucanGPUSim::SimDesignInfo &d, const StimMap &stim, uint32_t nWaves,
+                             uint32_t vpRows = 0);
 
 struct Wave64Schedule {
   uint32_t groups = 0;

@@ -46,10 +46,11 @@ struct BatchArgs {
   uint32_t instPad;               // slots, a multiple of 64
   const uint32_t *words, *partOff, *partWords;  // wave64 image
   uint32_t n0, nTotal;
-  uint32_t vpBytes;               // value-pool slots per lane (16-multiple)
+  uint32_t vpBytes;               // value-pool slots per instance (16-multiple)
+  uint32_t progWords;             // words of one LDS program buffer (largest partition, 4-multiple)
   const uint32_t *runCycles;      // per slot: cycles to run in this launch
   const uint64_t *cycleBase;      // per slot: cycles completed before this launch
-  const uint32_t *groupsActive;   // per launch cycle: groups with a live slot (a prefix)
+  const uint32_t *groupsActive;   // per launch cycle: instance groups with a live slot (a prefix)
   uint32_t nCycles;               // longest run in this launch
   uint64_t *xs;                   // per slot xorshift64 state
   uint64_t *draws;                // [2][nDraws][instPad]
@@ -91,8 +92,13 @@ hipError_t launchStream(const StreamArgs &args, uint32_t nc, uint32_t grid, uint
 hipError_t streamMaxBlocksPerCU(int *blocks, uint32_t nc, uint32_t lds);
 hipError_t streamSetLds(uint32_t lds);
 
-hipError_t launchBatch(const BatchArgs &args, uint32_t grid, uint32_t block, uint32_t lds, hipStream_t stream);
-hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t block, uint32_t lds);
+// Batched kernel with `inst` instances per wave (8, 16, 32 or 64; 64 / inst ops
+// per round) and block / 64 waves per workgroup.
+bool batchSupportsGroup(uint32_t inst);
+uint32_t batchLdsBytes(uint32_t inst, uint32_t waves, uint32_t vpBytes, uint32_t progWords);
+hipError_t launchBatch(const BatchArgs &args, uint32_t inst, uint32_t grid, uint32_t block, uint32_t lds,
+                       hipStream_t stream);
+hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t inst, uint32_t block, uint32_t lds);
 hipError_t batchSetLds(uint32_t lds);
 // out[0 .. n) = value (a kernel, so callers that record their work into a
 // HIP graph get no memset node).

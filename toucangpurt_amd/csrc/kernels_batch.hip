@@ -1,19 +1,28 @@
 // Batched cycle kernel for gfx950: many stimulus instances of one design in
-// one persistent launch, one instance per lane.
+// one persistent launch.
 //
 // Semantics per instance are the reference evaluator's (src/SimEval.cu:304-761,
 // :765-864, :889-965). The execution is laid out for 64-wide waves:
-//   * every pool is instance-minor (nibble k of slot s at k * instPad + s), so
-//     a 64-instance group's nibble k is 64 contiguous bytes; L0 reads and the
-//     last-level register / exchange copies move them 16 bytes per lane,
-//     16 ops per wave instruction;
-//   * a work item is (partition, up to W groups): the partition's program is
-//     staged into LDS once and each of the W waves runs the whole partition
-//     for its own group with its own value pool [slot][lane] and level buffer
-//     [2][32][64] in LDS, so waves never wait on each other inside a region;
-//   * ops of one micropartition level are issued four at a time so their LDS
-//     latencies overlap; a lane index past the previous level's size reads 0,
-//     as the reference's idle lanes do (SimEval.cu:372-375,406-408).
+//   * a wave serves a group of I instances (I = 16, 32 or 64) and W = 64 / I
+//     ops at once: lane l evaluates op sub = l / I of the current round for
+//     instance ii = l % I. A round of W ops costs what one op costs when
+//     every lane is an instance, so small I cuts the latency of one
+//     (partition, group) item by W and its LDS by W, which lets W times more
+//     waves share a CU;
+//   * every pool is instance-minor (nibble k of slot s at k * instPad + s),
+//     so a group's nibble k is I contiguous bytes, moved 16 bytes per lane by
+//     the L0 reads and the last-level word copies;
+//   * a work item is (partition, up to NW groups): the workgroup stages the
+//     partition's program words in LDS once and each of its NW waves runs the
+//     whole partition for its own group, with its own value pool [slot][I]
+//     and micropartition level buffers [2][half][32][I] in LDS, so waves
+//     never wait on each other inside a partition;
+//   * an op index past the end of a level is clamped to the level's last op:
+//     the lane recomputes that op and rewrites the same value, so no lane
+//     needs a branch;
+//   * a reference lane index past the previous level's size (which reads 0,
+//     SimEval.cu:372-375,406-408) is stored as the immediate 0 by the image
+//     builder.
 // Instances run uneven cycle counts: the host orders instance slots by
 // decreasing cycle count, so the live groups of a cycle are a prefix and the
 // live lanes of a group are a prefix; lanes past their own count never write.
@@ -22,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernel_args.h"
 
@@ -31,233 +41,276 @@ namespace {
 
 constexpr uint32_t kLutBytesB = ((kLutSize + 15) / 16) * 16;
 
-struct Ctx {
+// One wave's view of its group of I instances.
+template <int I>
+struct Grp {
+  static constexpr uint32_t W = 64 / I;  // ops per round
+  uint8_t *vp;         // value pool rows [slot][I], then the level rows [kLevelRows][I]
+  uint8_t *lvl;        // row 0 of the level rows
+  const uint8_t *lut;  // the LUT (4,642 entries + zero padding)
   const uint8_t *mem;
   uint64_t memSize;
   size_t instPad;
-  uint8_t *vp;    // this wave's value pool [slot][64]
-  uint8_t *lvl;   // this wave's level buffer [2][32][64]
-  const uint8_t *lut;
-  uint32_t lane, slot0;
+  uint32_t ii, sub;    // instance within the group, op within the round
+  uint32_t slot0;      // first instance slot of the group
+  __device__ __forceinline__ uint32_t at(uint32_t s) const { return vp[s * uint32_t(I) + ii]; }
+  // Operand: slots below 16 are immediates. The read is unconditional (every
+  // slot a program names is inside the value pool, which has at least 16
+  // slots), so operand reads need no branch and are issued together.
+  __device__ __forceinline__ uint32_t opnd(uint32_t s) const {
+    const uint32_t v = at(s);
+    return s < 16 ? s : v;
+  }
+  __device__ __forceinline__ void put(uint32_t s, uint32_t v) const { vp[s * uint32_t(I) + ii] = uint8_t(v); }
+  // op index of this lane in round `r` of a level of n >= 1 ops
+  __device__ __forceinline__ uint32_t opOf(uint32_t r, uint32_t n) const {
+    const uint32_t q = r + sub;
+    return q < n ? q : n - 1;
+  }
 };
 
-__device__ __forceinline__ uint32_t vpAt(const Ctx &c, uint32_t s) { return c.vp[s * 64u + c.lane]; }
-__device__ __forceinline__ uint32_t opnd(const Ctx &c, uint32_t s) { return s < 16 ? s : vpAt(c, s); }
-__device__ __forceinline__ void vpPut(const Ctx &c, uint32_t s, uint32_t v) { c.vp[s * 64u + c.lane] = uint8_t(v); }
+// A wave-uniform value read from LDS, kept in a scalar register.
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
-// Middle-level operand: immediate below 32, else lane (o mod 32) of the
-// previous level, 0 past that level's size (SimEval.cu:372-404).
-__device__ __forceinline__ uint32_t midOp(const uint8_t *__restrict__ P, uint32_t o, uint32_t nPrev, uint32_t lane) {
-  return o < 32 ? o : ((o & 31u) < nPrev ? P[(o & 31u) * 64u + lane] : 0u);
+__device__ __forceinline__ uint32_t lutAt(const uint8_t *lut, uint32_t lt, uint32_t a, uint32_t b, uint32_t c) {
+  const uint32_t pos = lutIndex(lt, a, b, c);
+  const uint32_t v = lut[pos < kLutSize ? pos : kLutSize];  // the padding byte at kLutSize is 0
+  return lt == 0 ? c : v;
 }
 
-// One on-disk micropartition, lanes = instances.
-__device__ __forceinline__ void lutHalf(const Ctx &c, const uint32_t *S, const uint32_t *top, const uint32_t *ops,
-                                        uint32_t nTop, uint32_t nMid, uint32_t half, const uint32_t *last,
-                                        uint32_t nLast) {
-  uint8_t *__restrict__ L = c.lvl;
-  const uint8_t *__restrict__ vp = c.vp;
-  const uint8_t *__restrict__ lut = c.lut;
-  const uint32_t lane = c.lane;
-  auto topOp = [&](uint32_t i) -> uint32_t {
-    const uint32_t x = top[2 * i], y = top[2 * i + 1];
-    const uint32_t lt = x & 0xFFFFu, o0 = x >> 16, o1 = y & 0xFFFFu, o2 = y >> 16;
-    const uint32_t cc = o2 < 16 ? o2 : vp[o2 * 64u + lane];
-    const uint32_t aa = o0 < 16 ? o0 : vp[o0 * 64u + lane];
-    const uint32_t bb = o1 < 16 ? o1 : vp[o1 * 64u + lane];
-    const uint32_t pos = lutIndex(lt, aa, bb, cc);
-    const uint32_t lv = pos < kLutSize ? lut[pos] : 0u;
-    return lt == 0 ? cc : lv;
-  };
-  uint32_t i = 0;
-  for (; i + 4 <= nTop; i += 4) {
-    const uint32_t v0 = topOp(i), v1 = topOp(i + 1), v2 = topOp(i + 2), v3 = topOp(i + 3);
-    L[i * 64u + lane] = uint8_t(v0);
-    L[(i + 1) * 64u + lane] = uint8_t(v1);
-    L[(i + 2) * 64u + lane] = uint8_t(v2);
-    L[(i + 3) * 64u + lane] = uint8_t(v3);
-  }
-  for (; i < nTop; ++i) L[i * 64u + lane] = uint8_t(topOp(i));
-  uint32_t nPrev = nTop, cur = 0;
-  for (uint32_t m = 0; m < nMid; ++m) {
-    const uint32_t sz = (S[m >> 1] >> ((m & 1u) << 4)) & 0xFFFFu;
-    const uint32_t na = sz & 0xFFu, nb = sz >> 8;
-    const uint32_t n = half ? nb : na;
-    const uint32_t *w = ops + (half ? na : 0u);
-    const uint8_t *__restrict__ P = L + cur * 2048u;
-    uint8_t *__restrict__ Q = L + (cur ^ 1u) * 2048u;
-    auto midEval = [&](uint32_t op) -> uint32_t {
-      const uint32_t o2 = op & 63u, o1 = (op >> 6) & 63u, o0 = (op >> 12) & 63u, lt = op >> 18;
-      const uint32_t aa = midOp(P, o0, nPrev, lane), bb = midOp(P, o1, nPrev, lane), dd = midOp(P, o2, nPrev, lane);
-      const uint32_t pos = lutIndex(lt, aa, bb, dd);
-      const uint32_t lv = pos < kLutSize ? lut[pos] : 0u;
-      return lt == 0 ? dd : lv;
-    };
-    uint32_t k = 0;
-    for (; k + 4 <= n; k += 4) {
-      const uint32_t w0 = w[k], w1 = w[k + 1], w2 = w[k + 2], w3 = w[k + 3];
-      const uint32_t v0 = midEval(w0), v1 = midEval(w1), v2 = midEval(w2), v3 = midEval(w3);
-      Q[k * 64u + lane] = uint8_t(v0);
-      Q[(k + 1) * 64u + lane] = uint8_t(v1);
-      Q[(k + 2) * 64u + lane] = uint8_t(v2);
-      Q[(k + 3) * 64u + lane] = uint8_t(v3);
-    }
-    for (; k < n; ++k) Q[k * 64u + lane] = uint8_t(midEval(w[k]));
-    ops += na + nb;
-    nPrev = n;
-    cur ^= 1u;
-  }
-  const uint8_t *__restrict__ P = L + cur * 2048u;
-  uint8_t *__restrict__ vpw = c.vp;
-  for (uint32_t k = 0; k < nLast; ++k) {
-    const uint32_t w = last[k];
-    const uint32_t sid = (w >> 16) & 0xFFu;
-    const uint32_t v = sid < 32 ? sid : ((sid & 31u) < nPrev ? P[(sid & 31u) * 64u + lane] : 0u);
-    vpw[(w & 0xFFFFu) * 64u + lane] = uint8_t(v);
-  }
-}
-
-__device__ __forceinline__ void lutTask(const Ctx &c, const uint32_t *T) {
-  const uint32_t w0 = T[0], w1 = T[1];
+// A T_LUTROWS task: two on-disk micropartitions with the same number of
+// middle levels (SimEval.cu:319-448), every operand and destination already
+// resolved by the image builder to a row of the wave's LDS area (value-pool
+// rows, level buffers, constant rows), so an op costs its row reads, one LUT
+// lookup and one write. Two rounds of W ops are in flight per iteration, and
+// the program words of each level's first iteration are read while the
+// level before it is evaluated, so a level's dependent chain is its operand
+// reads and its LUT lookups.
+template <int I>
+__device__ __forceinline__ void lutTask(const Grp<I> &g, const uint32_t *T) {
+  constexpr uint32_t W = Grp<I>::W;
+  // counts are wave-uniform: scalar registers, scalar loop control
+  const uint32_t w0 = uni(T[0]), w1 = uni(T[1]);
   const uint32_t nMid = (w0 >> 8) & 0xFFFFu;
-  const uint32_t ntA = w1 & 0xFFu, ntB = (w1 >> 8) & 0xFFu, nlA = (w1 >> 16) & 0xFFu, nlB = w1 >> 24;
-  const uint32_t *S = T + 3;
-  const uint32_t *top = S + ((nMid + 1) >> 1);
-  const uint32_t *mids = top + 2 * (ntA + ntB);
-  const uint32_t *last = T + T[2];
-  lutHalf(c, S, top, mids, ntA, nMid, 0, last, nlA);
-  if (ntB | nlB) lutHalf(c, S, top + 2 * ntA, mids, ntB, nMid, 1, last + nlA, nlB);
-}
-
-// VecRead (SimEval.cu:449-481). Ops are evaluated one after another; the
-// loader rejects programs in which one op of an exec level reads a slot
-// another op of that level writes, so the order is unobservable.
-__device__ __forceinline__ void vecReadB(const Ctx &c, const uint32_t *T, const uint8_t *consts) {
-  const uint32_t n = T[0] >> 8;
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t *o = T + 1 + 5 * k;
-    const uint32_t vb = o[0] & 0xFFFFu, len = o[0] >> 16, offset = o[1] & 0xFFFFu, isConst = o[1] >> 16;
-    const uint32_t off = ((opnd(c, o[2] & 0xFFFFu) << 12) | (opnd(c, o[2] >> 16) << 8) |
-                          (opnd(c, o[3] & 0xFFFFu) << 4) | opnd(c, o[3] >> 16)) + offset;
-    const uint32_t v = off < len ? (isConst ? consts[vb + off] : vpAt(c, vb + off)) : opnd(c, o[4] & 0xFFFFu);
-    vpPut(c, o[4] >> 16, v);
+  const uint32_t nt = w1 & 0xFFFFu, nl = w1 >> 16;
+  const uint32_t *S = T + 4;
+  const uint2 *top = reinterpret_cast<const uint2 *>(S + (((nMid + 3) >> 2) + 1 & ~1u));
+  const uint2 *ops = reinterpret_cast<const uint2 *>(T + uni(T[2]));
+  const uint32_t *last = T + uni(T[3]);
+  const uint8_t *lut = g.lut;
+  uint8_t *vb = g.vp + g.ii;   // value-pool row 0 of this lane's instance
+  uint8_t *lb = g.lvl + g.ii;  // level row 0
+  auto sizeOf = [&](uint32_t m) -> uint32_t { return uni((S[m >> 2] >> ((m & 3u) * 8)) & 0xFFu); };
+  // top level (SimEval.cu:338-359): value-pool or constant rows in, a level
+  // row out. Both ops of an iteration read before either writes (ops of one
+  // level are independent), so their LDS round trips overlap.
+  auto topPair = [&](uint2 a, uint2 b) {
+    const uint32_t va = lutAt(lut, a.x & 0xFFFFu, vb[(a.x >> 16) * I], vb[(a.y & 0x1FFFu) * I],
+                              vb[((a.y >> 13) & 0x1FFFu) * I]);
+    const uint32_t vbb = lutAt(lut, b.x & 0xFFFFu, vb[(b.x >> 16) * I], vb[(b.y & 0x1FFFu) * I],
+                               vb[((b.y >> 13) & 0x1FFFu) * I]);
+    lb[(a.y >> 26) * I] = uint8_t(va);
+    lb[(b.y >> 26) * I] = uint8_t(vbb);
+  };
+  // middle level (SimEval.cu:362-421): level or constant rows in, an absolute row out
+  auto midPair = [&](uint2 a, uint2 b) {
+    const uint32_t va = lutAt(lut, a.x & 0x3FFFu, lb[((a.x >> 14) & 0xFFu) * I], lb[(a.x >> 22) * I],
+                              lb[(a.y & 0xFFu) * I]);
+    const uint32_t vbb = lutAt(lut, b.x & 0x3FFFu, lb[((b.x >> 14) & 0xFFu) * I], lb[(b.x >> 22) * I],
+                               lb[(b.y & 0xFFu) * I]);
+    vb[(a.y >> 8) * I] = uint8_t(va);
+    vb[(b.y >> 8) * I] = uint8_t(vbb);
+  };
+  const uint2 zero = make_uint2(0, 0);
+  uint2 c0 = zero, c1 = zero;  // the current level's first-iteration words
+  if (nt) {
+    c0 = top[g.opOf(0, nt)];
+    c1 = top[g.opOf(W, nt)];
+  }
+  uint32_t n = nMid ? sizeOf(0) : 0;
+  uint2 d0 = zero, d1 = zero;
+  if (n) {
+    d0 = ops[g.opOf(0, n)];
+    d1 = ops[g.opOf(W, n)];
+  }
+  if (nt) {
+    topPair(c0, c1);
+    for (uint32_t r = 2 * W; r < nt; r += 2 * W) topPair(top[g.opOf(r, nt)], top[g.opOf(r + W, nt)]);
+  }
+  for (uint32_t m = 0; m < nMid; ++m) {
+    c0 = d0;
+    c1 = d1;
+    const uint2 *cur = ops;
+    ops += n;
+    const uint32_t cn = n;
+    n = m + 1 < nMid ? sizeOf(m + 1) : 0;
+    if (n) {  // next level's first words, read before this level's chain
+      d0 = ops[g.opOf(0, n)];
+      d1 = ops[g.opOf(W, n)];
+    }
+    if (cn) {
+      midPair(c0, c1);
+      for (uint32_t r = 2 * W; r < cn; r += 2 * W) midPair(cur[g.opOf(r, cn)], cur[g.opOf(r + W, cn)]);
+    }
+  }
+  // last level (SimEval.cu:425-447): the copies not folded into the last
+  // middle level (fan-outs, immediates, a last level with no middle levels)
+  for (uint32_t r = 0; r < nl; r += W) {
+    const uint32_t w = last[g.opOf(r, nl)];
+    vb[(w & 0xFFFFu) * I] = lb[((w >> 16) & 0xFFu) * I];
   }
 }
 
-// VecArithOrLogic (SimEval.cu:482-548).
-__device__ __forceinline__ void vecOpB(const Ctx &c, const uint32_t *T, const uint8_t *consts) {
+// VecRead (SimEval.cu:449-481), W ops per round. The loader rejects programs
+// in which one op of an exec level reads a slot another op of that level
+// writes, so the order of ops is unobservable.
+template <int I>
+__device__ __forceinline__ void vecReadB(const Grp<I> &g, const uint32_t *T, const uint8_t *consts) {
   const uint32_t n = T[0] >> 8;
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t *o = T + 1 + 3 * k;
+  for (uint32_t r = 0; r < n; r += Grp<I>::W) {
+    const uint32_t *o = T + 1 + 5 * g.opOf(r, n);
+    const uint32_t vb = o[0] & 0xFFFFu, len = o[0] >> 16, offset = o[1] & 0xFFFFu, isConst = o[1] >> 16;
+    const uint32_t off = ((g.opnd(o[2] & 0xFFFFu) << 12) | (g.opnd(o[2] >> 16) << 8) |
+                          (g.opnd(o[3] & 0xFFFFu) << 4) | g.opnd(o[3] >> 16)) + offset;
+    const uint32_t v = off < len ? (isConst ? consts[vb + off] : g.at(vb + off)) : g.opnd(o[4] & 0xFFFFu);
+    g.put(o[4] >> 16, v);
+  }
+}
+
+// VecArithOrLogic (SimEval.cu:482-548), W ops per round.
+template <int I>
+__device__ __forceinline__ void vecOpB(const Grp<I> &g, const uint32_t *T, const uint8_t *consts) {
+  const uint32_t n = T[0] >> 8;
+  for (uint32_t r = 0; r < n; r += Grp<I>::W) {
+    const uint32_t *o = T + 1 + 3 * g.opOf(r, n);
     const uint32_t b1 = o[0] & 0xFFFFu, b2 = o[0] >> 16;
     const uint32_t len = o[1] & 0xFFFFu, name = (o[1] >> 16) & 0xFFu, cf = o[1] >> 24, res = o[2];
     unsigned __int128 v1 = 0, v2 = 0;
     for (uint32_t j = 0; j < len; ++j) {
-      unsigned __int128 s1 = (cf & 2) ? consts[b1 + j] : vpAt(c, b1 + j);
-      unsigned __int128 s2 = (cf & 1) ? consts[b2 + j] : vpAt(c, b2 + j);
+      unsigned __int128 s1 = (cf & 2) ? consts[b1 + j] : g.at(b1 + j);
+      unsigned __int128 s2 = (cf & 1) ? consts[b2 + j] : g.at(b2 + j);
       v1 |= s1 << (4 * j);
       v2 |= s2 << (4 * j);
     }
-    if (len > 0 && len < 32) {
+    if (len > 0 && len < 32) {  // sign_extend_128 (SimEval.cu:222-228)
       const uint32_t bit = 4 * len - 1;
       const unsigned __int128 ext = ~((((unsigned __int128)1) << (bit + 1)) - 1);
       if ((v1 >> bit) & 1) v1 |= ext;
       if ((v2 >> bit) & 1) v2 |= ext;
     }
     if (name <= 2) {
-      unsigned __int128 r = name == 0 ? v1 + v2 : name == 1 ? v1 - v2 : v1 * v2;
+      unsigned __int128 rr = name == 0 ? v1 + v2 : name == 1 ? v1 - v2 : v1 * v2;
       for (uint32_t j = 0; j < len; ++j) {
-        vpPut(c, uint16_t(res + j), uint32_t(r & 15u));
-        r >>= 4;
+        g.put(uint16_t(res + j), uint32_t(rr & 15u));
+        rr >>= 4;
       }
     } else {
       const __int128 s1 = (__int128)v1, s2 = (__int128)v2;
-      vpPut(c, res, name == 3 ? (s1 == s2) : name == 4 ? (s1 < s2) : name == 5 ? (s1 <= s2) : 0);
+      g.put(res, name == 3 ? (s1 == s2) : name == 4 ? (s1 < s2) : name == 5 ? (s1 <= s2) : 0);
     }
   }
 }
 
-// MemRead (SimEval.cu:550-575).
-__device__ __forceinline__ void memReadB(const Ctx &c, const uint32_t *T) {
+// MemRead (SimEval.cu:550-575), W ops per round.
+template <int I>
+__device__ __forceinline__ void memReadB(const Grp<I> &g, const uint32_t *T) {
   const uint32_t n = T[0] >> 8;
-  for (uint32_t k = 0; k < n; ++k) {
-    const uint32_t *o = T + 1 + 4 * k;
-    const uint32_t en = opnd(c, o[2] & 0xFFFFu);
+  for (uint32_t r = 0; r < n; r += Grp<I>::W) {
+    const uint32_t *o = T + 1 + 4 * g.opOf(r, n);
+    const uint32_t en = g.opnd(o[2] & 0xFFFFu);
     if (en == 0) continue;
     uint32_t addr = 0;
 #pragma unroll
-    for (uint32_t j = 0; j < 8; ++j) addr |= vpAt(c, (o[2] >> 16) + j) << (4 * j);
+    for (uint32_t j = 0; j < 8; ++j) addr |= g.at((o[2] >> 16) + j) << (4 * j);
     if (o[3] >> 16) addr <<= 2;
     const uint64_t at = (uint64_t(o[0]) | uint64_t(o[1]) << 32) + addr;
-    vpPut(c, o[3] & 0xFFFFu, at < c.memSize ? c.mem[at * c.instPad + c.slot0 + c.lane] : 0u);
+    g.put(o[3] & 0xFFFFu, at < g.memSize ? g.mem[at * g.instPad + g.slot0 + g.ii] : 0u);
   }
 }
 
-// L0 reads of one group: op k copies pool[src_k][slot0 .. slot0+64) into
-// vp[slot_k][0 .. 64); lane l moves 16 bytes of op (k0 + l/4).
+// A lane's share of one row of a group: 16 bytes (I / 16 lanes per row) for
+// I >= 16, the whole 8-byte row for I = 8.
+template <int I>
+struct RowPart {
+  static constexpr uint32_t B = I >= 16 ? 16u : uint32_t(I);  // bytes per lane
+  static constexpr uint32_t LPO = I / B, OPR = 64 / LPO;       // lanes per row, rows per instruction
+  using T = typename std::conditional<(I >= 16), uint4, uint2>::type;
+};
+
+// L0 reads of one group: op k copies pool[src_k][slot0 .. slot0 + I) into
+// vp[slot_k][0 .. I), RowPart<I>::B bytes per lane.
+template <int I>
 __device__ __forceinline__ void wideRead(uint8_t *__restrict__ vp, const uint8_t *__restrict__ pool, size_t instPad,
                                          uint32_t slot0, const uint32_t *__restrict__ L, uint32_t n, uint32_t lane) {
-  const uint32_t q = lane >> 2, j = (lane & 3u) * 16u;
+  using R = RowPart<I>;
+  using V = typename R::T;
+  constexpr uint32_t opr = R::OPR;
+  const uint32_t q = lane / R::LPO, j = (lane % R::LPO) * R::B;
   const uint8_t *base = pool + slot0 + j;
   uint32_t k = q;
-  for (; k + 48 < n; k += 64) {
-    const uint32_t s0 = L[2 * k], d0 = L[2 * k + 1], s1 = L[2 * k + 32], d1 = L[2 * k + 33];
-    const uint32_t s2 = L[2 * k + 64], d2 = L[2 * k + 65], s3 = L[2 * k + 96], d3 = L[2 * k + 97];
-    const uint4 v0 = *reinterpret_cast<const uint4 *>(base + size_t(s0) * instPad);
-    const uint4 v1 = *reinterpret_cast<const uint4 *>(base + size_t(s1) * instPad);
-    const uint4 v2 = *reinterpret_cast<const uint4 *>(base + size_t(s2) * instPad);
-    const uint4 v3 = *reinterpret_cast<const uint4 *>(base + size_t(s3) * instPad);
-    *reinterpret_cast<uint4 *>(vp + d0 * 64u + j) = v0;
-    *reinterpret_cast<uint4 *>(vp + d1 * 64u + j) = v1;
-    *reinterpret_cast<uint4 *>(vp + d2 * 64u + j) = v2;
-    *reinterpret_cast<uint4 *>(vp + d3 * 64u + j) = v3;
+  for (; k + 3 * opr < n; k += 4 * opr) {
+    const uint32_t s0 = L[2 * k], d0 = L[2 * k + 1], s1 = L[2 * (k + opr)], d1 = L[2 * (k + opr) + 1];
+    const uint32_t s2 = L[2 * (k + 2 * opr)], d2 = L[2 * (k + 2 * opr) + 1];
+    const uint32_t s3 = L[2 * (k + 3 * opr)], d3 = L[2 * (k + 3 * opr) + 1];
+    const V v0 = *reinterpret_cast<const V *>(base + size_t(s0) * instPad);
+    const V v1 = *reinterpret_cast<const V *>(base + size_t(s1) * instPad);
+    const V v2 = *reinterpret_cast<const V *>(base + size_t(s2) * instPad);
+    const V v3 = *reinterpret_cast<const V *>(base + size_t(s3) * instPad);
+    *reinterpret_cast<V *>(vp + d0 * uint32_t(I) + j) = v0;
+    *reinterpret_cast<V *>(vp + d1 * uint32_t(I) + j) = v1;
+    *reinterpret_cast<V *>(vp + d2 * uint32_t(I) + j) = v2;
+    *reinterpret_cast<V *>(vp + d3 * uint32_t(I) + j) = v3;
   }
-  for (; k < n; k += 16) {
+  for (; k < n; k += opr) {
     const uint32_t s0 = L[2 * k], d0 = L[2 * k + 1];
-    *reinterpret_cast<uint4 *>(vp + d0 * 64u + j) = *reinterpret_cast<const uint4 *>(base + size_t(s0) * instPad);
+    *reinterpret_cast<V *>(vp + d0 * uint32_t(I) + j) = *reinterpret_cast<const V *>(base + size_t(s0) * instPad);
   }
 }
 
 // Word copy of the last level (SimEval.cu:658-665,751-758) for one group:
-// rows [0, n) of vp from `dat` to `pool` from `row0`, live lanes only
-// (the live lanes of a group are a prefix of length `live`).
+// rows [0, n) of vp from `dat` to `pool` from `row0`, live instances only
+// (the live instances of a group are a prefix of length `live`).
+template <int I>
 __device__ __forceinline__ void wideWrite(uint8_t *pool, size_t instPad, uint32_t slot0, uint32_t row0,
                                           const uint8_t *vp, uint32_t dat, uint32_t n, uint32_t lane, uint32_t live) {
-  const uint32_t q = lane >> 2, j = (lane & 3u) * 16u;
-  for (uint32_t k = q; k < n; k += 16) {
-    const uint4 v = *reinterpret_cast<const uint4 *>(vp + (dat + k) * 64u + j);
+  using R = RowPart<I>;
+  using V = typename R::T;
+  const uint32_t q = lane / R::LPO, j = (lane % R::LPO) * R::B;
+  for (uint32_t k = q; k < n; k += R::OPR) {
+    const uint8_t *src = vp + (dat + k) * uint32_t(I) + j;
     uint8_t *dst = pool + size_t(row0 + k) * instPad + slot0 + j;
-    if (j + 16 <= live) {
-      *reinterpret_cast<uint4 *>(dst) = v;
+    if (j + R::B <= live) {
+      *reinterpret_cast<V *>(dst) = *reinterpret_cast<const V *>(src);
     } else if (j < live) {
-      for (uint32_t t = 0; t < live - j; ++t) {
-        const uint32_t w = t < 4 ? v.x : t < 8 ? v.y : t < 12 ? v.z : v.w;
-        dst[t] = uint8_t(w >> ((t & 3u) * 8u));
-      }
+      for (uint32_t t = 0; t < live - j; ++t) dst[t] = src[t];
     }
   }
 }
 
 // One (partition, group) evaluated by one wave, no workgroup barriers.
-__device__ __forceinline__ void waveItem(const BatchArgs &a, uint32_t part, const uint32_t *P, uint32_t g, uint8_t *vp,
+template <int I>
+__device__ __forceinline__ void waveItem(const BatchArgs &a, uint32_t part, const uint32_t *P, uint32_t grp, uint8_t *vp,
                                          uint8_t *lvl, const uint8_t *lut, const uint64_t *draws, uint32_t c) {
+  constexpr uint32_t W = Grp<I>::W;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t slot0 = g * 64u, inst = slot0 + lane;
+  const uint32_t ii = lane % I, sub = lane / I;
+  const uint32_t slot0 = grp * I, inst = slot0 + ii;
   const bool liveLane = c < a.runCycles[inst];
-  const uint32_t live = __popcll(__ballot(liveLane));
+  // live instances of the group (a prefix): count the lanes of op sub-lane 0
+  const uint32_t live = __popcll(__ballot(liveLane && sub == 0));
   const uint64_t cycle = a.cycleBase[inst] + c;
   const size_t IP = a.instPad;
-  Ctx cx{a.mem, a.memSize, IP, vp, lvl, lut, lane, slot0};
+  Grp<I> g{vp, lvl, lut, a.mem, a.memSize, IP, ii, sub, slot0};
   {
     const uint32_t nrr = P[H_NRR], ner = P[H_NER], nsr = P[H_NSR];
     const uint32_t *L = P + P[H_OFF_L0];
-    wideRead(vp, a.reg, IP, slot0, L, nrr, lane);
+    wideRead<I>(vp, a.reg, IP, slot0, L, nrr, lane);
     L += 2 * nrr;
-    wideRead(vp, a.exg, IP, slot0, L, ner, lane);
+    wideRead<I>(vp, a.exg, IP, slot0, L, ner, lane);
     L += 2 * ner;
-    for (uint32_t k = 0; k < nsr; ++k) {
+    for (uint32_t k = sub; k < nsr; k += W) {
       const uint32_t in = L[2 * k];
-      vpPut(cx, L[2 * k + 1], uint32_t((draws[size_t(in >> 6) * IP + inst] >> (in & 63)) & 1u));
+      g.put(L[2 * k + 1], uint32_t((draws[size_t(in >> 6) * IP + inst] >> (in & 63)) & 1u));
     }
   }
   const uint8_t *consts = a.consts + P[H_CONST_BASE];
@@ -268,45 +321,47 @@ __device__ __forceinline__ void waveItem(const BatchArgs &a, uint32_t part, cons
     for (uint32_t t = lev[2 * e]; t < lev[2 * e + 1]; ++t) {
       const uint32_t *T = P + tab[t];
       const uint32_t kind = T[0] & 0xFFu;
-      if (kind == T_LUT2) lutTask(cx, T);
-      else if (kind == T_VECREAD) vecReadB(cx, T, consts);
-      else if (kind == T_VECOP) vecOpB(cx, T, consts);
-      else memReadB(cx, T);
+      if (kind == T_LUTROWS) lutTask<I>(g, T);
+      else if (kind == T_VECREAD) vecReadB<I>(g, T, consts);
+      else if (kind == T_VECOP) vecOpB<I>(g, T, consts);
+      else memReadB<I>(g, T);
     }
   }
-  // signal probes of the live lanes (slot-major, [probe][instPad])
+  // signal probes of the live instances (slot-major, [probe][instPad])
   if (a.probeOff != nullptr && liveLane)
-    for (uint32_t k = a.probeOff[part]; k < a.probeOff[part + 1]; ++k)
-      a.probeVal[size_t(k) * IP + inst] = uint8_t(vpAt(cx, a.probeSlot[k]));
-  // last level (SimEval.cu:625-761); lanes past their cycle count write nothing
+    for (uint32_t k = a.probeOff[part] + sub; k < a.probeOff[part + 1]; k += W)
+      a.probeVal[size_t(k) * IP + inst] = uint8_t(g.at(a.probeSlot[k]));
+  // last level (SimEval.cu:625-761); instances past their cycle count write nothing
   const uint32_t ewc = P[H_EW_DC] >> 16;
   if (ewc) {
-    wideWrite(a.exg, IP, slot0, (P[H_EW_EXG] >> 2) * 4, vp, ((P[H_EW_DC] & 0xFFFFu) >> 2) * 4, (ewc + 3) / 4 * 4, lane,
-              live);
+    wideWrite<I>(a.exg, IP, slot0, (P[H_EW_EXG] >> 2) * 4, vp, ((P[H_EW_DC] & 0xFFFFu) >> 2) * 4, (ewc + 3) / 4 * 4,
+                 lane, live);
     return;
   }
   const uint32_t rwc = P[H_RW_DC] >> 16;
   if (rwc)
-    wideWrite(a.reg, IP, slot0, (P[H_RW_REG] >> 2) * 4, vp, ((P[H_RW_DC] & 0xFFFFu) >> 2) * 4, (rwc + 3) / 4 * 4, lane,
-              live);
+    wideWrite<I>(a.reg, IP, slot0, (P[H_RW_REG] >> 2) * 4, vp, ((P[H_RW_DC] & 0xFFFFu) >> 2) * 4, (rwc + 3) / 4 * 4,
+                 lane, live);
   const uint32_t nmw = P[H_NMW], npr = P[H_NPR], nst = P[H_NST];
   const uint32_t *L = P + P[H_OFF_LAST];
-  if (liveLane)
+  // memory writes one op at a time in program order (a later write to the
+  // same address wins, as in the oracle)
+  if (liveLane && sub == 0)
     for (uint32_t k = 0; k < nmw; ++k) {
       const uint32_t *o = L + 4 * k;
-      if (opnd(cx, o[3] & 0xFFFFu) == 0) continue;
-      const uint32_t dat = opnd(cx, o[2] >> 16);
+      if (g.opnd(o[3] & 0xFFFFu) == 0) continue;
+      const uint32_t dat = g.opnd(o[2] >> 16);
       uint32_t addr = 0;
 #pragma unroll
-      for (uint32_t j = 0; j < 8; ++j) addr |= vpAt(cx, (o[2] & 0xFFFFu) + j) << (4 * j);
+      for (uint32_t j = 0; j < 8; ++j) addr |= g.at((o[2] & 0xFFFFu) + j) << (4 * j);
       if (o[3] >> 16) addr <<= 2;
       const uint64_t at = (uint64_t(o[0]) | uint64_t(o[1]) << 32) + addr;
       if (at < a.memSize) a.mem[at * IP + inst] = uint8_t(dat);
     }
   L += 4 * nmw;
   if (liveLane && a.printEnable)
-    for (uint32_t k = 0; k < npr; ++k)
-      if (opnd(cx, L[k] & 0xFFFFu) != 0) {
+    for (uint32_t k = sub; k < npr; k += W)
+      if (g.opnd(L[k] & 0xFFFFu) != 0) {
         const uint64_t q = atomicAdd(reinterpret_cast<unsigned long long *>(a.logCount), 1ull);
         if (q < a.logCap) {
           a.logCycle[q] = cycle;
@@ -315,8 +370,8 @@ __device__ __forceinline__ void waveItem(const BatchArgs &a, uint32_t part, cons
       }
   L += npr;
   if (liveLane)
-    for (uint32_t k = 0; k < nst; ++k)
-      if (opnd(cx, L[k] & 0xFFFFu) != 0) {
+    for (uint32_t k = sub; k < nst; k += W)
+      if (g.opnd(L[k] & 0xFFFFu) != 0) {
         atomicOr(reinterpret_cast<unsigned long long *>(&a.stop[inst]), 1ull);
         atomicMin(reinterpret_cast<unsigned long long *>(&a.stopCycle[inst]), (unsigned long long)cycle);
       }
@@ -324,11 +379,11 @@ __device__ __forceinline__ void waveItem(const BatchArgs &a, uint32_t part, cons
 
 // Advance the stimulus of every live instance of the active groups and store
 // the draws of launch cycle `c` in buffer `buf`.
-__device__ __forceinline__ void stimulusFor(const BatchArgs &a, uint32_t buf, uint32_t c, uint32_t activeGroups) {
+__device__ __forceinline__ void stimulusFor(const BatchArgs &a, uint32_t buf, uint32_t c, uint32_t activeSlots) {
   if (a.nIn == 0) return;
   const uint32_t nDraws = (a.nIn + 63) / 64;
   uint64_t *draws = a.draws + size_t(buf) * nDraws * a.instPad;
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < activeGroups * 64u; s += gridDim.x * blockDim.x) {
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < activeSlots; s += gridDim.x * blockDim.x) {
     if (c >= a.runCycles[s]) continue;
     uint64_t x = a.xs[s];
     for (uint32_t d = 0; d < nDraws; ++d) {
@@ -339,81 +394,130 @@ __device__ __forceinline__ void stimulusFor(const BatchArgs &a, uint32_t buf, ui
   }
 }
 
-// Copy a partition's program words (16-byte aligned, padded to 4 words) into LDS.
-__device__ __forceinline__ void stageProgram(uint32_t *dst, const uint32_t *src, uint32_t words) {
+// Start copying a partition's program words (16-byte aligned, padded to 4
+// words) into LDS with global-to-LDS loads: each wave instruction moves 64
+// consecutive 16-byte chunks (lane k's chunk lands at the wave's LDS base +
+// 16 k), so nothing passes through registers and the copy completes while
+// the workgroup works on something else; s_waitcnt vmcnt(0) and a
+// workgroup barrier precede any read of it.
+__device__ __forceinline__ void stageProgramAsync(uint32_t *dst, const uint32_t *src, uint32_t words) {
   const uint32_t n4 = (words + 3) >> 2;
-  for (uint32_t k = threadIdx.x; k < n4; k += blockDim.x)
-    reinterpret_cast<uint4 *>(dst)[k] = reinterpret_cast<const uint4 *>(src)[k];
+  const uint32_t wave = threadIdx.x & ~63u, lane = threadIdx.x & 63u;
+  for (uint32_t k0 = wave; k0 < n4; k0 += blockDim.x) {
+    if (k0 + lane < n4)
+      __builtin_amdgcn_global_load_lds(reinterpret_cast<const uint4 *>(src) + k0 + lane,
+                                       (__attribute__((address_space(3))) void *)(reinterpret_cast<uint4 *>(dst) + k0),
+                                       16, 0, 0);
+  }
 }
 
-// Work items of one region: (partition, W consecutive groups), one group per wave.
+// Work items of one region: (partition, NW consecutive groups), one group per wave.
+template <int I>
 __device__ __forceinline__ void batchRegion(const BatchArgs &a, uint32_t pb, uint32_t pe, uint32_t active,
-                                            uint8_t *vpAll, uint8_t *lvlAll, uint32_t *prog, const uint8_t *lut,
-                                            const uint64_t *draws, uint32_t c) {
-  const uint32_t np = pe - pb, W = blockDim.x >> 6, wave = threadIdx.x >> 6;
-  const uint32_t nChunks = (active + W - 1) / W;
+                                            uint8_t *vpAll, uint32_t *prog, const uint8_t *lut, const uint64_t *draws,
+                                            uint32_t c) {
+  const uint32_t np = pe - pb, NW = blockDim.x >> 6, wave = threadIdx.x >> 6;
+  const uint32_t nChunks = (active + NW - 1) / NW;
   const uint32_t items = np * nChunks;
-  uint8_t *vp = vpAll + wave * a.vpBytes * 64u;
-  uint8_t *lvl = lvlAll + wave * 4096u;
-  for (uint32_t k = blockIdx.x; k < items; k += gridDim.x) {
-    const uint32_t p = pb + k / nChunks, g = (k % nChunks) * W + wave;
-    stageProgram(prog, a.words + a.partOff[p], a.partWords[p]);
+  uint8_t *vp = vpAll + wave * (a.vpBytes + kLevelRows) * uint32_t(I);
+  uint8_t *lvl = vp + a.vpBytes * uint32_t(I);
+  // two program buffers: the next item's program streams into one while
+  // the current item runs from the other
+  uint32_t k = blockIdx.x;
+  if (k < items) {
+    const uint32_t p = pb + k / nChunks;
+    stageProgramAsync(prog, a.words + a.partOff[p], a.partWords[p]);
+  }
+  for (uint32_t buf = 0; k < items; k += gridDim.x, buf ^= 1u) {
+    const uint32_t p = pb + k / nChunks, grp = (k % nChunks) * NW + wave;
+    uint32_t *cur = prog + buf * a.progWords;
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's share of the copy has landed
     __syncthreads();
-    if (g < active) waveItem(a, p, prog, g, vp, lvl, lut, draws, c);
-    __syncthreads();
+    const uint32_t kn = k + gridDim.x;
+    if (kn < items) {
+      const uint32_t pn = pb + kn / nChunks;
+      stageProgramAsync(prog + (buf ^ 1u) * a.progWords, a.words + a.partOff[pn], a.partWords[pn]);
+    }
+    if (grp < active) waveItem<I>(a, p, cur, grp, vp, lvl, lut, draws, c);
+    __syncthreads();  // every wave is done with `cur` before it is overwritten
   }
 }
 
 }  // namespace
 
-__global__ __launch_bounds__(1024) void batchCycles(BatchArgs a) {
+// groupsActive[c] counts groups of I instances.
+template <int I>
+__global__ __launch_bounds__(512) void batchCycles(BatchArgs a) {
   extern __shared__ __align__(16) uint8_t bLds[];
-  const uint32_t W = blockDim.x >> 6;
+  const uint32_t NW = blockDim.x >> 6;
   uint8_t *lut = bLds;
-  uint8_t *vp = bLds + kLutBytesB;
-  uint8_t *lvl = vp + W * a.vpBytes * 64u;
-  uint32_t *prog = reinterpret_cast<uint32_t *>(lvl + W * 4096u);
+  uint8_t *vp = bLds + kLutBytesB;  // per wave: value-pool rows, then level rows
+  uint32_t *prog = reinterpret_cast<uint32_t *>(vp + NW * (a.vpBytes + kLevelRows) * uint32_t(I));
   for (uint32_t k = threadIdx.x; k < kLutBytesB / 4; k += blockDim.x)
     reinterpret_cast<uint32_t *>(lut)[k] = reinterpret_cast<const uint32_t *>(a.lut)[k];
+  {  // constant rows: level row kConstRow + v holds v for every instance
+    uint8_t *cr = vp + (threadIdx.x >> 6) * (a.vpBytes + kLevelRows) * uint32_t(I) + (a.vpBytes + kConstRow) * I;
+    for (uint32_t k = threadIdx.x & 63u; k < (kLevelRows - kConstRow) * I; k += 64) cr[k] = uint8_t(k / I);
+  }
   const uint32_t nDraws = (a.nIn + 63) / 64;
   uint32_t bar = 0;
   // draws of the first cycle, before anyone reads them
-  stimulusFor(a, 0, 0, a.groupsActive[0]);
+  stimulusFor(a, 0, 0, a.groupsActive[0] * uint32_t(I));
   bar++;
   gridBarrier(a.barrier, bar * gridDim.x, false);
   for (uint32_t c = 0; c < a.nCycles; ++c) {
     const uint32_t active = a.groupsActive[c];
     if (active == 0) break;
+    const uint32_t activeSlots = active * uint32_t(I);
     const uint64_t *draws = a.draws + size_t(c & 1u) * nDraws * a.instPad;
     // input registers of live instances (never read back through normal register reads)
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < active * 64u; s += gridDim.x * blockDim.x)
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < activeSlots; s += gridDim.x * blockDim.x)
       if (c < a.runCycles[s])
         for (uint32_t i = 0; i < a.nIn; ++i)
           a.reg[size_t(a.inIds[i]) * a.instPad + s] = uint8_t((draws[size_t(i >> 6) * a.instPad + s] >> (i & 63)) & 1u);
-    batchRegion(a, 0, a.n0, active, vp, lvl, prog, lut, draws, c);
+    batchRegion<I>(a, 0, a.n0, active, vp, prog, lut, draws, c);
     bar++;
     gridBarrier(a.barrier, bar * gridDim.x, false);
-    if (c + 1 < a.nCycles) stimulusFor(a, (c + 1) & 1u, c + 1, a.groupsActive[c + 1]);
-    batchRegion(a, a.n0, a.nTotal, active, vp, lvl, prog, lut, draws, c);
+    if (c + 1 < a.nCycles) stimulusFor(a, (c + 1) & 1u, c + 1, a.groupsActive[c + 1] * uint32_t(I));
+    batchRegion<I>(a, a.n0, a.nTotal, active, vp, prog, lut, draws, c);
     bar++;
     gridBarrier(a.barrier, bar * gridDim.x, false);
   }
 }
 
-hipError_t launchBatch(const BatchArgs &args, uint32_t grid, uint32_t block, uint32_t lds, hipStream_t stream) {
-  BatchArgs a = args;
-  void *params[] = {&a};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void *>(batchCycles), dim3(grid), dim3(block), params, lds,
-                                    stream);
+namespace {
+const void *batchFn(uint32_t inst) {
+  return inst == 8    ? reinterpret_cast<const void *>(batchCycles<8>)
+         : inst == 16 ? reinterpret_cast<const void *>(batchCycles<16>)
+         : inst == 32 ? reinterpret_cast<const void *>(batchCycles<32>)
+                      : reinterpret_cast<const void *>(batchCycles<64>);
+}
+}  // namespace
+
+bool batchSupportsGroup(uint32_t inst) { return inst == 8 || inst == 16 || inst == 32 || inst == 64; }
+
+// LUT, per wave value-pool and level rows, two program buffers.
+uint32_t batchLdsBytes(uint32_t inst, uint32_t waves, uint32_t vpBytes, uint32_t progWords) {
+  return kLutBytesB + waves * (vpBytes + kLevelRows) * inst + 2u * progWords * 4u;
 }
 
-hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t block, uint32_t lds) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, reinterpret_cast<const void *>(batchCycles), block, lds);
+hipError_t launchBatch(const BatchArgs &args, uint32_t inst, uint32_t grid, uint32_t block, uint32_t lds,
+                       hipStream_t stream) {
+  BatchArgs a = args;
+  void *params[] = {&a};
+  return hipLaunchCooperativeKernel(batchFn(inst), dim3(grid), dim3(block), params, lds, stream);
+}
+
+hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t inst, uint32_t block, uint32_t lds) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, batchFn(inst), block, lds);
 }
 
 hipError_t batchSetLds(uint32_t lds) {
-  return hipFuncSetAttribute(reinterpret_cast<const void *>(batchCycles), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             lds);
+  for (uint32_t inst : {8u, 16u, 32u, 64u}) {
+    hipError_t e = hipFuncSetAttribute(batchFn(inst), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 // Reorder instance slots of an instance-minor pool: out[r][s] = in[r][src[s]].

@@ -174,7 +174,10 @@ struct tgs_ctx {
   // groups keeps twice the waves busy of a four-group workgroup (measured on
   // a 200K-LUT4 design: 584 against 1,057 us per launch-cycle with 64
   // instances, equal from 256 instances up).
-  uint32_t batchBlock = 128, batchBlockEff = 128, batchGrid = 0, batchLds = 0, batchGridOverride = 0;
+  // Batched launch shape: instances per wave (16, 32 or 64; 0 = chosen at
+  // image build), waves per workgroup (0 = chosen) and the effective values.
+  uint32_t batchInst = 0, batchWaves = 0, batchInstEff = 64, batchBlockEff = 128;
+  uint32_t batchGrid = 0, batchLds = 0, batchGridOverride = 0, batchProgWords = 0;
   bool batchReady = false;
 
   DBuf lut, inIds;
@@ -865,10 +868,23 @@ int downloadAll(tgs_ctx *c, std::vector<HostInst> &h) { return c->batched ? batc
 int uploadAll(tgs_ctx *c, const std::vector<HostInst> &h) { return c->batched ? batchUpload(c, h) : instUpload(c, h); }
 uint32_t instanceCount(const tgs_ctx *c) { return c->batched ? c->bs.n : uint32_t(c->inst.size()); }
 
+// Build and upload the batched kernel's program image and choose its launch
+// shape: instances per wave I (a wave evaluates 64 / I ops per instruction
+// round for I instances) and waves per workgroup. Each wave holds its group's
+// value pool and level buffers in LDS ([slot][I], [2][2][32][I]); the LUT and
+// the staged partition program are shared by the workgroup. Among the shapes
+// that fit, the one with the most co-resident waves per CU (up to 16) runs,
+// then the one with the smallest I (shortest (partition, group) items).
 int ensureBatchImage(tgs_ctx *c) {
   if (c->batchReady) return TGS_OK;
-  const uint32_t nWaves = c->batchBlock / 64;
-  c->w64 = buildWave64Image(c->design, c->stim, nWaves);
+  const uint32_t imageWaves = c->batchWaves ? c->batchWaves : 4;
+  uint32_t maxVp = 16;
+  for (const auto &p : c->design.parts) maxVp = std::max(maxVp, p.valuePoolSize);
+  const uint32_t vpRows = up16(maxVp);
+  if (vpRows + kLevelRows > 8192)
+    return c->fail(TGS_E_LIMIT, "batched execution addresses value pools of at most %u slots (%u here)",
+                   8192 - kLevelRows, maxVp);
+  c->w64 = buildWave64Image(c->design, c->stim, imageWaves, vpRows);
   c->w64Ready = false;  // the wave64 schedule must be rebuilt for this image
   auto &im = c->w64;
   int rc;
@@ -879,23 +895,48 @@ int ensureBatchImage(tgs_ctx *c) {
   const uint32_t vpBytes = up16(std::max<uint32_t>(im.maxVp, 16));
   uint32_t maxPart = 0;
   for (auto w : im.partWords) maxPart = std::max(maxPart, (w + 3) & ~3u);
-  // Each wave holds one 64-instance group's value pool and level buffer; the
-  // LUT and the staged partition program are shared by the workgroup.
-  const uint32_t perWave = vpBytes * 64 + 4096, shared = wave64LutBytes() + maxPart * 4;
   const uint32_t ldsMax = uint32_t(c->prop.sharedMemPerBlock) - kLdsGranuleMargin;
-  if (shared + perWave > ldsMax)
+  const char *envInst = std::getenv("TGS_BATCH_INST");
+  const uint32_t forcedInst = c->batchInst ? c->batchInst : envInst ? uint32_t(std::atoi(envInst)) : 0u;
+  std::vector<uint32_t> insts, waveList;
+  for (uint32_t i : {8u, 16u, 32u, 64u})
+    if (!forcedInst || forcedInst == i) insts.push_back(i);
+  if (insts.empty()) return c->fail(TGS_E_ARG, "instances per wave must be 8, 16, 32 or 64 (%u given)", forcedInst);
+  if (c->batchWaves) waveList.push_back(c->batchWaves);
+  else waveList = {2u, 4u, 8u};
+  bool have = false;
+  uint32_t bestScore = 0, bestInst = 0, bestWaves = 0, bestLds = 0;
+  int bestPerCU = 0;
+  for (uint32_t inst : insts)
+    for (uint32_t waves : waveList) {
+      const uint32_t lds = batchLdsBytes(inst, waves, vpBytes, maxPart);
+      if (lds > ldsMax) continue;
+      HIPCHK(c, batchSetLds(lds));
+      int perCU = 0;
+      HIPCHK(c, batchMaxBlocksPerCU(&perCU, inst, waves * 64, lds));
+      perCU = residentPerCU(c, perCU, lds);
+      if (perCU < 1) continue;
+      const uint32_t score = std::min<uint32_t>(16, uint32_t(perCU) * waves);
+      if (!have || score > bestScore || (score == bestScore && inst < bestInst)) {
+        have = true;
+        bestScore = score;
+        bestInst = inst;
+        bestWaves = waves;
+        bestLds = lds;
+        bestPerCU = perCU;
+      }
+    }
+  if (!have)
     return c->fail(TGS_E_LIMIT,
-                   "batched execution needs %u B of LDS (value pool %u slots x 64 instances); device allows %u B",
-                   shared + perWave, im.maxVp, ldsMax);
-  const uint32_t waves = std::min(nWaves, (ldsMax - shared) / perWave);
-  c->batchBlockEff = waves * 64;
-  c->batchLds = shared + waves * perWave;
+                   "batched execution needs %u B of LDS (value pool %u slots x %u instances); device allows %u B",
+                   batchLdsBytes(insts[0], 1, vpBytes, maxPart), im.maxVp, insts[0], ldsMax);
+  c->batchInstEff = bestInst;
+  c->batchProgWords = maxPart;
+  c->batchBlockEff = bestWaves * 64;
+  c->batchLds = bestLds;
   HIPCHK(c, batchSetLds(c->batchLds));
-  int perCU = 0;
-  HIPCHK(c, batchMaxBlocksPerCU(&perCU, c->batchBlockEff, c->batchLds));
-  perCU = residentPerCU(c, perCU, c->batchLds);
-  if (perCU < 1) return c->fail(TGS_E_LIMIT, "batched kernel cannot be resident with %u B LDS", c->batchLds);
-  perCU = std::min<int>(perCU, std::max<uint32_t>(1, 32 / waves));
+  // at most 16 waves per CU: more only lengthens the grid barrier
+  const int perCU = std::min<int>(bestPerCU, std::max<uint32_t>(1, 16 / bestWaves));
   c->batchGrid = uint32_t(perCU) * c->prop.multiProcessorCount;
   if (c->batchGridOverride) c->batchGrid = std::min(c->batchGrid, c->batchGridOverride);
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -986,9 +1027,10 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles, const BatchScratch *fixed = n
     const uint32_t L = uint32_t(std::min<uint64_t>(maxC, perLaunch));
     for (uint32_t s = 0; s < P; ++s) run[s] = uint32_t(std::min<uint64_t>(remaining[s], L));
     active.assign(L, 0);
+    const uint32_t I = c->batchInstEff;
     for (uint32_t cy = 0; cy < L; ++cy) {
       uint32_t g = 0;
-      while (g < P / 64 && run[g * 64] > cy) ++g;  // slots sorted by decreasing cycles
+      while (g < P / I && run[g * I] > cy) ++g;  // slots sorted by decreasing cycles
       active[cy] = g;
     }
     BatchScratch sc;
@@ -1022,6 +1064,7 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles, const BatchScratch *fixed = n
     a.n0 = c->w64.n0;
     a.nTotal = c->w64.nTotal;
     a.vpBytes = up16(std::max<uint32_t>(c->w64.maxVp, 16));
+    a.progWords = c->batchProgWords;
     a.runCycles = sc.runCycles;
     a.cycleBase = sc.cycleBase;
     a.groupsActive = sc.groupsActive;
@@ -1045,7 +1088,7 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles, const BatchScratch *fixed = n
       a.probeVal = b.probe.as<uint8_t>();
     }
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    HIPCHK(c, launchBatch(a, c->batchGrid, c->batchBlockEff, c->batchLds, c->stream));
+    HIPCHK(c, launchBatch(a, c->batchInstEff, c->batchGrid, c->batchBlockEff, c->batchLds, c->stream));
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0;
@@ -1056,6 +1099,7 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles, const BatchScratch *fixed = n
     if (c->printEnable && c->stats.n_print) c->logDirty = true;
     c->prof.grid = c->batchGrid;
     c->prof.block = c->batchBlockEff;
+    c->prof.batch_group = c->batchInstEff;
     c->prof.lds_bytes = c->batchLds;
     c->prof.prog_bytes = c->w64.words.size() * 4;
     maxC = 0;
@@ -1306,7 +1350,8 @@ int tgs_set_launch(tgs_ctx *c, int kernel, uint32_t block, uint32_t groups) {
       c->w64Ready = false;
       break;
     case TGS_KERNEL_BATCH:
-      if (block) c->batchBlock = block;
+      if (block > 512) return c->fail(TGS_E_ARG, "batched kernel block must be at most 512 (eight waves)");
+      c->batchWaves = block / 64;
       c->batchGridOverride = groups;
       c->batchReady = false;
       if (c->loaded && c->batched) {  // the batched image is rebuilt now, not at the next run
