@@ -16,6 +16,12 @@ launch per slice of --slice cycles. An autotuner first sweeps the partition
 size (256..4,096 LUT4s) and the workgroup size on a short run; the logical design is identical for every
 partition size, so results do not depend on its choice.
 
+Large leg: the 10,000,000-LUT4 design (depth 128, 500,000 registers, 64
+SRAMs) on the streamed kernel for --large-cycles cycles (default 2,000, a
+reduced count of its 10,000-cycle workload: the rate is per cycle), with its
+own roofline, the plain kernel beside it and parity of all three pools
+against the oracle after --large-parity cycles.
+
 Serving leg: --serve-designs seeded designs of log-uniform size, jobs of 1-64
 instances with Poisson arrivals, served by the continuous-batching engine
 (toucangpurt_amd.serve) under an HBM budget; reports p50/p99 job latency
@@ -49,7 +55,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["single", "batched", "serve", "testbench", "both", "all"], default="all",
+    ap.add_argument("--mode", choices=["single", "batched", "large", "serve", "testbench", "both", "all"], default="all",
                     help="all = single + batched + serving + testbench legs")
     ap.add_argument("--tb-steps", type=int, default=300, help="testbench steps timed per path")
     ap.add_argument("--serve-jobs", type=int, default=48)
@@ -68,6 +74,8 @@ def parse():
     ap.add_argument("--kernel", choices=["auto", "stream", "wave64"], default="auto", help="single-leg kernel")
     ap.add_argument("--tune-single", type=int, default=2_000, help="cycles per kernel in the single-leg choice")
     ap.add_argument("--plain-cycles", type=int, default=1_000)
+    ap.add_argument("--large-cycles", type=int, default=2_000, help="timed cycles of the 10M-LUT4 leg")
+    ap.add_argument("--large-parity", type=int, default=40, help="cycles of the 10M-LUT4 leg checked against the oracle")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -246,6 +254,71 @@ def run_single(args):
         out["parity_cycles"] = total
         chk.cleanup()
         log(f"single: cpu {out['cpu']['value']:.1f} cycles/s, parity {out['parity']}")
+    return out
+
+
+def run_large(args):
+    """The 10M-LUT4 design on the streamed kernel: --large-cycles timed cycles
+    (a reduced count of the 10,000-cycle workload; the rate is per cycle),
+    roofline from the program's byte model, parity of all three pools against
+    the multi-threaded oracle after --large-parity cycles."""
+    import numpy as np
+    from toucangpurt_amd import KERNEL_STREAM, POOL_EXCHANGE, POOL_MEM, POOL_REG, gen, rt
+    d = gen.generate("lut10m", seed=args.seed)
+    log(f"large: design generated {d.stats['n_parts']} partitions")
+    sim = rt.Simulator(0)
+    assert sim.init(0, d.design, d.symbols, False, False) == 0
+    sim.set_kernel(KERNEL_STREAM)
+    sim.set_stimulus(d.inputs, args.seed)
+    stats = sim.program_stats()
+    sim.eval_free_running(20)
+    sim.reset_profile()
+    t0 = time.perf_counter()
+    sim.eval_free_running(args.large_cycles)
+    dt = time.perf_counter() - t0
+    prof = sim.profile()
+    sim.cleanup()
+    launch_ms = prof["kernel_ms"] / max(prof["launches"], 1)
+    cyc_per_launch = prof["cycles"] / max(prof["launches"], 1)
+    bytes_per_cycle = stats["b_prog"] + stats["b_state"]
+    achieved = bytes_per_cycle * cyc_per_launch / (launch_ms / 1e3) / 1e9
+    out = {"value": args.large_cycles / dt, "unit": "simulated cycles/s", "cycles": args.large_cycles,
+           "seconds": dt, "kernel": "stream", "grid": prof["grid"], "block": prof["block"],
+           "lds_bytes": prof["lds_bytes"],
+           "workload": f"lut10m: 10,000,000 LUT4, depth 128, 500,000 regs, 64 SRAM 8192x64b, 256 inputs; "
+                       f"{args.large_cycles} timed cycles (reduced from 10,000; the rate is per cycle); " + FANIN,
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cycle": bytes_per_cycle,
+                        "launch_ms": launch_ms, "cycles_per_launch": cyc_per_launch},
+           "barrier_stall_frac": (prof["barrier_ticks"] / prof["total_ticks"]) if prof["total_ticks"] else None}
+    log(f"large: stream {out['value']:.0f} cycles/s, launch {launch_ms:.1f} ms")
+    if not args.no_plain:
+        out["plain"] = plain_rate(d, max(20, args.plain_cycles // 10), args.seed)
+        out["speedup_vs_plain"] = out["value"] / out["plain"]["value"]
+        log(f"large: plain {out['plain']['value']:.0f} cycles/s")
+    if not args.no_cpu and args.large_parity > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        st = O.State(O.Program(d.design))
+        st.set_stimulus(d.inputs, args.seed)
+        nt = cpu_threads()
+        t0 = time.perf_counter()
+        st.run(args.large_parity, threads=nt)
+        cdt = time.perf_counter() - t0
+        out["cpu"] = {"value": args.large_parity / cdt, "unit": "simulated cycles/s", "cores": nt, "kind": "port",
+                      "cpu": cpu_model(),
+                      "sample": f"the first {args.large_parity} cycles of the 10M-LUT4 design, C oracle, "
+                                f"partitions of each region over {nt} threads"}
+        chk = rt.Simulator(0)
+        assert chk.init(0, d.design, d.symbols, False, False) == 0
+        chk.set_kernel(KERNEL_STREAM)
+        chk.set_stimulus(d.inputs, args.seed)
+        chk.eval_free_running(args.large_parity)
+        out["parity"] = all(bool(np.array_equal(chk.read_pool(pool), st.pool(k)))
+                            for k, pool in enumerate((POOL_REG, POOL_MEM, POOL_EXCHANGE)))
+        out["parity_cycles"] = args.large_parity
+        chk.cleanup()
+        log(f"large: cpu {out['cpu']['value']:.1f} cycles/s, parity {out['parity']}")
     return out
 
 
@@ -485,8 +558,18 @@ def main():
         sys.exit(2)
     single = run_single(args) if args.mode in ("single", "both", "all") else None
     batched = run_batched(args) if args.mode in ("batched", "both", "all") else None
+    large = run_large(args) if args.mode in ("large", "all") else None
     serving = run_serve(args) if args.mode in ("serve", "all") else None
     testbench = run_testbench(args) if args.mode in ("testbench", "all") else None
+    if single is None and batched is None and large is not None:
+        out = {"metric": "simulated cycles/sec @10M-LUT design", "value": large["value"],
+               "unit": "simulated cycles/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+               "ms_per_step": large["seconds"] * 1e3, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "u8", "data": "synthetic", "config": {"workload": large["workload"]},
+               "roofline": large["roofline"], "cpu_baseline": large.get("cpu"), "large": large,
+               "serving": serving, "testbench_step": testbench}
+        print(json.dumps(out))
+        return
     if single is None and batched is None and serving is None:
         out = {"metric": "testbench step latency (write 4 registers, 1 cycle, read 4 registers)",
                "value": testbench["single"]["tgs_step_us"], "unit": "us per step", "n_gpus": 1,
@@ -512,7 +595,7 @@ def main():
                "speedup_vs_plain": (single["value"] / single["plain"]["value"]) if single.get("plain") else None,
                "cpu_baseline": single.get("cpu"), "cpu_baseline_1thread": single.get("cpu_1thread"),
                "parity_vs_oracle": single.get("parity"), "parity_cycles": single.get("parity_cycles"),
-               "batched": batched, "serving": serving, "testbench_step": testbench}
+               "batched": batched, "large": large, "serving": serving, "testbench_step": testbench}
     else:
         out = {"metric": "batched stimulus-cycles/sec @200K-LUT design x instances", "value": batched["value"],
                "unit": "stimulus-cycles/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -520,7 +603,8 @@ def main():
                "vs_baseline": None, "dtype": "u8", "data": "synthetic",
                "config": {"workload": batched["workload"], "part_luts": batched["part_luts"]},
                "roofline": batched["roofline"], "cpu_baseline": batched.get("cpu"),
-               "speedup_vs_plain": batched.get("speedup_vs_plain"), "batched": batched, "serving": serving,
+               "speedup_vs_plain": batched.get("speedup_vs_plain"), "batched": batched, "large": large,
+               "serving": serving,
                "testbench_step": testbench}
     print(json.dumps(out))
 

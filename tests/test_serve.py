@@ -152,3 +152,10 @@ def test_small_jobs_use_their_own_contexts(designs):
         for i in range(j.instances):
             np.testing.assert_array_equal(j.reg[i], reg[i], err_msg=f"job {j.job_id} instance {i}")
             np.testing.assert_array_equal(j.mem[i], mem[i])
+
+
+def test_small_jobs_beyond_small_capacity_rejected(designs):
+    """A small-job bound above the small contexts' slot count would route jobs
+    to a context that can never hold them; the engine refuses it up front."""
+    with pytest.raises(ValueError, match="small_capacity"):
+        Engine(designs, capacity=8, backend=OracleBackend, small_jobs=5, small_capacity=4)

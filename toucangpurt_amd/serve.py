@@ -109,6 +109,9 @@ class Engine:
 
     def __init__(self, designs, budget=250 << 30, capacity=128, slice_cycles=2000, backend=gpu_backend,
                  clock=time.perf_counter, park_after=1.0, small_jobs=0, small_capacity=64):
+        if small_jobs > small_capacity:
+            # a job routed to the small context could never find enough free slots
+            raise ValueError(f"small_jobs ({small_jobs}) exceeds small_capacity ({small_capacity})")
         self.designs = designs
         self.budget = budget
         self.park_after = park_after
