@@ -71,7 +71,8 @@ typedef struct tgs_profile {
                              every workgroup of every launch */
   uint64_t total_ticks;   /* wave64 / stream: real-time ticks of the cycle loop, summed the same way */
   uint32_t batch_group;   /* batched: instances per wave (each wave evaluates 64 / batch_group ops per round) */
-  uint32_t reserved;
+  uint32_t batch_fast_tasks; /* batched: LUT micropartition pairs evaluated without index bounds checks
+                                (every value a nibble; 0 when the program or the state does not allow it) */
 } tgs_profile;
 
 /* ---- program inspection (host only, no device needed) ---- */

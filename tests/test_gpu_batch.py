@@ -101,3 +101,59 @@ def test_batched_trace_and_prints(gpu_available):
         assert sim.stopped(k) == o.st.stopped
         assert sim.stop_cycle(k) == o.st.stop_cycle
     sim.cleanup()
+
+
+def test_fast_lut_tasks_equal_bounds_checked_ones(gpu_available, monkeypatch):
+    """On a nibble program the batched image evaluates LUT pairs without index
+    bounds checks (T_LUTFAST, batch_fast_tasks > 0); TGS_BATCH_FAST=0 builds
+    the bounds-checked image. Both agree on every instance and with the oracle."""
+    d = design("mixed")
+    n = 40
+    cycles = np.random.default_rng(8).integers(1, 30, size=n).astype(np.uint32)
+    pools = {}
+    for fast in (True, False):
+        if fast:
+            monkeypatch.delenv("TGS_BATCH_FAST", raising=False)
+        else:
+            monkeypatch.setenv("TGS_BATCH_FAST", "0")
+        print(f"case batched fast={fast}", flush=True)
+        sim = gpu_sim(d.design, d.symbols, True, None, kernel=KERNEL_BATCH)
+        sim.set_instances(n)
+        sim.set_stimulus(d.inputs, 61)
+        sim.run_batch(cycles)
+        assert (sim.profile()["batch_fast_tasks"] > 0) == fast
+        pools[fast] = [gpu_pools(sim, k) for k in range(n)]
+        sim.cleanup()
+    for k in range(n):
+        assert_pools_equal(pools[True][k], pools[False][k], f"instance {k}: fast vs bounds-checked")
+    for k in (0, 17, n - 1):
+        orc = OracleRun(d.design, d.symbols, True, d.inputs, seed=61, instance=k)
+        orc.run(int(cycles[k]))
+        assert_pools_equal(pools[True][k], orc.pools(), f"instance {k}")
+
+
+def test_wide_register_bytes_switch_off_fast_tasks(gpu_available):
+    """Register bytes above 15 written through the API end the nibble state:
+    the next run rebuilds the image with bounds-checked LUT tasks, and the
+    instance still follows the oracle given the same bytes."""
+    from toucangpurt_amd._native import POOL_REG
+    d = design("mixed")
+    print("case batched wide register bytes", flush=True)
+    sim = gpu_sim(d.design, d.symbols, True, None, kernel=KERNEL_BATCH)
+    sim.set_instances(3)
+    sim.set_stimulus(d.inputs, 19)
+    sim.run_batch([5, 5, 5])
+    assert sim.profile()["batch_fast_tasks"] > 0
+    orc = OracleRun(d.design, d.symbols, True, d.inputs, seed=19, instance=1)
+    orc.run(5)
+    reg = sim.read_pool(POOL_REG, 1).copy()
+    wide = np.random.default_rng(2).integers(0, 256, size=reg.size).astype(np.uint8)
+    inputs = np.asarray(d.inputs, np.int64)
+    wide[inputs] = reg[inputs]
+    sim.write_pool(POOL_REG, wide, instance=1)
+    orc.st.set_pool(0, wide)
+    sim.run_batch([6, 6, 6])
+    assert sim.profile()["batch_fast_tasks"] == 0
+    orc.run(6)
+    assert_pools_equal(gpu_pools(sim, 1), orc.pools(), "instance 1 after wide register bytes")
+    sim.cleanup()

@@ -120,7 +120,7 @@ class Profile(ctypes.Structure):
         ("barrier_ticks", ctypes.c_uint64),
         ("total_ticks", ctypes.c_uint64),
         ("batch_group", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("batch_fast_tasks", ctypes.c_uint32),
     ]
 
 

@@ -57,7 +57,7 @@ enum W64Hdr : uint32_t {
 };
 
 // Task kinds (low byte of a task's first word).
-enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3, T_LUTROWS = 4 };
+enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3, T_LUTROWS = 4, T_LUTFAST = 5 };
 // T_LUTROWS (batched image only): the ops of both micropartitions of a
 // T_LUT2 pair with every operand and destination resolved to a row of the
 // wave's LDS area: value-pool rows [0, V) then level rows: buffer X
@@ -75,6 +75,33 @@ enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3,
 //       one last-level op that copies the lane, which is then dropped)
 //   last ops, 1 word each: value-pool slot | source level row << 16
 constexpr uint32_t kLevelRows = 160, kConstRow = 128;
+// T_LUTFAST (batched image only, built for one instance count I per wave):
+// the ops of a T_LUT2 pair with every operand and destination resolved to a
+// byte offset row * I from the lane's column of the wave's LDS area (rows as
+// in T_LUTROWS), and every LUT index resolved to a position in the LDS LUT
+// region below that needs no bounds check. Only for programs in which every
+// value is a nibble (nibbleProgram), and only for pairs whose ops satisfy:
+// top-op LUT index 0 or <= 65536 - 4096 (no 16-bit wrap), no middle-op LUT
+// operand an immediate >= 16 (operands <= 15 keep the index <= 0xFFF).
+//   w0  kind | nMid << 8
+//   w1  nTop (A + B) | nLast << 16
+//   w2  word offset (from w0) of the middle ops, w3 of the last ops
+//   ceil(nMid / 4) words of level sizes, 8 bits each (A + B ops)
+//   top ops (even offset), 2 words each: lut' | r0 << 16, r1 | r2 << 16;
+//       top op k writes level row k (A's ops, then B's)
+//   middle ops per level, 2 words each: lut' | r0 << 16, r1 | r2 << 16;
+//       op k of level l writes level row k of buffer Y (l even) or X (l odd),
+//       except on the last middle level, whose ops are followed by one u16
+//       destination offset each (padded to a word): a level row or, when
+//       exactly one last-level op copies the lane, that op's value-pool slot
+//   last ops, 1 word each: destination | source << 16
+// lut' = LUT position of index 0: the op's own LUT index when it is below
+// 4642, 4642 (a zero window) when it is not, kLutIdBase for index 0 (pass
+// op2 through: op0 and op1 read the constant row of 0, and the identity
+// bytes return op2).
+constexpr uint32_t kLutZeroEnd = kLutSize + 4096;  // zeros [kLutSize, kLutZeroEnd)
+constexpr uint32_t kLutIdBase = (kLutZeroEnd + 15) / 16 * 16;  // 32 identity bytes 0..31
+constexpr uint32_t kLutRegionB = kLutIdBase + 32;              // LDS bytes of the batched kernel's LUT region
 // T_LUT2: two on-disk 32-lane micropartitions in one wave (lanes 0-31 = A,
 //   lanes 32-63 = B). Words:
 //   w0  kind | nMid << 8

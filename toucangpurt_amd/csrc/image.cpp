@@ -519,7 +519,141 @@ static std::vector<uint32_t> lutRowsTask(const CGMicroPartInfo *A, const CGMicro
   return t;
 }
 
-Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32_t nWaves, uint32_t vpRows) {
+// A T_LUT2 pair as a T_LUTFAST task for I instances per wave and a value
+// pool of V rows; false (and no words) when an op needs the bounds-checked
+// lookup of T_LUTROWS.
+static bool lutFastTask(const CGMicroPartInfo *A, const CGMicroPartInfo *B, uint32_t V, uint32_t I,
+                        std::vector<uint32_t> &t) {
+  t.clear();
+  const uint32_t nMid = A->middleLevels.size();
+  const CGMicroPartInfo *H[2] = {A, B};
+  auto sizeAt = [&](uint32_t h, uint32_t l) -> uint32_t {  // ops of half h at level l (0 = top)
+    if (!H[h]) return 0;
+    return uint32_t(l == 0 ? H[h]->topLevel.size() : H[h]->middleLevels[l - 1].size());
+  };
+  auto off = [&](uint32_t row) -> uint32_t { return row * I; };
+  const uint32_t zeroRow = V + kConstRow;
+  // level buffer written by level p (0 = top: X; middle level m = p - 1:
+  // Y when m is even, X when it is odd)
+  auto bufOf = [](uint32_t p) -> uint32_t { return p == 0 ? 0u : ((p - 1) & 1u) ? 0u : 64u; };
+  // row read through reference lane o by an op of half h whose sources are
+  // level p (lanes at or past that level's size read 0; o < 32 is an immediate)
+  auto laneRow = [&](uint32_t o, uint32_t h, uint32_t p) -> uint32_t {
+    if (o < 32) return V + kConstRow + o;
+    const uint32_t lane = o & 31u;
+    if (lane >= sizeAt(h, p)) return zeroRow;
+    return V + bufOf(p) + (h ? sizeAt(0, p) : 0u) + lane;
+  };
+  auto lutPos = [](uint32_t lt) -> uint32_t { return lt == 0 ? kLutIdBase : std::min(lt, kLutSize); };
+  uint32_t nt = sizeAt(0, 0) + sizeAt(1, 0), nl = 0;
+  if (nt > 64) return false;
+  t.push_back(T_LUTFAST | nMid << 8);
+  t.push_back(0);
+  t.push_back(0);
+  t.push_back(0);
+  for (uint32_t l = 0; l < nMid; l += 4) {
+    uint32_t v = 0;
+    for (uint32_t k = 0; k < 4 && l + k < nMid; ++k) v |= (sizeAt(0, l + k + 1) + sizeAt(1, l + k + 1)) << (8 * k);
+    t.push_back(v);
+  }
+  if (t.size() & 1) t.push_back(0);
+  auto topRow = [&](uint32_t s) -> uint32_t { return s < 16 ? V + kConstRow + s : s; };
+  for (uint32_t h = 0; h < 2; ++h) {
+    if (!H[h]) continue;
+    for (const auto &o : H[h]->topLevel) {
+      const uint32_t lt = o.lutIndex;
+      if (lt > 65536u - 4096u) return false;  // the 16-bit index could wrap
+      const uint32_t r0 = lt ? topRow(o.op0) : zeroRow, r1 = lt ? topRow(o.op1) : zeroRow;
+      t.push_back(lutPos(lt) | off(r0) << 16);
+      t.push_back(off(r1) | off(topRow(o.op2)) << 16);
+    }
+  }
+  t[2] = t.size();
+  // lanes of the last middle level copied by exactly one last-level op are
+  // written straight to that op's slot (as in T_LUTROWS)
+  std::vector<int32_t> direct[2];
+  for (uint32_t h = 0; h < 2 && nMid; ++h) {
+    if (!H[h]) continue;
+    const uint32_t nPrev = sizeAt(h, nMid);
+    direct[h].assign(32, -1);
+    for (uint32_t q = 0; q < H[h]->lastLevel.size(); ++q) {
+      const uint32_t sid = H[h]->lastLevel[q].shuffleId;
+      if (sid < 32 || (sid & 31u) >= nPrev) continue;
+      int32_t &e = direct[h][sid & 31u];
+      e = e == -1 ? int32_t(q) : -2;
+    }
+  }
+  for (uint32_t l = 0; l < nMid; ++l) {
+    const uint32_t wb = bufOf(l + 1);  // buffer this level writes
+    std::vector<uint32_t> dsts;
+    uint32_t k = 0;
+    for (uint32_t h = 0; h < 2; ++h) {
+      if (!H[h]) continue;
+      uint32_t q = 0;
+      for (const auto &o : H[h]->middleLevels[l]) {
+        const uint32_t lt = o.lutIndex();
+        const uint32_t o0 = o.op0(), o1 = o.op1(), o2 = o.op2();
+        if (lt && ((o0 < 32 && o0 >= 16) || (o1 < 32 && o1 >= 16) || (o2 < 32 && o2 >= 16))) return false;
+        const uint32_t r0 = lt ? laneRow(o0, h, l) : zeroRow, r1 = lt ? laneRow(o1, h, l) : zeroRow;
+        t.push_back(lutPos(lt) | off(r0) << 16);
+        t.push_back(off(r1) | off(laneRow(o2, h, l)) << 16);
+        if (l + 1 == nMid) {
+          uint32_t dst = V + wb + k;
+          if (q < 32 && direct[h][q] >= 0) dst = H[h]->lastLevel[direct[h][q]].result;
+          dsts.push_back(off(dst));
+        }
+        ++k;
+        ++q;
+      }
+    }
+    if (k > 64) return false;
+    for (size_t j = 0; j < dsts.size(); j += 2)
+      t.push_back(dsts[j] | (j + 1 < dsts.size() ? dsts[j + 1] : dsts[j]) << 16);
+  }
+  t[3] = t.size();
+  const uint32_t fb = bufOf(nMid);  // buffer of the final level
+  for (uint32_t h = 0; h < 2; ++h) {
+    if (!H[h]) continue;
+    const uint32_t nPrev = sizeAt(h, nMid);
+    for (uint32_t q = 0; q < H[h]->lastLevel.size(); ++q) {
+      const auto &o = H[h]->lastLevel[q];
+      const uint32_t sid = o.shuffleId;
+      if (nMid && sid >= 32 && (sid & 31u) < nPrev && direct[h][sid & 31u] == int32_t(q)) continue;
+      uint32_t src;
+      if (sid < 32) src = V + kConstRow + sid;
+      else if ((sid & 31u) >= nPrev) src = zeroRow;
+      else src = V + fb + (h ? sizeAt(0, nMid) : 0u) + (sid & 31u);
+      t.push_back(off(o.result) | off(src) << 16);
+      ++nl;
+    }
+  }
+  t[1] = nt | nl << 16;
+  if (t.size() & 1) t.push_back(0);
+  return true;
+}
+
+bool nibbleProgram(const SimDesignInfo &d) {
+  for (uint8_t v : d.lut)
+    if (v > 15) return false;
+  for (const auto &p : d.parts) {
+    for (uint8_t v : p.constVecPool)
+      if (v > 15) return false;
+    for (const auto &lv : p.exec_mParts)
+      for (const auto &m : lv) {
+        if (!m.isLUTPart) continue;
+        // an immediate of 16..31 passed through or copied out becomes a value
+        for (const auto &ml : m.middleLevels)
+          for (const auto &o : ml)
+            if (o.lutIndex() == 0 && o.op2() >= 16 && o.op2() < 32) return false;
+        for (const auto &o : m.lastLevel)
+          if (o.shuffleId >= 16 && o.shuffleId < 32) return false;
+      }
+  }
+  return true;
+}
+
+Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32_t nWaves, uint32_t vpRows,
+                             uint32_t fastInst) {
   Wave64Image img;
   img.n0 = d.regionPartitionIds[0].size();
   img.nTotal = d.parts.size();
@@ -574,7 +708,14 @@ Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32
         i += B ? 2 : 1;
         uint32_t nMid = A->middleLevels.size();
         if (vpRows) {
-          st.push_back(lutRowsTask(A, B, vpRows));
+          std::vector<uint32_t> fast;
+          if (fastInst && lutFastTask(A, B, vpRows, fastInst, fast)) {
+            st.push_back(std::move(fast));
+            ++img.fastTasks;
+          } else {
+            st.push_back(lutRowsTask(A, B, vpRows));
+            ++img.rowTasks;
+          }
           levelCost += nMid + 2;
           continue;
         }

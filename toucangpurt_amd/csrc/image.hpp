@@ -47,12 +47,19 @@ struct Wave64Image {
   std::vector<uint8_t> consts;
   uint32_t maxVp = 0;
   uint32_t n0 = 0, nTotal = 0;
+  uint32_t fastTasks = 0, rowTasks = 0;  // batched image: LUT pairs as T_LUTFAST / T_LUTROWS
 };
 // vpRows > 0: the batched kernel's image, LUT pairs as T_LUTROWS tasks for
 // waves whose value pool has vpRows rows (level rows follow them); every
-// task starts at an even word.
+// task starts at an even word. fastInst > 0 (instances per wave; only for
+// programs that pass nibbleProgram and whose rows times fastInst fit in 16
+// bits): pairs whose ops allow it become T_LUTFAST tasks for that count.
 Wave64Image buildWave64Image(const toucanGPUSim::SimDesignInfo &d, const StimMap &stim, uint32_t nWaves,
-                             uint32_t vpRows = 0);
+                             uint32_t vpRows = 0, uint32_t fastInst = 0);
+// True when no LUT entry, constant-vector nibble or program immediate that
+// can become a value exceeds 15, so that every value the program computes
+// from nibble-valued pools is a nibble (LUT indices then stay below 0x1000).
+bool nibbleProgram(const toucanGPUSim::SimDesignInfo &d);
 
 struct Wave64Schedule {
   uint32_t groups = 0;

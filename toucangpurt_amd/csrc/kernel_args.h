@@ -100,6 +100,9 @@ hipError_t launchBatch(const BatchArgs &args, uint32_t inst, uint32_t grid, uint
                        hipStream_t stream);
 hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t inst, uint32_t block, uint32_t lds);
 hipError_t batchSetLds(uint32_t lds);
+// hipErrorInvalidKernelFile when a batched kernel was built with static LDS
+// (its LUT region must sit at LDS address 0).
+hipError_t batchCheckNoStaticLds();
 // out[0 .. n) = value (a kernel, so callers that record their work into a
 // HIP graph get no memset node).
 hipError_t launchFill32(uint32_t *out, uint64_t n, uint32_t value, hipStream_t stream);

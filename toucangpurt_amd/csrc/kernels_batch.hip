@@ -41,6 +41,11 @@ namespace {
 
 constexpr uint32_t kLutBytesB = ((kLutSize + 15) / 16) * 16;
 
+// The batched kernel's dynamic LDS: the LUT region at offset 0, then the
+// waves' rows and the program buffers (a file-scope name, so LUT positions
+// become immediate offsets of the lookups).
+extern __shared__ __align__(16) uint8_t bLds[];
+
 // One wave's view of its group of I instances.
 template <int I>
 struct Grp {
@@ -63,10 +68,7 @@ struct Grp {
   }
   __device__ __forceinline__ void put(uint32_t s, uint32_t v) const { vp[s * uint32_t(I) + ii] = uint8_t(v); }
   // op index of this lane in round `r` of a level of n >= 1 ops
-  __device__ __forceinline__ uint32_t opOf(uint32_t r, uint32_t n) const {
-    const uint32_t q = r + sub;
-    return q < n ? q : n - 1;
-  }
+  __device__ __forceinline__ uint32_t opOf(uint32_t r, uint32_t n) const { return min(r + sub, n - 1); }
 };
 
 // A wave-uniform value read from LDS, kept in a scalar register.
@@ -158,6 +160,99 @@ __device__ __forceinline__ void lutTask(const Grp<I> &g, const uint32_t *T) {
   for (uint32_t r = 0; r < nl; r += W) {
     const uint32_t w = last[g.opOf(r, nl)];
     vb[(w & 0xFFFFu) * I] = lb[((w >> 16) & 0xFFu) * I];
+  }
+}
+
+// LUT position of a T_LUTFAST op: (a << 4 | b) << 4 | c plus the low half
+// of the op's first word, in three VALU instructions (two shift-ors and one
+// add of a selected 16-bit half), which instruction selection does not
+// find on its own for this expression.
+__device__ __forceinline__ uint32_t lutPos(uint32_t w, uint32_t a, uint32_t b, uint32_t c) {
+  uint32_t t, idx, pos;
+  asm("v_lshl_or_b32 %0, %1, 4, %2" : "=v"(t) : "v"(a), "v"(b));
+  asm("v_lshl_or_b32 %0, %1, 4, %2" : "=v"(idx) : "v"(t), "v"(c));
+  asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+      : "=v"(pos)
+      : "v"(idx), "v"(w));
+  return pos;
+}
+
+// A T_LUTFAST task: the same two micropartitions as T_LUTROWS with every
+// operand and destination a byte offset from the lane's column of the wave's
+// LDS area and every LUT index a position in the LDS LUT region that needs
+// no bounds check (the image builder emits it only where every value is a
+// nibble). An op is its program words, three operand reads, the index
+// (op0 << 8 | op1 << 4 | op2) plus the op's LUT position, one lookup and one
+// write; top and middle ops share that form. Level sizes, prefetch and the
+// two rounds per iteration are as in lutTask.
+template <int I>
+__device__ __forceinline__ void lutFastTask(const Grp<I> &g, const uint32_t *T) {
+  constexpr uint32_t W = Grp<I>::W;
+  const uint32_t w0 = uni(T[0]), w1 = uni(T[1]);
+  const uint32_t nMid = (w0 >> 8) & 0xFFFFu;
+  const uint32_t nt = w1 & 0xFFFFu, nl = w1 >> 16;
+  const uint32_t *S = T + 4;
+  const uint2 *cur = reinterpret_cast<const uint2 *>(S + (((nMid + 3) >> 2) + 1 & ~1u));
+  const uint2 *mids = reinterpret_cast<const uint2 *>(T + uni(T[2]));
+  const uint32_t *last = T + uni(T[3]);
+  uint8_t *vb = g.vp + g.ii;   // offsets are relative to this lane's column
+  uint8_t *lb = g.lvl + g.ii;  // level row 0 (buffer X); buffer Y at row 64
+  auto sizeOf = [&](uint32_t m) -> uint32_t { return uni((S[m >> 2] >> ((m & 3u) * 8)) & 0xFFu); };
+  // operands are nibbles here, so (op0 << 4 | op1) << 4 | op2 is the index;
+  // the LUT region starts at LDS address 0 (the kernel has no static LDS,
+  // which the runtime checks before the first launch)
+  auto eval = [&](uint2 w) -> uint32_t {
+    const uint32_t pos = lutPos(w.x, vb[w.x >> 16], vb[w.y & 0xFFFFu], vb[w.y >> 16]);
+    return *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(pos);
+  };
+  const uint2 zero = make_uint2(0, 0);
+  uint2 c0 = zero, c1 = zero;
+  uint32_t n = nt;
+  if (n) {
+    c0 = cur[g.opOf(0, n)];
+    c1 = cur[g.opOf(W, n)];
+  }
+  // level l: 0 = top (writes X), m + 1 = middle level m (writes Y for even
+  // m, X for odd m; the last one writes through its destination list)
+  for (uint32_t l = 0; l <= nMid; ++l) {
+    const uint32_t nn = l < nMid ? sizeOf(l) : 0;
+    const uint2 *next = l == 0 ? mids : cur + n;
+    uint2 d0 = zero, d1 = zero;
+    if (nn) {  // next level's first words, read before this level's chain
+      d0 = next[g.opOf(0, nn)];
+      d1 = next[g.opOf(W, nn)];
+    }
+    if (n) {
+      if (l > 0 && l == nMid) {
+        const uint16_t *dst = reinterpret_cast<const uint16_t *>(cur + n);
+        for (uint32_t r = 0; r < n; r += 2 * W) {
+          const uint32_t q0 = g.opOf(r, n), q1 = g.opOf(r + W, n);
+          const uint2 a = r ? cur[q0] : c0, b = r ? cur[q1] : c1;
+          const uint32_t o0 = dst[q0], o1 = dst[q1];
+          const uint32_t va = eval(a), vbb = eval(b);
+          vb[o0] = uint8_t(va);
+          vb[o1] = uint8_t(vbb);
+        }
+      } else {
+        uint8_t *wr = lb + ((l == 0 || ((l - 1) & 1u)) ? 0u : 64u * uint32_t(I));
+        for (uint32_t r = 0; r < n; r += 2 * W) {
+          const uint32_t q0 = g.opOf(r, n), q1 = g.opOf(r + W, n);
+          const uint2 a = r ? cur[q0] : c0, b = r ? cur[q1] : c1;
+          const uint32_t va = eval(a), vbb = eval(b);
+          wr[q0 * uint32_t(I)] = uint8_t(va);
+          wr[q1 * uint32_t(I)] = uint8_t(vbb);
+        }
+      }
+    }
+    cur = next;
+    n = nn;
+    c0 = d0;
+    c1 = d1;
+  }
+  // last level: the copies not folded into the last middle level
+  for (uint32_t r = 0; r < nl; r += W) {
+    const uint32_t w = last[g.opOf(r, nl)];
+    vb[w & 0xFFFFu] = vb[w >> 16];
   }
 }
 
@@ -321,7 +416,8 @@ __device__ __forceinline__ void waveItem(const BatchArgs &a, uint32_t part, cons
     for (uint32_t t = lev[2 * e]; t < lev[2 * e + 1]; ++t) {
       const uint32_t *T = P + tab[t];
       const uint32_t kind = T[0] & 0xFFu;
-      if (kind == T_LUTROWS) lutTask<I>(g, T);
+      if (kind == T_LUTFAST) lutFastTask<I>(g, T);
+      else if (kind == T_LUTROWS) lutTask<I>(g, T);
       else if (kind == T_VECREAD) vecReadB<I>(g, T, consts);
       else if (kind == T_VECOP) vecOpB<I>(g, T, consts);
       else memReadB<I>(g, T);
@@ -448,15 +544,27 @@ __device__ __forceinline__ void batchRegion(const BatchArgs &a, uint32_t pb, uin
 // groupsActive[c] counts groups of I instances.
 template <int I>
 __global__ __launch_bounds__(512) void batchCycles(BatchArgs a) {
-  extern __shared__ __align__(16) uint8_t bLds[];
   const uint32_t NW = blockDim.x >> 6;
   uint8_t *lut = bLds;
-  uint8_t *vp = bLds + kLutBytesB;  // per wave: value-pool rows, then level rows
+  uint8_t *vp = bLds + kLutRegionB;  // per wave: value-pool rows, then level rows
   uint32_t *prog = reinterpret_cast<uint32_t *>(vp + NW * (a.vpBytes + kLevelRows) * uint32_t(I));
-  for (uint32_t k = threadIdx.x; k < kLutBytesB / 4; k += blockDim.x)
-    reinterpret_cast<uint32_t *>(lut)[k] = reinterpret_cast<const uint32_t *>(a.lut)[k];
-  {  // constant rows: level row kConstRow + v holds v for every instance
-    uint8_t *cr = vp + (threadIdx.x >> 6) * (a.vpBytes + kLevelRows) * uint32_t(I) + (a.vpBytes + kConstRow) * I;
+  // LUT region: the table (zero-padded to kLutBytesB), zeros to kLutIdBase,
+  // then the identity bytes 0..31
+  for (uint32_t k = threadIdx.x; k < kLutRegionB / 4; k += blockDim.x) {
+    uint32_t v = 0;
+    if (k < kLutBytesB / 4) v = reinterpret_cast<const uint32_t *>(a.lut)[k];
+    else if (k >= kLutIdBase / 4) {
+      const uint32_t b = 4 * k - kLutIdBase;
+      v = b | (b + 1) << 8 | (b + 2) << 16 | (b + 3) << 24;
+    }
+    reinterpret_cast<uint32_t *>(lut)[k] = v;
+  }
+  {  // the wave's rows start at 0 (a slot read before any write reads a
+     // nibble); constant rows: level row kConstRow + v holds v
+    const uint32_t rowBytes = (a.vpBytes + kLevelRows) * uint32_t(I);
+    uint32_t *wv = reinterpret_cast<uint32_t *>(vp + (threadIdx.x >> 6) * rowBytes);
+    for (uint32_t k = threadIdx.x & 63u; k < rowBytes / 4; k += 64) wv[k] = 0;
+    uint8_t *cr = vp + (threadIdx.x >> 6) * rowBytes + (a.vpBytes + kConstRow) * I;
     for (uint32_t k = threadIdx.x & 63u; k < (kLevelRows - kConstRow) * I; k += 64) cr[k] = uint8_t(k / I);
   }
   const uint32_t nDraws = (a.nIn + 63) / 64;
@@ -498,7 +606,7 @@ bool batchSupportsGroup(uint32_t inst) { return inst == 8 || inst == 16 || inst 
 
 // LUT, per wave value-pool and level rows, two program buffers.
 uint32_t batchLdsBytes(uint32_t inst, uint32_t waves, uint32_t vpBytes, uint32_t progWords) {
-  return kLutBytesB + waves * (vpBytes + kLevelRows) * inst + 2u * progWords * 4u;
+  return kLutRegionB + waves * (vpBytes + kLevelRows) * inst + 2u * progWords * 4u;
 }
 
 hipError_t launchBatch(const BatchArgs &args, uint32_t inst, uint32_t grid, uint32_t block, uint32_t lds,
@@ -510,6 +618,17 @@ hipError_t launchBatch(const BatchArgs &args, uint32_t inst, uint32_t grid, uint
 
 hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t inst, uint32_t block, uint32_t lds) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, batchFn(inst), block, lds);
+}
+
+// The kernels address the LUT region at LDS address 0: no static LDS.
+hipError_t batchCheckNoStaticLds() {
+  for (uint32_t inst : {8u, 16u, 32u, 64u}) {
+    hipFuncAttributes fa;
+    hipError_t e = hipFuncGetAttributes(&fa, batchFn(inst));
+    if (e != hipSuccess) return e;
+    if (fa.sharedSizeBytes != 0) return hipErrorInvalidKernelFile;
+  }
+  return hipSuccess;
 }
 
 hipError_t batchSetLds(uint32_t lds) {

@@ -169,16 +169,26 @@ struct tgs_ctx {
   std::vector<std::unique_ptr<Instance>> inst;
   bool batched = false;  // instance state lives in `bs` (batched kernel)
   BatchState bs;
-  // Two 64-instance groups per workgroup by default: with LDS for two groups'
-  // value pools two workgroups share a CU, so a launch with one or two live
-  // groups keeps twice the waves busy of a four-group workgroup (measured on
-  // a 200K-LUT4 design: 584 against 1,057 us per launch-cycle with 64
-  // instances, equal from 256 instances up).
-  // Batched launch shape: instances per wave (16, 32 or 64; 0 = chosen at
+  // Batched launch shape: instances per wave (8, 16, 32 or 64; 0 = chosen at
   // image build), waves per workgroup (0 = chosen) and the effective values.
   uint32_t batchInst = 0, batchWaves = 0, batchInstEff = 64, batchBlockEff = 128;
   uint32_t batchGrid = 0, batchLds = 0, batchGridOverride = 0, batchProgWords = 0;
   bool batchReady = false;
+  // Every byte of every pool is a nibble (true after a load whose initial
+  // pools are nibbles, false from the first wider byte written through the
+  // API); with a nibble program the batched image then uses T_LUTFAST tasks.
+  bool nibbleState = true, batchFast = false;
+  // A write of bytes `p`: a byte above 15 ends the nibble state, and a
+  // T_LUTFAST image is rebuilt without those tasks before the next run.
+  void noteWrite(const uint8_t *p, size_t n) {
+    if (!nibbleState) return;
+    for (size_t i = 0; i < n; ++i)
+      if (p[i] > 15) {
+        nibbleState = false;
+        if (batchFast) batchReady = false;
+        return;
+      }
+  }
 
   DBuf lut, inIds;
   bool plainReady = false, w64Ready = false;
@@ -886,16 +896,16 @@ int ensureBatchImage(tgs_ctx *c) {
                    8192 - kLevelRows, maxVp);
   c->w64 = buildWave64Image(c->design, c->stim, imageWaves, vpRows);
   c->w64Ready = false;  // the wave64 schedule must be rebuilt for this image
-  auto &im = c->w64;
-  int rc;
-  if ((rc = upload(c, c->wWords, im.words.data(), im.words.size() * 4))) return rc;
-  if ((rc = upload(c, c->wPartOff, im.partOff.data(), im.partOff.size() * 4))) return rc;
-  if ((rc = upload(c, c->wPartWords, im.partWords.data(), im.partWords.size() * 4))) return rc;
-  if ((rc = upload(c, c->wConsts, im.consts.data(), im.consts.size()))) return rc;
-  const uint32_t vpBytes = up16(std::max<uint32_t>(im.maxVp, 16));
-  uint32_t maxPart = 0;
-  for (auto w : im.partWords) maxPart = std::max(maxPart, (w + 3) & ~3u);
+  c->batchFast = false;
+  const uint32_t vpBytes = up16(std::max<uint32_t>(c->w64.maxVp, 16));
+  auto maxWords = [](const Wave64Image &im) -> uint32_t {
+    uint32_t m = 0;
+    for (auto w : im.partWords) m = std::max(m, (w + 3) & ~3u);
+    return m;
+  };
+  uint32_t maxPart = maxWords(c->w64);
   const uint32_t ldsMax = uint32_t(c->prop.sharedMemPerBlock) - kLdsGranuleMargin;
+  HIPCHK(c, batchCheckNoStaticLds());
   const char *envInst = std::getenv("TGS_BATCH_INST");
   const uint32_t forcedInst = c->batchInst ? c->batchInst : envInst ? uint32_t(std::atoi(envInst)) : 0u;
   std::vector<uint32_t> insts, waveList;
@@ -929,7 +939,38 @@ int ensureBatchImage(tgs_ctx *c) {
   if (!have)
     return c->fail(TGS_E_LIMIT,
                    "batched execution needs %u B of LDS (value pool %u slots x %u instances); device allows %u B",
-                   batchLdsBytes(insts[0], 1, vpBytes, maxPart), im.maxVp, insts[0], ldsMax);
+                   batchLdsBytes(insts[0], 1, vpBytes, maxPart), c->w64.maxVp, insts[0], ldsMax);
+  // T_LUTFAST tasks for the chosen instance count, when every value is a
+  // nibble and the rows' byte offsets fit 16 bits (kept only if the shape
+  // still fits as many waves per CU)
+  const char *envFast = std::getenv("TGS_BATCH_FAST");
+  const bool fastAllowed = !(envFast && envFast[0] == '0');
+  if (fastAllowed && c->nibbleState && (vpRows + kLevelRows) * bestInst <= 65536u && nibbleProgram(c->design)) {
+    Wave64Image f = buildWave64Image(c->design, c->stim, imageWaves, vpRows, bestInst);
+    const uint32_t fPart = maxWords(f);
+    const uint32_t lds = batchLdsBytes(bestInst, bestWaves, vpBytes, fPart);
+    int perCU = 0;
+    if (f.fastTasks && lds <= ldsMax) {
+      HIPCHK(c, batchSetLds(lds));
+      HIPCHK(c, batchMaxBlocksPerCU(&perCU, bestInst, bestWaves * 64, lds));
+      perCU = residentPerCU(c, perCU, lds);
+    }
+    if (perCU >= 1 && std::min<uint32_t>(16, uint32_t(perCU) * bestWaves) >= bestScore) {
+      c->w64 = std::move(f);
+      maxPart = fPart;
+      bestLds = lds;
+      bestPerCU = perCU;
+      c->batchFast = true;
+    }
+  }
+  {
+    auto &im = c->w64;
+    int rc;
+    if ((rc = upload(c, c->wWords, im.words.data(), im.words.size() * 4))) return rc;
+    if ((rc = upload(c, c->wPartOff, im.partOff.data(), im.partOff.size() * 4))) return rc;
+    if ((rc = upload(c, c->wPartWords, im.partWords.data(), im.partWords.size() * 4))) return rc;
+    if ((rc = upload(c, c->wConsts, im.consts.data(), im.consts.size()))) return rc;
+  }
   c->batchInstEff = bestInst;
   c->batchProgWords = maxPart;
   c->batchBlockEff = bestWaves * 64;
@@ -1100,6 +1141,7 @@ int runBatched(tgs_ctx *c, const uint32_t *cycles, const BatchScratch *fixed = n
     c->prof.grid = c->batchGrid;
     c->prof.block = c->batchBlockEff;
     c->prof.batch_group = c->batchInstEff;
+    c->prof.batch_fast_tasks = c->batchFast ? c->w64.fastTasks : 0;
     c->prof.lds_bytes = c->batchLds;
     c->prof.prog_bytes = c->w64.words.size() * 4;
     maxC = 0;
@@ -1282,6 +1324,10 @@ int tgs_load(tgs_ctx *c, const uint8_t *design, size_t designSize, const uint8_t
   if (randomize) c->design.Randomize(0, c->symbols);
   c->initReg = c->design.regPool;
   c->initMem = c->design.memPool;
+  c->nibbleState = true;
+  c->batchFast = false;
+  c->noteWrite(c->initReg.data(), c->initReg.size());
+  c->noteWrite(c->initMem.data(), c->initMem.size());
   c->designHash = 0xCBF29CE484222325ull;
   for (size_t i = 0; i < designSize; ++i) c->designHash = (c->designHash ^ design[i]) * 0x100000001B3ull;
   c->stim = StimMap();
@@ -1502,6 +1548,7 @@ static int poolIO(tgs_ctx *c, uint32_t instance, int pool, uint64_t off, uint64_
   if (off > size || n > size - off) return c->fail(TGS_E_ARG, "pool range out of bounds");
   if (n == 0) return TGS_OK;
   if (!host) return TGS_E_ARG;
+  if (write) c->noteWrite(host, n);
   if (!c->batched) {
     Instance *in = c->inst[instance].get();
     DBuf &b = pool == TGS_POOL_REG ? in->reg : pool == TGS_POOL_MEM ? in->mem : in->exg;
@@ -1595,6 +1642,7 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
     HIPCHK(c, c->boxDev.alloc(bytes));
     c->boxBytes = bytes;
   }
+  c->noteWrite(writeVals, size_t(ni) * nWrite);
   uint8_t *h = static_cast<uint8_t *>(c->boxHost), *d = c->boxDev.as<uint8_t>();
   std::memcpy(h, writeIds, 4ull * nWrite);
   std::memcpy(h + oRead, readIds, 4ull * nRead);
