@@ -4,10 +4,10 @@
 // It builds the batched image twice for one instance count I per wave: with
 // LUT pairs as T_LUTROWS tasks (bounds-checked lookups) and with T_LUTFAST
 // tasks where the builder allows them, then, partition by partition, runs
-// every LUT task of both images on a model of one wave's LDS (value-pool
+// the LUT tasks of both images on a model of one wave's LDS (value-pool
 // rows, level rows, constant rows, I instance columns, the LUT region) the
 // way the batched kernel's lane rounds do, from the same random nibble
-// state, and requires equal value pools after every task. Every offset a
+// state, and requires equal value pools after every exec level. Every offset a
 // T_LUTFAST op reads or writes is checked against the wave's rows and the
 // LUT region, and every operand it reads must be a nibble. Level rows start
 // with different random nibbles in the two models, so a row read before the
@@ -142,19 +142,20 @@ void runFast(Model &m, Area &A) {
     const uint32_t nn = l < nMid ? sizeOf(l) : 0;
     const uint64_t next = l == 0 ? midOff : cur + 2ull * n;
     const bool lastMid = l > 0 && l == nMid;
-    const uint32_t wb = (l == 0 || ((l - 1) & 1u)) ? 0u : 64u;
+    const uint32_t wb = (l == 0 || ((l - 1) & 1u)) ? 0u : kFastBufY;
+    if (n % W) die("task level not padded to whole rounds");
     for (uint32_t r = 0; r < n; r += 2 * W) {
       wr.clear();
-      for (uint32_t half = 0; half < 2; ++half)
+      for (uint32_t half = 0; half < (r + W < n ? 2u : 1u); ++half)
         for (uint32_t sub = 0; sub < W; ++sub) {
-          const uint32_t q = minOp(r + half * W + sub, n);
+          const uint32_t q = r + half * W + sub;
           const uint32_t x = m.word(cur + 2 * q), y = m.word(cur + 2 * q + 1);
           uint64_t dst;
           if (lastMid) {
             const uint32_t d = m.word(cur + 2ull * n + q / 2);
             dst = (q & 1u) ? d >> 16 : d & 0xFFFFu;
           } else {
-            if (wb + q >= 128) die("level row past the level buffers");
+            if (q >= kFastBufY || wb + q >= kConstRow) die("level row past its level buffer");
             dst = uint64_t(V + wb + q) * I;
           }
           for (uint32_t ii = 0; ii < I; ++ii) wr.push_back({dst + ii, eval(x, y, ii)});
@@ -164,9 +165,10 @@ void runFast(Model &m, Area &A) {
     cur = next;
     n = nn;
   }
+  if (nl % W) die("last ops not padded to whole rounds");
   for (uint32_t r = 0; r < nl; r += W)
     for (uint32_t sub = 0; sub < W; ++sub) {
-      const uint32_t w = m.word(lastOff + minOp(r + sub, nl));
+      const uint32_t w = m.word(lastOff + r + sub);
       for (uint32_t ii = 0; ii < I; ++ii) A.at((w & 0xFFFFu) + ii, "last write") = uint8_t(nib((w >> 16) + ii));
     }
 }
@@ -215,33 +217,37 @@ int main(int argc, char **argv) {
           ga.b[size_t(V + r) * I + ii] = fa.b[size_t(V + r) * I + ii] = uint8_t(r - kConstRow);
       const uint32_t nLev = GP[H_NLEV];
       if (FP[H_NLEV] != nLev) die("exec level counts differ");
+      // the LUT tasks of one exec level are independent: run all of them in
+      // each image, then compare (T_LUTFAST packs several pairs' parts)
       for (uint32_t e = 0; e < nLev; ++e) {
         const uint32_t *gl = GP + GP[H_OFF_LEV], *fl = FP + FP[H_OFF_LEV];
-        if (gl[2 * e + 1] - gl[2 * e] != fl[2 * e + 1] - fl[2 * e]) die("task counts differ");
-        for (uint32_t k = 0; k < gl[2 * e + 1] - gl[2 * e]; ++k) {
-          const uint32_t gt = (GP + GP[H_OFF_TASK])[gl[2 * e] + k], ft = (FP + FP[H_OFF_TASK])[fl[2 * e] + k];
-          const uint32_t gk = GP[gt] & 0xFFu, fk = FP[ft] & 0xFFu;
-          if (gk != T_LUTROWS) {
-            if (fk != gk) die("task kinds differ");
+        uint32_t gOther = 0, fOther = 0;
+        for (uint32_t k = gl[2 * e]; k < gl[2 * e + 1]; ++k) {
+          const uint32_t gt = (GP + GP[H_OFF_TASK])[k];
+          if ((GP[gt] & 0xFFu) != T_LUTROWS) {
+            ++gOther;
             continue;
           }
           Model gm{gi.words, gi.partOff[p] + gt, gi.partWords[p] - gt, lut};
           runRows(gm, ga);
+        }
+        for (uint32_t k = fl[2 * e]; k < fl[2 * e + 1]; ++k) {
+          const uint32_t ft = (FP + FP[H_OFF_TASK])[k];
+          const uint32_t fk = FP[ft] & 0xFFu;
+          Model fm{fi.words, fi.partOff[p] + ft, fi.partWords[p] - ft, lut};
           if (fk == T_LUTFAST) {
-            Model fm{fi.words, fi.partOff[p] + ft, fi.partWords[p] - ft, lut};
             runFast(fm, fa);
             ++nFast;
           } else if (fk == T_LUTROWS) {
-            Model fm{fi.words, fi.partOff[p] + ft, fi.partWords[p] - ft, lut};
             runRows(fm, fa);
             ++nRows;
           } else {
-            die("a LUT pair became another task kind");
+            ++fOther;
           }
-          if (!std::equal(ga.b.begin(), ga.b.begin() + size_t(V) * I, fa.b.begin()))
-            die("value pools differ after partition " + std::to_string(p) + " level " + std::to_string(e) +
-                " task " + std::to_string(k));
         }
+        if (gOther != fOther) die("other task counts differ");
+        if (!std::equal(ga.b.begin(), ga.b.begin() + size_t(V) * I, fa.b.begin()))
+          die("value pools differ after partition " + std::to_string(p) + " exec level " + std::to_string(e));
       }
     }
   }

@@ -62,7 +62,8 @@ enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3,
 // T_LUT2 pair with every operand and destination resolved to a row of the
 // wave's LDS area: value-pool rows [0, V) then level rows: buffer X
 // [0, 64), buffer Y [64, 128) (halves A and B at +0 / +32), and constant
-// rows [128, 160) holding 0..31 (level rows are relative to row V).
+// rows [kConstRow, kConstRow + 32) holding 0..31 (level rows are relative
+// to row V).
 //   w0  kind | nMid << 8
 //   w1  nTop (A + B) | nLast << 16
 //   w2  word offset (from w0) of the middle ops, w3 of the last ops
@@ -74,26 +75,38 @@ enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3,
 //       row + V, or, on the last middle level, the value-pool slot of the
 //       one last-level op that copies the lane, which is then dropped)
 //   last ops, 1 word each: value-pool slot | source level row << 16
-constexpr uint32_t kLevelRows = 160, kConstRow = 128;
+constexpr uint32_t kLevelRows = 176, kConstRow = 144;
+// T_LUTFAST's level buffers: X at level row 0, Y at kFastBufY, each with
+// room for 64 ops plus a round's padding (at most 7 ops)
+constexpr uint32_t kFastBufY = 72;
 // T_LUTFAST (batched image only, built for one instance count I per wave):
-// the ops of a T_LUT2 pair with every operand and destination resolved to a
-// byte offset row * I from the lane's column of the wave's LDS area (rows as
-// in T_LUTROWS), and every LUT index resolved to a position in the LDS LUT
-// region below that needs no bounds check. Only for programs in which every
-// value is a nibble (nibbleProgram), and only for pairs whose ops satisfy:
-// top-op LUT index 0 or <= 65536 - 4096 (no 16-bit wrap), no middle-op LUT
-// operand an immediate >= 16 (operands <= 15 keep the index <= 0xFFF).
+// LUT micropartitions of one exec level packed into one task, with every
+// operand and destination resolved to a byte offset row * I from the lane's
+// column of the wave's LDS area (rows as in T_LUTROWS), and every LUT index
+// resolved to a position in the LDS LUT region below that needs no bounds
+// check. Parts with fewer middle levels are aligned at the end: with nMid
+// the largest count, a part with m middle levels has its top level on task
+// level nMid - m and its last middle level on task level nMid; at most 64
+// ops on a task level, each task level padded to a multiple of the round
+// width 64 / I with copies of its last op (a copy writes the slack row
+// after the level's ops, or the same destination on task level nMid), and
+// the last ops padded the same way. Only for programs in which every value is a nibble
+// (nibbleProgram), and only for parts whose ops satisfy: top-op LUT index 0
+// or <= 65536 - 4096 (no 16-bit wrap), no middle-op LUT operand an
+// immediate >= 16 (operands <= 15 keep the index <= 0xFFF).
 //   w0  kind | nMid << 8
-//   w1  nTop (A + B) | nLast << 16
-//   w2  word offset (from w0) of the middle ops, w3 of the last ops
-//   ceil(nMid / 4) words of level sizes, 8 bits each (A + B ops)
-//   top ops (even offset), 2 words each: lut' | r0 << 16, r1 | r2 << 16;
-//       top op k writes level row k (A's ops, then B's)
-//   middle ops per level, 2 words each: lut' | r0 << 16, r1 | r2 << 16;
-//       op k of level l writes level row k of buffer Y (l even) or X (l odd),
-//       except on the last middle level, whose ops are followed by one u16
-//       destination offset each (padded to a word): a level row or, when
-//       exactly one last-level op copies the lane, that op's value-pool slot
+//   w1  ops on task level 0 | nLast << 16
+//   w2  word offset (from w0) of task level 1's ops, w3 of the last ops
+//   ceil(nMid / 4) words of the sizes of task levels 1..nMid, 8 bits each
+//   per task level (level 0 at an even offset), 2 words per op:
+//       lut' | r0 << 16, r1 | r2 << 16 (a part's top ops read value-pool or
+//       constant rows, its middle ops the rows of its own ops one task level
+//       up); op k of task level L writes level row k of buffer X (L = 0),
+//       Y (L odd, row kFastBufY + k) or X (L even), except on task level
+//       nMid >= 1, whose ops
+//       are followed by one u16 destination offset each (padded to a word):
+//       a level row or, when exactly one last-level op copies the lane, that
+//       op's value-pool slot
 //   last ops, 1 word each: destination | source << 16
 // lut' = LUT position of index 0: the op's own LUT index when it is below
 // 4642, 4642 (a zero window) when it is not, kLutIdBase for index 0 (pass

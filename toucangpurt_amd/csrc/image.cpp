@@ -519,61 +519,81 @@ static std::vector<uint32_t> lutRowsTask(const CGMicroPartInfo *A, const CGMicro
   return t;
 }
 
-// A T_LUT2 pair as a T_LUTFAST task for I instances per wave and a value
-// pool of V rows; false (and no words) when an op needs the bounds-checked
-// lookup of T_LUTROWS.
-static bool lutFastTask(const CGMicroPartInfo *A, const CGMicroPartInfo *B, uint32_t V, uint32_t I,
+// Level sizes of LUT micropartitions packed into one T_LUTFAST task, levels
+// aligned at the end: with nMid the largest middle-level count, part h
+// (middle-level count m_h) has its top level on task level nMid - m_h and
+// its last middle level on task level nMid. Entry L is the ops on task
+// level L (0 = the first).
+static std::vector<uint32_t> packedSizes(const std::vector<const CGMicroPartInfo *> &H) {
+  uint32_t nMid = 0;
+  for (auto *m : H) nMid = std::max<uint32_t>(nMid, m->middleLevels.size());
+  std::vector<uint32_t> tot(nMid + 1, 0);
+  for (auto *m : H) {
+    const uint32_t s0 = nMid - m->middleLevels.size();
+    tot[s0] += m->topLevel.size();
+    for (size_t l = 0; l < m->middleLevels.size(); ++l) tot[s0 + 1 + l] += m->middleLevels[l].size();
+  }
+  return tot;
+}
+
+// LUT micropartitions of one exec level (at most 64 ops on any task level
+// together, levels aligned at the end as in packedSizes) as one T_LUTFAST
+// task for I instances per wave and a value pool of V rows; false (and no
+// words) when an op needs the bounds-checked lookup of T_LUTROWS. On every
+// task level the parts' ops follow each other (part h's after parts
+// 0..h-1's); top and middle ops have one form, so a part's top level may sit
+// on any task level.
+static bool lutFastTask(const std::vector<const CGMicroPartInfo *> &H, uint32_t V, uint32_t I,
                         std::vector<uint32_t> &t) {
   t.clear();
-  const uint32_t nMid = A->middleLevels.size();
-  const CGMicroPartInfo *H[2] = {A, B};
-  auto sizeAt = [&](uint32_t h, uint32_t l) -> uint32_t {  // ops of half h at level l (0 = top)
-    if (!H[h]) return 0;
-    return uint32_t(l == 0 ? H[h]->topLevel.size() : H[h]->middleLevels[l - 1].size());
+  const uint32_t nh = H.size();
+  const std::vector<uint32_t> tot = packedSizes(H);
+  const uint32_t nMid = tot.size() - 1;
+  std::vector<uint32_t> start(nh);  // task level of part h's top level
+  for (uint32_t h = 0; h < nh; ++h) start[h] = nMid - H[h]->middleLevels.size();
+  // ops of part h on task level L (0 before its top level)
+  auto sizeAt = [&](uint32_t h, uint32_t L) -> uint32_t {
+    if (L < start[h]) return 0;
+    return uint32_t(L == start[h] ? H[h]->topLevel.size() : H[h]->middleLevels[L - start[h] - 1].size());
   };
+  std::vector<std::vector<uint32_t>> first(nMid + 1, std::vector<uint32_t>(nh + 1, 0));  // part h's op 0
+  for (uint32_t L = 0; L <= nMid; ++L) {
+    for (uint32_t h = 0; h < nh; ++h) first[L][h + 1] = first[L][h] + sizeAt(h, L);
+    if (first[L][nh] > 64) return false;
+  }
   auto off = [&](uint32_t row) -> uint32_t { return row * I; };
   const uint32_t zeroRow = V + kConstRow;
-  // level buffer written by level p (0 = top: X; middle level m = p - 1:
-  // Y when m is even, X when it is odd)
-  auto bufOf = [](uint32_t p) -> uint32_t { return p == 0 ? 0u : ((p - 1) & 1u) ? 0u : 64u; };
-  // row read through reference lane o by an op of half h whose sources are
-  // level p (lanes at or past that level's size read 0; o < 32 is an immediate)
-  auto laneRow = [&](uint32_t o, uint32_t h, uint32_t p) -> uint32_t {
+  const uint32_t W = 64 / I;  // ops per round: levels are padded to a multiple
+  auto padded = [&](uint32_t n) -> uint32_t { return (n + W - 1) / W * W; };
+  // level buffer written by task level L (0: X; then alternately Y, X, ...)
+  auto bufOf = [](uint32_t L) -> uint32_t { return L == 0 ? 0u : ((L - 1) & 1u) ? 0u : kFastBufY; };
+  // row read through reference lane o by an op of part h whose sources are
+  // task level L (lanes at or past that level's size read 0; o < 32 is an
+  // immediate)
+  auto laneRow = [&](uint32_t o, uint32_t h, uint32_t L) -> uint32_t {
     if (o < 32) return V + kConstRow + o;
     const uint32_t lane = o & 31u;
-    if (lane >= sizeAt(h, p)) return zeroRow;
-    return V + bufOf(p) + (h ? sizeAt(0, p) : 0u) + lane;
+    if (lane >= sizeAt(h, L)) return zeroRow;
+    return V + bufOf(L) + first[L][h] + lane;
   };
   auto lutPos = [](uint32_t lt) -> uint32_t { return lt == 0 ? kLutIdBase : std::min(lt, kLutSize); };
-  uint32_t nt = sizeAt(0, 0) + sizeAt(1, 0), nl = 0;
-  if (nt > 64) return false;
+  auto topRow = [&](uint32_t s) -> uint32_t { return s < 16 ? V + kConstRow + s : s; };
+  uint32_t nl = 0;
   t.push_back(T_LUTFAST | nMid << 8);
   t.push_back(0);
   t.push_back(0);
   t.push_back(0);
   for (uint32_t l = 0; l < nMid; l += 4) {
     uint32_t v = 0;
-    for (uint32_t k = 0; k < 4 && l + k < nMid; ++k) v |= (sizeAt(0, l + k + 1) + sizeAt(1, l + k + 1)) << (8 * k);
+    for (uint32_t k = 0; k < 4 && l + k < nMid; ++k) v |= padded(tot[l + k + 1]) << (8 * k);
     t.push_back(v);
   }
   if (t.size() & 1) t.push_back(0);
-  auto topRow = [&](uint32_t s) -> uint32_t { return s < 16 ? V + kConstRow + s : s; };
-  for (uint32_t h = 0; h < 2; ++h) {
-    if (!H[h]) continue;
-    for (const auto &o : H[h]->topLevel) {
-      const uint32_t lt = o.lutIndex;
-      if (lt > 65536u - 4096u) return false;  // the 16-bit index could wrap
-      const uint32_t r0 = lt ? topRow(o.op0) : zeroRow, r1 = lt ? topRow(o.op1) : zeroRow;
-      t.push_back(lutPos(lt) | off(r0) << 16);
-      t.push_back(off(r1) | off(topRow(o.op2)) << 16);
-    }
-  }
-  t[2] = t.size();
-  // lanes of the last middle level copied by exactly one last-level op are
-  // written straight to that op's slot (as in T_LUTROWS)
-  std::vector<int32_t> direct[2];
-  for (uint32_t h = 0; h < 2 && nMid; ++h) {
-    if (!H[h]) continue;
+  // lanes of a part's last level copied by exactly one last-level op are
+  // written straight to that op's slot when that level is a middle task
+  // level (it has a destination list)
+  std::vector<std::vector<int32_t>> direct(nh);
+  for (uint32_t h = 0; h < nh && nMid; ++h) {
     const uint32_t nPrev = sizeAt(h, nMid);
     direct[h].assign(32, -1);
     for (uint32_t q = 0; q < H[h]->lastLevel.size(); ++q) {
@@ -583,37 +603,51 @@ static bool lutFastTask(const CGMicroPartInfo *A, const CGMicroPartInfo *B, uint
       e = e == -1 ? int32_t(q) : -2;
     }
   }
-  for (uint32_t l = 0; l < nMid; ++l) {
-    const uint32_t wb = bufOf(l + 1);  // buffer this level writes
+  for (uint32_t L = 0; L <= nMid; ++L) {
+    if (L == 1) t[2] = t.size();
     std::vector<uint32_t> dsts;
-    uint32_t k = 0;
-    for (uint32_t h = 0; h < 2; ++h) {
-      if (!H[h]) continue;
+    for (uint32_t h = 0; h < nh; ++h) {
+      if (L < start[h]) continue;
       uint32_t q = 0;
-      for (const auto &o : H[h]->middleLevels[l]) {
-        const uint32_t lt = o.lutIndex();
-        const uint32_t o0 = o.op0(), o1 = o.op1(), o2 = o.op2();
-        if (lt && ((o0 < 32 && o0 >= 16) || (o1 < 32 && o1 >= 16) || (o2 < 32 && o2 >= 16))) return false;
-        const uint32_t r0 = lt ? laneRow(o0, h, l) : zeroRow, r1 = lt ? laneRow(o1, h, l) : zeroRow;
-        t.push_back(lutPos(lt) | off(r0) << 16);
-        t.push_back(off(r1) | off(laneRow(o2, h, l)) << 16);
-        if (l + 1 == nMid) {
-          uint32_t dst = V + wb + k;
+      auto push = [&](uint32_t lutp, uint32_t r0, uint32_t r1, uint32_t r2) {
+        t.push_back(lutp | off(r0) << 16);
+        t.push_back(off(r1) | off(r2) << 16);
+        if (L == nMid && L > 0) {
+          uint32_t dst = V + bufOf(L) + first[L][h] + q;
           if (q < 32 && direct[h][q] >= 0) dst = H[h]->lastLevel[direct[h][q]].result;
           dsts.push_back(off(dst));
         }
-        ++k;
         ++q;
+      };
+      if (L == start[h]) {
+        for (const auto &o : H[h]->topLevel) {
+          const uint32_t lt = o.lutIndex;
+          if (lt > 65536u - 4096u) return false;  // the 16-bit index could wrap
+          push(lutPos(lt), lt ? topRow(o.op0) : zeroRow, lt ? topRow(o.op1) : zeroRow, topRow(o.op2));
+        }
+      } else {
+        for (const auto &o : H[h]->middleLevels[L - start[h] - 1]) {
+          const uint32_t lt = o.lutIndex();
+          const uint32_t o0 = o.op0(), o1 = o.op1(), o2 = o.op2();
+          if (lt && ((o0 < 32 && o0 >= 16) || (o1 < 32 && o1 >= 16) || (o2 < 32 && o2 >= 16))) return false;
+          push(lutPos(lt), lt ? laneRow(o0, h, L - 1) : zeroRow, lt ? laneRow(o1, h, L - 1) : zeroRow,
+               laneRow(o2, h, L - 1));
+        }
       }
     }
-    if (k > 64) return false;
+    // padding: copies of the level's last op (and of its destination)
+    for (uint32_t k = tot[L]; k < padded(tot[L]); ++k) {
+      const uint32_t x = t[t.size() - 2], y = t[t.size() - 1];
+      t.push_back(x);
+      t.push_back(y);
+      if (L == nMid && L > 0) dsts.push_back(dsts.back());
+    }
     for (size_t j = 0; j < dsts.size(); j += 2)
       t.push_back(dsts[j] | (j + 1 < dsts.size() ? dsts[j + 1] : dsts[j]) << 16);
   }
+  if (nMid == 0) t[2] = t.size();
   t[3] = t.size();
-  const uint32_t fb = bufOf(nMid);  // buffer of the final level
-  for (uint32_t h = 0; h < 2; ++h) {
-    if (!H[h]) continue;
+  for (uint32_t h = 0; h < nh; ++h) {
     const uint32_t nPrev = sizeAt(h, nMid);
     for (uint32_t q = 0; q < H[h]->lastLevel.size(); ++q) {
       const auto &o = H[h]->lastLevel[q];
@@ -622,12 +656,13 @@ static bool lutFastTask(const CGMicroPartInfo *A, const CGMicroPartInfo *B, uint
       uint32_t src;
       if (sid < 32) src = V + kConstRow + sid;
       else if ((sid & 31u) >= nPrev) src = zeroRow;
-      else src = V + fb + (h ? sizeAt(0, nMid) : 0u) + (sid & 31u);
+      else src = V + bufOf(nMid) + first[nMid][h] + (sid & 31u);
       t.push_back(off(o.result) | off(src) << 16);
       ++nl;
     }
   }
-  t[1] = nt | nl << 16;
+  for (; nl % W; ++nl) t.push_back(t.back());
+  t[1] = padded(tot[0]) | nl << 16;
   if (t.size() & 1) t.push_back(0);
   return true;
 }
@@ -707,15 +742,42 @@ Wave64Image buildWave64Image(const SimDesignInfo &d, const StimMap &stim, uint32
         if (i + 1 < luts.size() && luts[i + 1]->middleLevels.size() == A->middleLevels.size()) B = luts[i + 1];
         i += B ? 2 : 1;
         uint32_t nMid = A->middleLevels.size();
-        if (vpRows) {
+        if (vpRows && fastInst) {
+          // T_LUTFAST: this part and the following ones (fewer middle
+          // levels: aligned at the end) while every task level holds at
+          // most 64 ops
+          std::vector<const CGMicroPartInfo *> grp{A};
+          if (B) grp.push_back(B);
+          while (i < luts.size()) {
+            grp.push_back(luts[i]);
+            const auto tot = packedSizes(grp);
+            if (*std::max_element(tot.begin(), tot.end()) > 64) {
+              grp.pop_back();
+              break;
+            }
+            ++i;
+          }
           std::vector<uint32_t> fast;
-          if (fastInst && lutFastTask(A, B, vpRows, fastInst, fast)) {
+          if (lutFastTask(grp, vpRows, fastInst, fast)) {
             st.push_back(std::move(fast));
             ++img.fastTasks;
           } else {
-            st.push_back(lutRowsTask(A, B, vpRows));
-            ++img.rowTasks;
+            for (size_t j = 0; j < grp.size();) {
+              const CGMicroPartInfo *b = j + 1 < grp.size() &&
+                                                 grp[j + 1]->middleLevels.size() == grp[j]->middleLevels.size()
+                                             ? grp[j + 1]
+                                             : nullptr;
+              st.push_back(lutRowsTask(grp[j], b, vpRows));
+              ++img.rowTasks;
+              j += b ? 2 : 1;
+            }
           }
+          levelCost += nMid + 2;
+          continue;
+        }
+        if (vpRows) {
+          st.push_back(lutRowsTask(A, B, vpRows));
+          ++img.rowTasks;
           levelCost += nMid + 2;
           continue;
         }

@@ -192,11 +192,12 @@ __device__ __forceinline__ void lutFastTask(const Grp<I> &g, const uint32_t *T) 
   const uint32_t nMid = (w0 >> 8) & 0xFFFFu;
   const uint32_t nt = w1 & 0xFFFFu, nl = w1 >> 16;
   const uint32_t *S = T + 4;
-  const uint2 *cur = reinterpret_cast<const uint2 *>(S + (((nMid + 3) >> 2) + 1 & ~1u));
-  const uint2 *mids = reinterpret_cast<const uint2 *>(T + uni(T[2]));
-  const uint32_t *last = T + uni(T[3]);
-  uint8_t *vb = g.vp + g.ii;   // offsets are relative to this lane's column
-  uint8_t *lb = g.lvl + g.ii;  // level row 0 (buffer X); buffer Y at row 64
+  // this lane's op in every round: op words at + sub, level rows at + sub * I
+  const uint2 *cur = reinterpret_cast<const uint2 *>(S + (((nMid + 3) >> 2) + 1 & ~1u)) + g.sub;
+  const uint2 *mids = reinterpret_cast<const uint2 *>(T + uni(T[2])) + g.sub;
+  const uint32_t *last = T + uni(T[3]) + g.sub;
+  uint8_t *vb = g.vp + g.ii;                        // offsets are relative to this lane's column
+  uint8_t *lb = g.lvl + g.ii + g.sub * uint32_t(I);  // level row `sub` of buffer X
   auto sizeOf = [&](uint32_t m) -> uint32_t { return uni((S[m >> 2] >> ((m & 3u) * 8)) & 0xFFu); };
   // operands are nibbles here, so (op0 << 4 | op1) << 4 | op2 is the index;
   // the LUT region starts at LDS address 0 (the kernel has no static LDS,
@@ -205,43 +206,47 @@ __device__ __forceinline__ void lutFastTask(const Grp<I> &g, const uint32_t *T) 
     const uint32_t pos = lutPos(w.x, vb[w.x >> 16], vb[w.y & 0xFFFFu], vb[w.y >> 16]);
     return *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(pos);
   };
+  // levels are padded to whole rounds: round r's op of this lane is r + sub
   const uint2 zero = make_uint2(0, 0);
   uint2 c0 = zero, c1 = zero;
   uint32_t n = nt;
-  if (n) {
-    c0 = cur[g.opOf(0, n)];
-    c1 = cur[g.opOf(W, n)];
-  }
-  // level l: 0 = top (writes X), m + 1 = middle level m (writes Y for even
-  // m, X for odd m; the last one writes through its destination list)
+  if (n) c0 = cur[0];
+  if (n > W) c1 = cur[W];
+  // level l: 0 = the first task level (writes X), then Y, X, ... (the last
+  // one writes through its destination list)
   for (uint32_t l = 0; l <= nMid; ++l) {
     const uint32_t nn = l < nMid ? sizeOf(l) : 0;
     const uint2 *next = l == 0 ? mids : cur + n;
     uint2 d0 = zero, d1 = zero;
-    if (nn) {  // next level's first words, read before this level's chain
-      d0 = next[g.opOf(0, nn)];
-      d1 = next[g.opOf(W, nn)];
-    }
+    if (nn) d0 = next[0];  // next level's first words, read before this level's chain
+    if (nn > W) d1 = next[W];
+    // rounds of W ops: two per iteration while a second round remains, then
+    // at most one more
     if (n) {
       if (l > 0 && l == nMid) {
-        const uint16_t *dst = reinterpret_cast<const uint16_t *>(cur + n);
-        for (uint32_t r = 0; r < n; r += 2 * W) {
-          const uint32_t q0 = g.opOf(r, n), q1 = g.opOf(r + W, n);
-          const uint2 a = r ? cur[q0] : c0, b = r ? cur[q1] : c1;
-          const uint32_t o0 = dst[q0], o1 = dst[q1];
+        const uint16_t *dst = reinterpret_cast<const uint16_t *>(cur - g.sub + n) + g.sub;
+        uint32_t r = 0;
+        for (; r + W < n; r += 2 * W) {
+          const uint2 a = r ? cur[r] : c0, b = r ? cur[r + W] : c1;
+          const uint32_t o0 = dst[r], o1 = dst[r + W];
           const uint32_t va = eval(a), vbb = eval(b);
           vb[o0] = uint8_t(va);
           vb[o1] = uint8_t(vbb);
         }
-      } else {
-        uint8_t *wr = lb + ((l == 0 || ((l - 1) & 1u)) ? 0u : 64u * uint32_t(I));
-        for (uint32_t r = 0; r < n; r += 2 * W) {
-          const uint32_t q0 = g.opOf(r, n), q1 = g.opOf(r + W, n);
-          const uint2 a = r ? cur[q0] : c0, b = r ? cur[q1] : c1;
-          const uint32_t va = eval(a), vbb = eval(b);
-          wr[q0 * uint32_t(I)] = uint8_t(va);
-          wr[q1 * uint32_t(I)] = uint8_t(vbb);
+        if (r < n) {
+          const uint32_t o0 = dst[r];
+          vb[o0] = uint8_t(eval(r ? cur[r] : c0));
         }
+      } else {
+        uint8_t *wr = lb + ((l == 0 || ((l - 1) & 1u)) ? 0u : kFastBufY * uint32_t(I));
+        uint32_t r = 0;
+        for (; r + W < n; r += 2 * W) {
+          const uint2 a = r ? cur[r] : c0, b = r ? cur[r + W] : c1;
+          const uint32_t va = eval(a), vbb = eval(b);
+          wr[r * uint32_t(I)] = uint8_t(va);
+          wr[(r + W) * uint32_t(I)] = uint8_t(vbb);
+        }
+        if (r < n) wr[r * uint32_t(I)] = uint8_t(eval(r ? cur[r] : c0));
       }
     }
     cur = next;
@@ -251,7 +256,7 @@ __device__ __forceinline__ void lutFastTask(const Grp<I> &g, const uint32_t *T) 
   }
   // last level: the copies not folded into the last middle level
   for (uint32_t r = 0; r < nl; r += W) {
-    const uint32_t w = last[g.opOf(r, nl)];
+    const uint32_t w = last[r];
     vb[w & 0xFFFFu] = vb[w >> 16];
   }
 }
