@@ -25,9 +25,11 @@ against the oracle after --large-parity cycles.
 Serving leg: --serve-designs seeded designs of log-uniform size, jobs of 1-64
 instances with Poisson arrivals, served by the continuous-batching engine
 (toucangpurt_amd.serve) under an HBM budget; reports p50/p99 job latency
-(arrival to result) and aggregate instance-cycles/s. The default sizes and
-cycle counts are scaled down from the 50K-5M LUT4 / 1K-100K cycle serving
-workload so the leg runs in about a minute; the flags restore the full one.
+(arrival to result) and aggregate instance-cycles/s. Designs span the
+serving workload's 50K-5M LUT4; job count (48) and cycles per job
+(100-2,000) are reduced from its 1K-100K so the leg runs in about a minute,
+and the flags restore the full range. The batched contexts' slices run in
+shared launches (tgs_run_batch_multi).
 
 The single leg picks the faster of the streamed and the wave64 persistent
 kernels on a short run (--kernel names one).
@@ -56,18 +58,20 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", choices=["single", "batched", "large", "serve", "testbench", "both", "all"], default="all",
-                    help="all = single + batched + serving + testbench legs")
+                    help="all = single + batched + large + serving + testbench legs")
     ap.add_argument("--tb-steps", type=int, default=300, help="testbench steps timed per path")
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals, jobs/s")
     ap.add_argument("--serve-designs", type=int, default=8)
     ap.add_argument("--serve-min-luts", type=int, default=50_000)
-    ap.add_argument("--serve-max-luts", type=int, default=1_000_000)
+    ap.add_argument("--serve-max-luts", type=int, default=5_000_000)
     ap.add_argument("--serve-min-cycles", type=int, default=100)
     ap.add_argument("--serve-max-cycles", type=int, default=2_000)
     ap.add_argument("--serve-slice", type=int, default=500)
     ap.add_argument("--serve-capacity", type=int, default=128)
     ap.add_argument("--serve-budget-gb", type=float, default=250.0)
+    ap.add_argument("--serve-separate", action="store_true",
+                    help="serving: one launch per design context per step instead of shared launches")
     ap.add_argument("--serve-small-jobs", type=int, default=0,
                     help="jobs of at most this many instances run on the streamed kernel (0: all batched)")
     ap.add_argument("--chunk", type=int, default=10_000, help="cycles per step (single leg)")
@@ -392,7 +396,8 @@ def run_batched(args):
            "instances_per_wave": prof["batch_group"], "lds_bytes": prof["lds_bytes"], "tuning": tuning,
            "workload": "lut200k: 200,000 LUT4, depth 48, 8 SRAM 1024x32b, 1,024 instances, "
                        f"cycles uniform in [{args.min_cycles}, {args.max_cycles}]; " + FANIN,
-           "bytes_per_cycle_shared": stats["b_prog"], "bytes_per_cycle_per_instance": stats["b_state"]}
+           "bytes_per_cycle_shared": stats["b_prog"], "bytes_per_cycle_per_instance": stats["b_state"],
+           "lut4_cycles_per_s": value * 200_000}
     # roofline: program bytes once per launch-cycle, state bytes per live instance-cycle
     algo = stats["b_prog"] * launch_cycles + stats["b_state"] * total
     achieved = algo / (prof["kernel_ms"] / 1e3) / 1e9
@@ -444,7 +449,7 @@ def run_serve(args):
     jobs = poisson_jobs(list(designs), args.serve_jobs, args.serve_rate, rng,
                         min_cycles=args.serve_min_cycles, max_cycles=args.serve_max_cycles)
     eng = Engine(designs, budget=int(args.serve_budget_gb * (1 << 30)), capacity=args.serve_capacity,
-                 slice_cycles=args.serve_slice, small_jobs=args.serve_small_jobs)
+                 slice_cycles=args.serve_slice, small_jobs=args.serve_small_jobs, shared=not args.serve_separate)
     done = [0]
 
     def progress():
@@ -466,10 +471,11 @@ def run_serve(args):
                             f"LUT4 (depth 32, partitions of 256), {args.serve_jobs} jobs, Poisson "
                             f"{args.serve_rate}/s, 1-64 instances x {args.serve_min_cycles}-{args.serve_max_cycles} "
                             f"cycles, {args.serve_budget_gb} GB budget, {args.serve_capacity} slots per design, "
-                            f"slice {args.serve_slice} cycles, jobs of <= {args.serve_small_jobs} instances on the streamed kernel",
+                            f"slice {args.serve_slice} cycles, jobs of <= {args.serve_small_jobs} instances on the streamed kernel, "
+                            f"{'one launch per design' if args.serve_separate else 'shared launches'}",
                 "unit": "instance-cycles/s"})
     log(f"serve: p50 {rep['p50_latency_s']:.3f} s p99 {rep['p99_latency_s']:.3f} s, "
-        f"{rep['cycles_per_s']:.3e} instance-cycles/s")
+        f"{rep['cycles_per_s']:.3e} instance-cycles/s, {rep['lut4_cycles_per_s']:.3e} LUT4-instance-cycles/s")
     return rep
 
 

@@ -140,12 +140,30 @@ int tgs_eval_free_running(tgs_ctx *ctx, uint32_t cycles, int *stopped);
 /* Batched run: instance k advances cycles[k] cycles (uneven counts). */
 int tgs_run_batch(tgs_ctx *ctx, const uint32_t *cycles, uint32_t n, int *any_stopped);
 
+/* No counterpart in the reference (the serving form of batching): tgs_run_batch
+ * of n batched contexts of one device (up to 64, distinct, typically one per
+ * design) in shared persistent launches, so that small batches of different
+ * designs occupy the GPU together instead of one after another. cycles[k]
+ * holds one count per instance of ctxs[k]; any_stopped (null or n entries)
+ * gets each context's flag as tgs_run_batch would. Every launch cycle runs
+ * region 0 of every context, then region 1; results are those of
+ * tgs_run_batch on each context alone. Contexts whose images were built for
+ * different instances per wave run in separate launches. The launches use
+ * the first context's stream after the other contexts' streams have
+ * drained; each context's profile counts the shared launch time. */
+int tgs_run_batch_multi(tgs_ctx *const *ctxs, uint32_t n, const uint32_t *const *cycles, int *any_stopped);
+
+/* No counterpart in the reference: device memory (bytes) the context holds
+ * now (program images, instance state, logs, probes, staging), for callers
+ * that budget HBM across many resident contexts. */
+int tgs_device_bytes(tgs_ctx *ctx, uint64_t *bytes);
+
 /* No counterpart in the reference: tgs_run_batch for callers that cannot
  * allocate device memory (serving loops, harnesses that forbid hipMalloc).
  * Same arguments and results as tgs_run_batch on a batched context; the
  * per-launch scratch of the batched kernel (slot cycle counts and bases, the
- * active-group table, the grid-barrier counter) lives in a static array of
- * 311,552 bytes per GPU, held until the process ends, instead of context
+ * active-group table, the grid-barrier counter, the launch arguments) lives
+ * in a static array of 312,064 bytes per GPU, held until the process ends, instead of context
  * allocations, and no hipMalloc / hipFree or other allocating call is made.
  * At most 4,096 instances (TGS_E_LIMIT beyond); the batched program image
  * must be built, which tgs_set_kernel(TGS_KERNEL_BATCH), tgs_set_instances,

@@ -7,7 +7,7 @@ import pytest
 
 from simharness import OracleRun
 from toucangpurt_amd import gen
-from toucangpurt_amd.serve import Engine, Job, run_alone
+from toucangpurt_amd.serve import Engine, Job, gpu_backend, run_alone
 
 pytestmark = pytest.mark.gpu
 
@@ -51,6 +51,17 @@ def test_serving_results_alone_equal_mix(gpu_available):
     assert rep["p99_latency_s"] >= rep["p50_latency_s"] > 0
 
 
+def context_device_bytes(d, capacity):
+    """Device bytes of a batched context of `capacity` slots after one run."""
+    sim = gpu_backend(d)
+    sim.set_instances(capacity)
+    sim.set_stimulus(d.inputs, 0)
+    sim.run_batch(np.ones(capacity, np.uint32))
+    n = sim.device_bytes()
+    sim.cleanup()
+    return n
+
+
 def test_parked_jobs_resume_exactly(gpu_available):
     """HBM budget for one design at a time: the second design's job parks the
     first design's running jobs (device checkpoints to host memory), and they
@@ -59,7 +70,12 @@ def test_parked_jobs_resume_exactly(gpu_available):
     designs = {k: gen.generate(None, **DESIGNS[k]) for k in ("small", "mixed")}
     t = [0.0]
     eng = Engine(designs, capacity=64, slice_cycles=40, clock=lambda: t[0], park_after=0.5)
-    eng.budget = max(eng.context_bytes(d) for d in designs.values()) + 1
+    # room for the larger context as the runtime reports it, not for both,
+    # and estimates calibrated on both
+    measured = {k: context_device_bytes(d, 64) for k, d in designs.items()}
+    for k, d in designs.items():
+        eng.calibrate(d, measured[k], 64)
+    eng.budget = max(measured.values()) + 1
     jobs = [Job(0, "small", 5, 300, 31, arrival=0.0), Job(1, "small", 3, 220, 32, arrival=0.0),
             Job(2, "mixed", 4, 150, 33, arrival=0.1)]
     print("case serving parking", flush=True)

@@ -159,3 +159,26 @@ def test_small_jobs_beyond_small_capacity_rejected(designs):
     to a context that can never hold them; the engine refuses it up front."""
     with pytest.raises(ValueError, match="small_capacity"):
         Engine(designs, capacity=8, backend=OracleBackend, small_jobs=5, small_capacity=4)
+
+
+class SizedOracleBackend(OracleBackend):
+    """Reports device bytes (tgs_device_bytes) of three times the estimate."""
+    estimate = {}
+
+    def __init__(self, design, kernel=None):
+        super().__init__(design, kernel)
+        self.design = design
+
+    def device_bytes(self):
+        return 3 * self.estimate[id(self.design)]
+
+
+def test_admission_charges_reported_device_bytes(designs):
+    """After a context's first run the engine holds what the runtime reports,
+    and estimates for later contexts scale by the ratio seen."""
+    eng = Engine(designs, capacity=4, slice_cycles=10, backend=SizedOracleBackend, small_jobs=0)
+    for d in designs.values():
+        SizedOracleBackend.estimate[id(d)] = eng.context_bytes(d)
+    eng.run([Job(0, "a", 2, 10, 1)])
+    assert eng.used == 3 * SizedOracleBackend.estimate[id(designs["a"])]
+    assert eng.context_bytes(designs["b"]) == 3 * SizedOracleBackend.estimate[id(designs["b"])]

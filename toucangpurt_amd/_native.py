@@ -157,7 +157,7 @@ RT_SYMBOLS = [
     "tgs_program_stats_from_bytes", "tgs_initial_state", "tgs_create", "tgs_create_on_stream", "tgs_destroy",
     "tgs_last_error", "tgs_load",
     "tgs_load_files", "tgs_set_kernel", "tgs_set_launch", "tgs_set_print", "tgs_set_instances", "tgs_set_stimulus",
-    "tgs_eval", "tgs_eval_free_running", "tgs_run_batch", "tgs_run_batch_noalloc", "tgs_read_regs", "tgs_write_regs", "tgs_step",
+    "tgs_eval", "tgs_eval_free_running", "tgs_run_batch", "tgs_run_batch_multi", "tgs_device_bytes", "tgs_run_batch_noalloc", "tgs_read_regs", "tgs_write_regs", "tgs_step",
     "tgs_read_pool", "tgs_write_pool", "tgs_pool_size", "tgs_reset_instance", "tgs_snapshot_begin",
     "tgs_snapshot_wait", "tgs_set_probes", "tgs_read_probes", "tgs_checkpoint_size", "tgs_checkpoint_save", "tgs_checkpoint_restore",
     "tgs_get_stop", "tgs_get_cycles",
@@ -195,6 +195,9 @@ def rt_lib():
         lib.tgs_eval_free_running.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]
         lib.tgs_run_batch.argtypes = [vp, _u32p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]
         lib.tgs_run_batch_noalloc.argtypes = [vp, _u32p, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)]
+        lib.tgs_run_batch_multi.argtypes = [ctypes.POINTER(vp), ctypes.c_uint32, ctypes.POINTER(_u32p),
+                                            ctypes.POINTER(ctypes.c_int)]
+        lib.tgs_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
         lib.tgs_read_regs.argtypes = [vp, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u8p]
         lib.tgs_write_regs.argtypes = [vp, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u8p]
         lib.tgs_step.argtypes = [vp, _u32p, ctypes.c_uint32, _u8p, ctypes.c_uint32, _u32p, ctypes.c_uint32, _u8p,

@@ -46,8 +46,7 @@ struct BatchArgs {
   uint32_t instPad;               // slots, a multiple of 64
   const uint32_t *words, *partOff, *partWords;  // wave64 image
   uint32_t n0, nTotal;
-  uint32_t vpBytes;               // value-pool slots per instance (16-multiple)
-  uint32_t progWords;             // words of one LDS program buffer (largest partition, 4-multiple)
+  uint32_t vpBytes;               // value-pool slots per instance (16-multiple); level rows follow
   const uint32_t *runCycles;      // per slot: cycles to run in this launch
   const uint64_t *cycleBase;      // per slot: cycles completed before this launch
   const uint32_t *groupsActive;   // per launch cycle: instance groups with a live slot (a prefix)
@@ -64,9 +63,20 @@ struct BatchArgs {
   const uint32_t *instOf;         // slot -> instance
   uint32_t logCap;
   uint32_t printEnable;
-  uint32_t *barrier;
   const uint32_t *probeOff, *probeSlot;  // signal probes (null when none), CSR by partition
   uint8_t *probeVal;                     // [probe][instPad]
+};
+
+// One persistent batched launch over one or more contexts of one device
+// (same instances per wave): their BatchArgs in device memory, the grid
+// barrier, and the LDS layout sized for the largest of them.
+struct BatchLaunch {
+  const BatchArgs *ctx;  // nCtx entries
+  uint32_t nCtx;
+  uint32_t nCycles;      // longest run of any context in this launch
+  uint32_t *barrier;
+  uint32_t vpRows;       // value-pool rows per wave (largest vpBytes)
+  uint32_t progWords;    // words of one LDS program buffer (largest partition, 4-multiple)
 };
 
 hipError_t launchPlain(const PlainArgs &args, uint32_t grid, uint32_t block, uint32_t lds, hipStream_t stream);
@@ -96,7 +106,7 @@ hipError_t streamSetLds(uint32_t lds);
 // per round) and block / 64 waves per workgroup.
 bool batchSupportsGroup(uint32_t inst);
 uint32_t batchLdsBytes(uint32_t inst, uint32_t waves, uint32_t vpBytes, uint32_t progWords);
-hipError_t launchBatch(const BatchArgs &args, uint32_t inst, uint32_t grid, uint32_t block, uint32_t lds,
+hipError_t launchBatch(const BatchLaunch &launch, uint32_t inst, uint32_t grid, uint32_t block, uint32_t lds,
                        hipStream_t stream);
 hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t inst, uint32_t block, uint32_t lds);
 hipError_t batchSetLds(uint32_t lds);

@@ -512,89 +512,137 @@ __device__ __forceinline__ void stageProgramAsync(uint32_t *dst, const uint32_t 
   }
 }
 
-// Work items of one region: (partition, NW consecutive groups), one group per wave.
+// Instance groups of context `a` with a live instance in launch cycle c.
+__device__ __forceinline__ uint32_t activeOf(const BatchArgs &a, uint32_t c) {
+  return c < a.nCycles ? a.groupsActive[c] : 0u;
+}
+
+// A wave's rows of the LDS area for context `a` (whose image puts its level
+// rows after its own value pool): all zero, then the constant rows.
 template <int I>
-__device__ __forceinline__ void batchRegion(const BatchArgs &a, uint32_t pb, uint32_t pe, uint32_t active,
-                                            uint8_t *vpAll, uint32_t *prog, const uint8_t *lut, const uint64_t *draws,
-                                            uint32_t c) {
-  const uint32_t np = pe - pb, NW = blockDim.x >> 6, wave = threadIdx.x >> 6;
-  const uint32_t nChunks = (active + NW - 1) / NW;
-  const uint32_t items = np * nChunks;
-  uint8_t *vp = vpAll + wave * (a.vpBytes + kLevelRows) * uint32_t(I);
-  uint8_t *lvl = vp + a.vpBytes * uint32_t(I);
+__device__ __forceinline__ void resetRows(uint8_t *vp, uint32_t rowBytes, const BatchArgs &a) {
+  uint32_t *wv = reinterpret_cast<uint32_t *>(vp);
+  for (uint32_t k = threadIdx.x & 63u; k < rowBytes / 4; k += 64) wv[k] = 0;
+  uint8_t *cr = vp + (a.vpBytes + kConstRow) * uint32_t(I);
+  for (uint32_t k = threadIdx.x & 63u; k < (kLevelRows - kConstRow) * I; k += 64) cr[k] = uint8_t(k / I);
+}
+
+// Work items of region r over every context of the launch, context by
+// context: (context, partition, NW consecutive groups), one group per wave.
+// A workgroup that moves to another context's items loads that context's
+// LUT and resets its waves' rows first.
+template <int I>
+__device__ __forceinline__ void batchRegion(const BatchLaunch &L, uint32_t r, uint32_t c, uint8_t *vp, uint32_t rowBytes,
+                                            uint32_t *prog, uint32_t &curCtx) {
+  const uint32_t NW = blockDim.x >> 6, wave = threadIdx.x >> 6;
+  auto partsOf = [&](const BatchArgs &a) -> uint32_t { return r == 0 ? a.n0 : a.nTotal - a.n0; };
+  auto itemsOf = [&](const BatchArgs &a) -> uint32_t { return partsOf(a) * ((activeOf(a, c) + NW - 1) / NW); };
+  uint32_t total = 0;
+  for (uint32_t j = 0; j < L.nCtx; ++j) total += itemsOf(L.ctx[j]);
+  // item k -> (context j, partition p, group)
+  auto locate = [&](uint32_t k, uint32_t &j, uint32_t &p, uint32_t &grp) {
+    j = 0;
+    for (uint32_t n; k >= (n = itemsOf(L.ctx[j])); ++j) k -= n;
+    const BatchArgs &a = L.ctx[j];
+    const uint32_t chunks = (activeOf(a, c) + NW - 1) / NW;
+    p = (r == 0 ? 0u : a.n0) + k / chunks;
+    grp = (k % chunks) * NW + wave;
+  };
   // two program buffers: the next item's program streams into one while
   // the current item runs from the other
   uint32_t k = blockIdx.x;
-  if (k < items) {
-    const uint32_t p = pb + k / nChunks;
+  if (k < total) {
+    uint32_t j, p, grp;
+    locate(k, j, p, grp);
+    const BatchArgs &a = L.ctx[j];
     stageProgramAsync(prog, a.words + a.partOff[p], a.partWords[p]);
   }
-  for (uint32_t buf = 0; k < items; k += gridDim.x, buf ^= 1u) {
-    const uint32_t p = pb + k / nChunks, grp = (k % nChunks) * NW + wave;
-    uint32_t *cur = prog + buf * a.progWords;
+  for (uint32_t buf = 0; k < total; k += gridDim.x, buf ^= 1u) {
+    uint32_t j, p, grp;
+    locate(k, j, p, grp);
+    const BatchArgs &a = L.ctx[j];
+    uint32_t *cur = prog + buf * L.progWords;
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's share of the copy has landed
     __syncthreads();
-    const uint32_t kn = k + gridDim.x;
-    if (kn < items) {
-      const uint32_t pn = pb + kn / nChunks;
-      stageProgramAsync(prog + (buf ^ 1u) * a.progWords, a.words + a.partOff[pn], a.partWords[pn]);
+    if (j != curCtx) {  // every wave is past its last item: switch the LUT and rows
+      for (uint32_t q = threadIdx.x; q < kLutBytesB / 4; q += blockDim.x)
+        reinterpret_cast<uint32_t *>(bLds)[q] = reinterpret_cast<const uint32_t *>(a.lut)[q];
+      resetRows<I>(vp, rowBytes, a);
+      curCtx = j;
+      __syncthreads();
     }
-    if (grp < active) waveItem<I>(a, p, cur, grp, vp, lvl, lut, draws, c);
+    const uint32_t kn = k + gridDim.x;
+    if (kn < total) {
+      uint32_t jn, pn, gn;
+      locate(kn, jn, pn, gn);
+      const BatchArgs &an = L.ctx[jn];
+      stageProgramAsync(prog + (buf ^ 1u) * L.progWords, an.words + an.partOff[pn], an.partWords[pn]);
+    }
+    if (grp < activeOf(a, c)) {
+      const uint64_t *draws = a.draws + size_t(c & 1u) * ((a.nIn + 63) / 64) * a.instPad;
+      waveItem<I>(a, p, cur, grp, vp, vp + a.vpBytes * uint32_t(I), bLds, draws, c);
+    }
     __syncthreads();  // every wave is done with `cur` before it is overwritten
   }
 }
 
 }  // namespace
 
-// groupsActive[c] counts groups of I instances.
+// One persistent launch over the launch's contexts: every launch cycle runs
+// region 0 of every context, a grid barrier, region 1, a grid barrier.
+// groupsActive[c] of a context counts its groups of I instances.
 template <int I>
-__global__ __launch_bounds__(512) void batchCycles(BatchArgs a) {
+__global__ __launch_bounds__(512) void batchCycles(BatchLaunch L) {
   const uint32_t NW = blockDim.x >> 6;
-  uint8_t *lut = bLds;
-  uint8_t *vp = bLds + kLutRegionB;  // per wave: value-pool rows, then level rows
-  uint32_t *prog = reinterpret_cast<uint32_t *>(vp + NW * (a.vpBytes + kLevelRows) * uint32_t(I));
-  // LUT region: the table (zero-padded to kLutBytesB), zeros to kLutIdBase,
-  // then the identity bytes 0..31
+  const uint32_t rowBytes = (L.vpRows + kLevelRows) * uint32_t(I);
+  uint8_t *vpAll = bLds + kLutRegionB;  // per wave: value-pool rows, then level rows
+  uint8_t *vp = vpAll + (threadIdx.x >> 6) * rowBytes;
+  uint32_t *prog = reinterpret_cast<uint32_t *>(vpAll + NW * rowBytes);
+  // LUT region: the first context's table (zero-padded to kLutBytesB), zeros
+  // to kLutIdBase, then the identity bytes 0..31
   for (uint32_t k = threadIdx.x; k < kLutRegionB / 4; k += blockDim.x) {
     uint32_t v = 0;
-    if (k < kLutBytesB / 4) v = reinterpret_cast<const uint32_t *>(a.lut)[k];
+    if (k < kLutBytesB / 4) v = reinterpret_cast<const uint32_t *>(L.ctx[0].lut)[k];
     else if (k >= kLutIdBase / 4) {
       const uint32_t b = 4 * k - kLutIdBase;
       v = b | (b + 1) << 8 | (b + 2) << 16 | (b + 3) << 24;
     }
-    reinterpret_cast<uint32_t *>(lut)[k] = v;
+    reinterpret_cast<uint32_t *>(bLds)[k] = v;
   }
-  {  // the wave's rows start at 0 (a slot read before any write reads a
-     // nibble); constant rows: level row kConstRow + v holds v
-    const uint32_t rowBytes = (a.vpBytes + kLevelRows) * uint32_t(I);
-    uint32_t *wv = reinterpret_cast<uint32_t *>(vp + (threadIdx.x >> 6) * rowBytes);
-    for (uint32_t k = threadIdx.x & 63u; k < rowBytes / 4; k += 64) wv[k] = 0;
-    uint8_t *cr = vp + (threadIdx.x >> 6) * rowBytes + (a.vpBytes + kConstRow) * I;
-    for (uint32_t k = threadIdx.x & 63u; k < (kLevelRows - kConstRow) * I; k += 64) cr[k] = uint8_t(k / I);
-  }
-  const uint32_t nDraws = (a.nIn + 63) / 64;
+  // the wave's rows start at 0 (a slot read before any write reads a nibble)
+  resetRows<I>(vp, rowBytes, L.ctx[0]);
+  uint32_t curCtx = 0;
   uint32_t bar = 0;
-  // draws of the first cycle, before anyone reads them
-  stimulusFor(a, 0, 0, a.groupsActive[0] * uint32_t(I));
+  auto stimulus = [&](uint32_t c) {  // draws of launch cycle c, before anyone reads them
+    for (uint32_t j = 0; j < L.nCtx; ++j) {
+      const BatchArgs &a = L.ctx[j];
+      stimulusFor(a, c & 1u, c, activeOf(a, c) * uint32_t(I));
+    }
+  };
+  stimulus(0);
   bar++;
-  gridBarrier(a.barrier, bar * gridDim.x, false);
-  for (uint32_t c = 0; c < a.nCycles; ++c) {
-    const uint32_t active = a.groupsActive[c];
-    if (active == 0) break;
-    const uint32_t activeSlots = active * uint32_t(I);
-    const uint64_t *draws = a.draws + size_t(c & 1u) * nDraws * a.instPad;
+  gridBarrier(L.barrier, bar * gridDim.x, false);
+  for (uint32_t c = 0; c < L.nCycles; ++c) {
+    uint32_t any = 0;
+    for (uint32_t j = 0; j < L.nCtx; ++j) any |= activeOf(L.ctx[j], c);
+    if (any == 0) break;
     // input registers of live instances (never read back through normal register reads)
-    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < activeSlots; s += gridDim.x * blockDim.x)
-      if (c < a.runCycles[s])
-        for (uint32_t i = 0; i < a.nIn; ++i)
-          a.reg[size_t(a.inIds[i]) * a.instPad + s] = uint8_t((draws[size_t(i >> 6) * a.instPad + s] >> (i & 63)) & 1u);
-    batchRegion<I>(a, 0, a.n0, active, vp, prog, lut, draws, c);
+    for (uint32_t j = 0; j < L.nCtx; ++j) {
+      const BatchArgs &a = L.ctx[j];
+      const uint32_t activeSlots = activeOf(a, c) * uint32_t(I);
+      const uint64_t *draws = a.draws + size_t(c & 1u) * ((a.nIn + 63) / 64) * a.instPad;
+      for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < activeSlots; s += gridDim.x * blockDim.x)
+        if (c < a.runCycles[s])
+          for (uint32_t i = 0; i < a.nIn; ++i)
+            a.reg[size_t(a.inIds[i]) * a.instPad + s] = uint8_t((draws[size_t(i >> 6) * a.instPad + s] >> (i & 63)) & 1u);
+    }
+    batchRegion<I>(L, 0, c, vp, rowBytes, prog, curCtx);
     bar++;
-    gridBarrier(a.barrier, bar * gridDim.x, false);
-    if (c + 1 < a.nCycles) stimulusFor(a, (c + 1) & 1u, c + 1, a.groupsActive[c + 1] * uint32_t(I));
-    batchRegion<I>(a, a.n0, a.nTotal, active, vp, prog, lut, draws, c);
+    gridBarrier(L.barrier, bar * gridDim.x, false);
+    if (c + 1 < L.nCycles) stimulus(c + 1);
+    batchRegion<I>(L, 1, c, vp, rowBytes, prog, curCtx);
     bar++;
-    gridBarrier(a.barrier, bar * gridDim.x, false);
+    gridBarrier(L.barrier, bar * gridDim.x, false);
   }
 }
 
@@ -609,14 +657,14 @@ const void *batchFn(uint32_t inst) {
 
 bool batchSupportsGroup(uint32_t inst) { return inst == 8 || inst == 16 || inst == 32 || inst == 64; }
 
-// LUT, per wave value-pool and level rows, two program buffers.
+// LUT region, per wave value-pool and level rows, two program buffers.
 uint32_t batchLdsBytes(uint32_t inst, uint32_t waves, uint32_t vpBytes, uint32_t progWords) {
   return kLutRegionB + waves * (vpBytes + kLevelRows) * inst + 2u * progWords * 4u;
 }
 
-hipError_t launchBatch(const BatchArgs &args, uint32_t inst, uint32_t grid, uint32_t block, uint32_t lds,
+hipError_t launchBatch(const BatchLaunch &launch, uint32_t inst, uint32_t grid, uint32_t block, uint32_t lds,
                        hipStream_t stream) {
-  BatchArgs a = args;
+  BatchLaunch a = launch;
   void *params[] = {&a};
   return hipLaunchCooperativeKernel(batchFn(inst), dim3(grid), dim3(block), params, lds, stream);
 }

@@ -30,14 +30,17 @@ using toucanGPUSim::SimDesignInfo;
 
 // Scratch memory of tgs_run_batch_noalloc: one static array per GPU, held
 // until the process ends. Layout: per-slot cycle counts (u32) and cycle bases
-// (u64) of up to kScratchSlots instance slots, the grid-barrier counter, then
-// the active-group table of up to kScratchCycles launch-cycles. In host code
+// (u64) of up to kScratchSlots instance slots, the grid-barrier counter, the
+// launch's BatchArgs, then the active-group table of up to kScratchCycles
+// launch-cycles. In host code
 // `scratch` names a host-side stand-in; its device address on each GPU comes
 // from hipGetSymbolAddress (scratchFor) and only that address is used.
 constexpr uint32_t kScratchSlots = 4096, kScratchCycles = 65536;
+constexpr uint32_t kMaxBatchContexts = 64;  // contexts per tgs_run_batch_multi call
 constexpr size_t kScratchRunOff = 0, kScratchBaseOff = 4ull * kScratchSlots,
-                 kScratchBarrierOff = kScratchBaseOff + 8ull * kScratchSlots, kScratchActiveOff = kScratchBarrierOff + 256,
-                 kScratchBytes = kScratchActiveOff + 4ull * kScratchCycles;
+                 kScratchBarrierOff = kScratchBaseOff + 8ull * kScratchSlots, kScratchArgsOff = kScratchBarrierOff + 256,
+                 kScratchActiveOff = kScratchArgsOff + 512, kScratchBytes = kScratchActiveOff + 4ull * kScratchCycles;
+static_assert(sizeof(BatchArgs) <= 512, "the scratch array holds one context's BatchArgs");
 static __device__ __align__(256) uint8_t scratch[kScratchBytes];
 
 namespace {
@@ -110,7 +113,7 @@ struct BatchState {
   uint32_t n = 0, instPad = 0;
   uint64_t logCap = 0;  // entries of the shared print log
   DBuf reg, mem, exg, xs, draws, stop, stopCycle, logCount, logCycle, logMsg, barrier, probe;
-  DBuf runCycles, cycleBase, groupsActive, instOfD, perm, scratch;
+  DBuf runCycles, cycleBase, groupsActive, instOfD, perm, scratch, argsDev;
   std::vector<uint32_t> slotOf, instOf;  // instance -> slot, slot -> instance
   std::vector<uint64_t> cycles;          // per instance
 };
@@ -219,6 +222,7 @@ struct tgs_ctx {
   // host bookkeeping of batched runs (kept so steady-state runs allocate nothing)
   std::vector<uint64_t> bhRemaining, bhBase, bhStop;
   std::vector<uint32_t> bhRun, bhActive;
+  std::vector<BatchArgs> bhArgs;
   void *scratchDev = nullptr;  // this device's static scratch array (tgs_run_batch_noalloc)
   SnapRing snap;
   bool timingPending = false;  // the last launch's events are not accounted yet
@@ -1038,122 +1042,219 @@ struct BatchScratch {
   uint64_t *cycleBase = nullptr;
   uint32_t *groupsActive = nullptr;
   uint32_t *barrier = nullptr;
+  BatchArgs *args = nullptr;  // the launch's BatchArgs (one context)
   uint64_t fixedCycles = 0;
 };
 
-int runBatched(tgs_ctx *c, const uint32_t *cycles, const BatchScratch *fixed = nullptr) {
-  int rc;
-  if ((rc = ensureBatchImage(c))) return rc;
+// BatchArgs of context c for one launch: its image and pools, and the
+// launch's per-slot cycle counts and bases and active-group table.
+BatchArgs batchArgsOf(tgs_ctx *c, const BatchScratch &sc, uint32_t nCycles) {
   auto &b = c->bs;
-  const uint32_t P = b.instPad;
-  if ((rc = batchOrder(c, cycles))) return rc;
-  // host bookkeeping kept in the context: no host allocation once sized
-  auto &remaining = c->bhRemaining, &base = c->bhBase;
-  auto &run = c->bhRun, &active = c->bhActive;
-  remaining.assign(P, 0);
-  base.assign(P, 0);
-  uint64_t maxC = 0;
-  for (uint32_t s = 0; s < P; ++s)
-    if (b.instOf[s] != 0xFFFFFFFFu) {
-      remaining[s] = cycles[b.instOf[s]];
-      base[s] = b.cycles[b.instOf[s]];
-      maxC = std::max(maxC, remaining[s]);
+  BatchArgs a{};
+  a.lut = c->lut.as<uint8_t>();
+  a.consts = c->wConsts.as<uint8_t>();
+  a.reg = b.reg.as<uint8_t>();
+  a.mem = b.mem.as<uint8_t>();
+  a.exg = b.exg.as<uint8_t>();
+  a.regSize = c->design.regPoolSize;
+  a.memSize = c->design.memPoolSize;
+  a.exgSize = c->design.exchangePoolSize;
+  a.instPad = b.instPad;
+  a.words = c->wWords.as<uint32_t>();
+  a.partOff = c->wPartOff.as<uint32_t>();
+  a.partWords = c->wPartWords.as<uint32_t>();
+  a.n0 = c->w64.n0;
+  a.nTotal = c->w64.nTotal;
+  a.vpBytes = up16(std::max<uint32_t>(c->w64.maxVp, 16));
+  a.runCycles = sc.runCycles;
+  a.cycleBase = sc.cycleBase;
+  a.groupsActive = sc.groupsActive;
+  a.nCycles = nCycles;
+  a.xs = b.xs.as<uint64_t>();
+  a.draws = b.draws.as<uint64_t>();
+  a.inIds = c->inIds.as<uint32_t>();
+  a.nIn = c->stim.ids.size();
+  a.stop = b.stop.as<uint64_t>();
+  a.stopCycle = b.stopCycle.as<uint64_t>();
+  a.logCount = b.logCount.as<uint64_t>();
+  a.logCycle = b.logCycle.as<uint64_t>();
+  a.logMsg = b.logMsg.as<uint64_t>();
+  a.instOf = b.instOfD.as<uint32_t>();
+  a.logCap = uint32_t(b.logCap);
+  a.printEnable = c->printEnable;
+  if (c->nProbe()) {
+    a.probeOff = c->prOff.as<uint32_t>();
+    a.probeSlot = c->prSlot.as<uint32_t>();
+    a.probeVal = b.probe.as<uint8_t>();
+  }
+  return a;
+}
+
+// Launch shape of a batched launch over several contexts: their common
+// instances per wave, the first context's waves per workgroup (fewer when
+// the largest value pool and program do not fit), LDS sized for the largest.
+struct BatchShape {
+  uint32_t inst = 0, block = 0, grid = 0, lds = 0, vpRows = 0, progWords = 0;
+};
+
+int batchShapeOf(tgs_ctx *const *cs, uint32_t nCtx, BatchShape *out) {
+  tgs_ctx *c0 = cs[0];
+  BatchShape sh;
+  sh.inst = c0->batchInstEff;
+  for (uint32_t k = 0; k < nCtx; ++k) {
+    sh.vpRows = std::max(sh.vpRows, up16(std::max<uint32_t>(cs[k]->w64.maxVp, 16)));
+    sh.progWords = std::max(sh.progWords, cs[k]->batchProgWords);
+  }
+  if (nCtx == 1) {
+    sh.block = c0->batchBlockEff;
+    sh.grid = c0->batchGrid;
+    sh.lds = c0->batchLds;
+    *out = sh;
+    return TGS_OK;
+  }
+  const uint32_t ldsMax = uint32_t(c0->prop.sharedMemPerBlock) - kLdsGranuleMargin;
+  for (uint32_t waves = c0->batchBlockEff / 64; waves >= 1; waves /= 2) {
+    const uint32_t lds = batchLdsBytes(sh.inst, waves, sh.vpRows, sh.progWords);
+    if (lds > ldsMax) continue;
+    HIPCHK(c0, batchSetLds(lds));
+    int perCU = 0;
+    HIPCHK(c0, batchMaxBlocksPerCU(&perCU, sh.inst, waves * 64, lds));
+    perCU = residentPerCU(c0, perCU, lds);
+    if (perCU < 1) continue;
+    perCU = std::min<int>(perCU, std::max<uint32_t>(1, 16 / waves));
+    sh.block = waves * 64;
+    sh.lds = lds;
+    sh.grid = uint32_t(perCU) * c0->prop.multiProcessorCount;
+    if (c0->batchGridOverride) sh.grid = std::min(sh.grid, c0->batchGridOverride);
+    *out = sh;
+    return TGS_OK;
+  }
+  return c0->fail(TGS_E_LIMIT, "the contexts' value pools and programs do not fit one workgroup's LDS together");
+}
+
+// Run the instances of nCtx batched contexts of one device in persistent
+// launches that serve all of them: each launch cycle runs region 0 of every
+// context, then region 1, so small batches of different designs share the
+// GPU. cycles[k] holds one count per instance of context k. Launches go on
+// the first context's stream after every context's stream has drained.
+// `fixed`: preallocated scratch (one context only, see BatchScratch).
+int runBatchedMulti(tgs_ctx *const *cs, const uint32_t *const *cycles, uint32_t nCtx,
+                    const BatchScratch *fixed = nullptr) {
+  int rc;
+  tgs_ctx *c0 = cs[0];
+  for (uint32_t k = 0; k < nCtx; ++k) {
+    tgs_ctx *c = cs[k];
+    if ((rc = ensureBatchImage(c))) return rc;
+    if ((rc = batchOrder(c, cycles[k]))) return rc;
+    if (c != c0) HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  BatchShape sh;
+  if ((rc = batchShapeOf(cs, nCtx, &sh))) return rc;
+  if (nCtx > 1) HIPCHK(c0, batchSetLds(sh.lds));
+  // host bookkeeping kept in the contexts: no host allocation once sized
+  uint64_t maxAll = 0, perLaunch = std::min<uint64_t>({1u << 20, 0xE0000000ull / (2ull * sh.grid + 2),
+                                                        fixed ? fixed->fixedCycles : uint64_t(UINT32_MAX)});
+  for (uint32_t k = 0; k < nCtx; ++k) {
+    tgs_ctx *c = cs[k];
+    auto &b = c->bs;
+    const uint32_t P = b.instPad;
+    auto &remaining = c->bhRemaining, &base = c->bhBase;
+    remaining.assign(P, 0);
+    base.assign(P, 0);
+    for (uint32_t s = 0; s < P; ++s)
+      if (b.instOf[s] != 0xFFFFFFFFu) {
+        remaining[s] = cycles[k][b.instOf[s]];
+        base[s] = b.cycles[b.instOf[s]];
+        maxAll = std::max(maxAll, remaining[s]);
+      }
+    perLaunch = std::min<uint64_t>(perLaunch, printCycleCap(c, b.logCap, P));
+    c->bhRun.assign(P, 0);
+  }
+  std::vector<BatchArgs> &args = c0->bhArgs;
+  args.resize(nCtx);
+  while (maxAll > 0) {
+    const uint32_t L = uint32_t(std::min<uint64_t>(maxAll, perLaunch));
+    const uint32_t I = sh.inst;
+    uint32_t *barrier = nullptr;
+    for (uint32_t k = 0; k < nCtx; ++k) {
+      tgs_ctx *c = cs[k];
+      auto &b = c->bs;
+      const uint32_t P = b.instPad;
+      if ((rc = drainLogs(c))) return rc;  // every launch starts with an empty device log
+      auto &run = c->bhRun, &active = c->bhActive;
+      uint32_t Lk = 0;
+      for (uint32_t s = 0; s < P; ++s) {
+        run[s] = uint32_t(std::min<uint64_t>(c->bhRemaining[s], L));
+        Lk = std::max(Lk, run[s]);
+      }
+      active.assign(std::max<uint32_t>(Lk, 1), 0);
+      for (uint32_t cy = 0; cy < Lk; ++cy) {
+        uint32_t g = 0;
+        while (g < P / I && run[g * I] > cy) ++g;  // slots sorted by decreasing cycles
+        active[cy] = g;
+      }
+      BatchScratch sc;
+      if (fixed) {
+        sc = *fixed;
+      } else {
+        HIPCHK(c, b.groupsActive.alloc(4ull * active.size()));
+        sc.runCycles = b.runCycles.as<uint32_t>();
+        sc.cycleBase = b.cycleBase.as<uint64_t>();
+        sc.groupsActive = b.groupsActive.as<uint32_t>();
+        sc.barrier = b.barrier.as<uint32_t>();
+      }
+      if (k == 0) barrier = sc.barrier;
+      HIPCHK(c, hipMemcpyAsync(sc.runCycles, run.data(), 4ull * P, hipMemcpyHostToDevice, c0->stream));
+      HIPCHK(c, hipMemcpyAsync(sc.cycleBase, c->bhBase.data(), 8ull * P, hipMemcpyHostToDevice, c0->stream));
+      HIPCHK(c, hipMemcpyAsync(sc.groupsActive, active.data(), 4ull * active.size(), hipMemcpyHostToDevice,
+                               c0->stream));
+      args[k] = batchArgsOf(c, sc, Lk);
     }
-  const uint64_t perLaunch = std::min<uint64_t>({1u << 20, 0xE0000000ull / (2ull * c->batchGrid + 2),
-                                                  printCycleCap(c, b.logCap, P),
-                                                  fixed ? fixed->fixedCycles : uint64_t(UINT32_MAX)});
-  run.assign(P, 0);
-  while (maxC > 0) {
-    if ((rc = drainLogs(c))) return rc;  // every launch starts with an empty device log
-    const uint32_t L = uint32_t(std::min<uint64_t>(maxC, perLaunch));
-    for (uint32_t s = 0; s < P; ++s) run[s] = uint32_t(std::min<uint64_t>(remaining[s], L));
-    active.assign(L, 0);
-    const uint32_t I = c->batchInstEff;
-    for (uint32_t cy = 0; cy < L; ++cy) {
-      uint32_t g = 0;
-      while (g < P / I && run[g * I] > cy) ++g;  // slots sorted by decreasing cycles
-      active[cy] = g;
-    }
-    BatchScratch sc;
+    BatchArgs *argsDev;
     if (fixed) {
-      sc = *fixed;
-      HIPCHK(c, launchFill32(sc.barrier, 1, 0, c->stream));  // a kernel, not a memset node
+      argsDev = fixed->args;
     } else {
-      HIPCHK(c, b.groupsActive.alloc(4ull * L));
-      sc.runCycles = b.runCycles.as<uint32_t>();
-      sc.cycleBase = b.cycleBase.as<uint64_t>();
-      sc.groupsActive = b.groupsActive.as<uint32_t>();
-      sc.barrier = b.barrier.as<uint32_t>();
-      HIPCHK(c, hipMemsetAsync(sc.barrier, 0, 4, c->stream));
+      HIPCHK(c0, c0->bs.argsDev.alloc(sizeof(BatchArgs) * nCtx));
+      argsDev = c0->bs.argsDev.as<BatchArgs>();
     }
-    HIPCHK(c, hipMemcpyAsync(sc.runCycles, run.data(), 4ull * P, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(sc.cycleBase, base.data(), 8ull * P, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(sc.groupsActive, active.data(), 4ull * L, hipMemcpyHostToDevice, c->stream));
-    BatchArgs a{};
-    a.lut = c->lut.as<uint8_t>();
-    a.consts = c->wConsts.as<uint8_t>();
-    a.reg = b.reg.as<uint8_t>();
-    a.mem = b.mem.as<uint8_t>();
-    a.exg = b.exg.as<uint8_t>();
-    a.regSize = c->design.regPoolSize;
-    a.memSize = c->design.memPoolSize;
-    a.exgSize = c->design.exchangePoolSize;
-    a.instPad = P;
-    a.words = c->wWords.as<uint32_t>();
-    a.partOff = c->wPartOff.as<uint32_t>();
-    a.partWords = c->wPartWords.as<uint32_t>();
-    a.n0 = c->w64.n0;
-    a.nTotal = c->w64.nTotal;
-    a.vpBytes = up16(std::max<uint32_t>(c->w64.maxVp, 16));
-    a.progWords = c->batchProgWords;
-    a.runCycles = sc.runCycles;
-    a.cycleBase = sc.cycleBase;
-    a.groupsActive = sc.groupsActive;
-    a.nCycles = L;
-    a.xs = b.xs.as<uint64_t>();
-    a.draws = b.draws.as<uint64_t>();
-    a.inIds = c->inIds.as<uint32_t>();
-    a.nIn = c->stim.ids.size();
-    a.stop = b.stop.as<uint64_t>();
-    a.stopCycle = b.stopCycle.as<uint64_t>();
-    a.logCount = b.logCount.as<uint64_t>();
-    a.logCycle = b.logCycle.as<uint64_t>();
-    a.logMsg = b.logMsg.as<uint64_t>();
-    a.instOf = b.instOfD.as<uint32_t>();
-    a.logCap = uint32_t(b.logCap);
-    a.printEnable = c->printEnable;
-    a.barrier = sc.barrier;
-    if (c->nProbe()) {
-      a.probeOff = c->prOff.as<uint32_t>();
-      a.probeSlot = c->prSlot.as<uint32_t>();
-      a.probeVal = b.probe.as<uint8_t>();
-    }
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    HIPCHK(c, launchBatch(a, c->batchInstEff, c->batchGrid, c->batchBlockEff, c->batchLds, c->stream));
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipEventSynchronize(c->ev1));
+    HIPCHK(c0, hipMemcpyAsync(argsDev, args.data(), sizeof(BatchArgs) * nCtx, hipMemcpyHostToDevice, c0->stream));
+    HIPCHK(c0, launchFill32(barrier, 1, 0, c0->stream));  // a kernel, not a memset node
+    BatchLaunch bl{argsDev, nCtx, L, barrier, sh.vpRows, sh.progWords};
+    HIPCHK(c0, hipEventRecord(c0->ev0, c0->stream));
+    HIPCHK(c0, launchBatch(bl, sh.inst, sh.grid, sh.block, sh.lds, c0->stream));
+    HIPCHK(c0, hipEventRecord(c0->ev1, c0->stream));
+    HIPCHK(c0, hipEventSynchronize(c0->ev1));
     float ms = 0;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->prof.kernel_ms += ms;
-    c->prof.launches += 1;
-    c->prof.kernel = TGS_KERNEL_BATCH;
-    if (c->printEnable && c->stats.n_print) c->logDirty = true;
-    c->prof.grid = c->batchGrid;
-    c->prof.block = c->batchBlockEff;
-    c->prof.batch_group = c->batchInstEff;
-    c->prof.batch_fast_tasks = c->batchFast ? c->w64.fastTasks : 0;
-    c->prof.lds_bytes = c->batchLds;
-    c->prof.prog_bytes = c->w64.words.size() * 4;
-    maxC = 0;
-    for (uint32_t s = 0; s < P; ++s) {
-      c->prof.cycles += run[s];
-      remaining[s] -= run[s];
-      base[s] += run[s];
-      maxC = std::max(maxC, remaining[s]);
+    HIPCHK(c0, hipEventElapsedTime(&ms, c0->ev0, c0->ev1));
+    maxAll = 0;
+    for (uint32_t k = 0; k < nCtx; ++k) {
+      tgs_ctx *c = cs[k];
+      const uint32_t P = c->bs.instPad;
+      c->prof.kernel_ms += ms;  // the shared launch, counted for every context it served
+      c->prof.launches += 1;
+      c->prof.kernel = TGS_KERNEL_BATCH;
+      if (c->printEnable && c->stats.n_print) c->logDirty = true;
+      c->prof.grid = sh.grid;
+      c->prof.block = sh.block;
+      c->prof.batch_group = sh.inst;
+      c->prof.batch_fast_tasks = c->batchFast ? c->w64.fastTasks : 0;
+      c->prof.lds_bytes = sh.lds;
+      c->prof.prog_bytes = c->w64.words.size() * 4;
+      for (uint32_t s = 0; s < P; ++s) {
+        c->prof.cycles += c->bhRun[s];
+        c->bhRemaining[s] -= c->bhRun[s];
+        c->bhBase[s] += c->bhRun[s];
+        maxAll = std::max(maxAll, c->bhRemaining[s]);
+      }
     }
   }
-  for (uint32_t i = 0; i < b.n; ++i) b.cycles[i] += cycles[i];
+  for (uint32_t k = 0; k < nCtx; ++k)
+    for (uint32_t i = 0; i < cs[k]->bs.n; ++i) cs[k]->bs.cycles[i] += cycles[k][i];
   return TGS_OK;
+}
+
+int runBatched(tgs_ctx *c, const uint32_t *cycles, const BatchScratch *fixed = nullptr) {
+  return runBatchedMulti(&c, &cycles, 1, fixed);
 }
 
 int switchLayout(tgs_ctx *c, bool batched) {
@@ -1488,6 +1589,77 @@ int tgs_run_batch(tgs_ctx *c, const uint32_t *cycles, uint32_t n, int *anyStoppe
   return TGS_OK;
 }
 
+// Device memory the context holds: every buffer of its instances, images,
+// batched state, probes, profile and testbench mailbox, and the waveform
+// staging ring.
+uint64_t deviceBytes(const tgs_ctx *c) {
+  uint64_t t = 0;
+  for (const DBuf *b : {&c->lut, &c->inIds, &c->pNet, &c->pParts, &c->pLevels, &c->pMparts, &c->pConsts, &c->wWords,
+                        &c->wPartOff, &c->wPartWords, &c->wSchedOff, &c->wSchedPart, &c->wSchedLds, &c->wConsts,
+                        &c->sWords, &c->sSteps, &c->sStepBase, &c->sStepCount, &c->sConsts, &c->profUseful,
+                        &c->profWg, &c->profOwner, &c->boxDev, &c->prOff, &c->prSlot})
+    t += b->p ? b->n : 0;
+  const auto &b = c->bs;
+  for (const DBuf *d : {&b.reg, &b.mem, &b.exg, &b.xs, &b.draws, &b.stop, &b.stopCycle, &b.logCount, &b.logCycle,
+                        &b.logMsg, &b.barrier, &b.probe, &b.runCycles, &b.cycleBase, &b.groupsActive, &b.instOfD,
+                        &b.perm, &b.scratch, &b.argsDev})
+    t += d->p ? d->n : 0;
+  for (const auto &in : c->inst)
+    for (const DBuf *d : {&in->reg, &in->mem, &in->exg, &in->xs, &in->ctl, &in->logCycle, &in->logMsg, &in->probe})
+      t += d->p ? d->n : 0;
+  if (c->snap.staging) t += uint64_t(c->snap.slots) * c->snap.bytes;
+  return t;
+}
+
+int tgs_device_bytes(tgs_ctx *c, uint64_t *bytes) {
+  if (!c || !bytes) return TGS_E_ARG;
+  *bytes = deviceBytes(c);
+  return TGS_OK;
+}
+
+int tgs_run_batch_multi(tgs_ctx *const *ctxs, uint32_t n, const uint32_t *const *cycles, int *anyStopped) {
+  if (!ctxs || !cycles || n == 0) return TGS_E_ARG;
+  for (uint32_t k = 0; k < n; ++k) {
+    tgs_ctx *c = ctxs[k];
+    if (!c) return TGS_E_ARG;
+    int rc = checkLoaded(c);
+    if (rc) return rc;
+    if (!c->batched) return c->fail(TGS_E_STATE, "tgs_run_batch_multi runs batched contexts (tgs_set_kernel batched)");
+    if (!cycles[k]) return c->fail(TGS_E_ARG, "need one cycle count per instance");
+    if (c->device != ctxs[0]->device) return c->fail(TGS_E_ARG, "the contexts of one call share one device");
+    for (uint32_t j = 0; j < k; ++j)
+      if (ctxs[j] == c) return c->fail(TGS_E_ARG, "a context appears twice in one call");
+  }
+  if (n > kMaxBatchContexts)
+    return ctxs[0]->fail(TGS_E_LIMIT, "at most %u contexts per call (%u given)", kMaxBatchContexts, n);
+  int rc;
+  for (uint32_t k = 0; k < n; ++k)
+    if ((rc = setDevice(ctxs[k])) || (rc = ensureBatchImage(ctxs[k]))) return rc;
+  // one launch per instance count per wave (the kernel is specialised on it)
+  std::vector<tgs_ctx *> grp;
+  std::vector<const uint32_t *> cyc;
+  for (uint32_t inst : {8u, 16u, 32u, 64u}) {
+    grp.clear();
+    cyc.clear();
+    for (uint32_t k = 0; k < n; ++k)
+      if (ctxs[k]->batchInstEff == inst) {
+        grp.push_back(ctxs[k]);
+        cyc.push_back(cycles[k]);
+      }
+    if (!grp.empty() && (rc = runBatchedMulti(grp.data(), cyc.data(), uint32_t(grp.size())))) return rc;
+  }
+  for (uint32_t k = 0; k < n; ++k) {
+    tgs_ctx *c = ctxs[k];
+    auto &st = c->bhStop;
+    st.resize(c->bs.instPad);
+    HIPCHK(c, hipMemcpy(st.data(), c->bs.stop.p, 8 * st.size(), hipMemcpyDeviceToHost));
+    int any = 0;
+    for (auto v : st) any |= v != 0;
+    if (anyStopped) anyStopped[k] = any;
+  }
+  return TGS_OK;
+}
+
 int tgs_run_batch_noalloc(tgs_ctx *c, const uint32_t *cycles, uint32_t n, int *anyStopped) {
   int rc = checkLoaded(c);
   if (rc) return rc;
@@ -1504,6 +1676,7 @@ int tgs_run_batch_noalloc(tgs_ctx *c, const uint32_t *cycles, uint32_t n, int *a
   sc.cycleBase = reinterpret_cast<uint64_t *>(base + kScratchBaseOff);
   sc.barrier = reinterpret_cast<uint32_t *>(base + kScratchBarrierOff);
   sc.groupsActive = reinterpret_cast<uint32_t *>(base + kScratchActiveOff);
+  sc.args = reinterpret_cast<BatchArgs *>(base + kScratchArgsOff);
   sc.fixedCycles = kScratchCycles;
   if ((rc = runBatched(c, cycles, &sc))) return rc;
   auto &st = c->bhStop;

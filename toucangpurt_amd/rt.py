@@ -251,6 +251,12 @@ class Simulator:
         ids = np.ascontiguousarray(input_ids, dtype=np.uint32)
         self._check(self._lib.tgs_set_stimulus(self._h, ids.ctypes.data_as(_u32p), ids.size, int(seed)))
 
+    def device_bytes(self):
+        """Device memory the context holds now (tgs_device_bytes)."""
+        v = ctypes.c_uint64()
+        self._check(self._lib.tgs_device_bytes(self._h, ctypes.byref(v)))
+        return int(v.value)
+
     def run_batch(self, cycles):
         c = np.ascontiguousarray(cycles, dtype=np.uint32)
         st = ctypes.c_int()
@@ -413,3 +419,26 @@ def parse_symbols(blob: bytes) -> dict:
     if pos != len(blob):
         raise ValueError("trailing bytes in symbol file")
     return out
+
+
+def run_batch_multi(sims, cycles):
+    """tgs_run_batch_multi: run_batch of several batched simulators of one
+    device in shared launches (small batches of different designs occupy the
+    GPU together). cycles[k] holds one count per instance of sims[k].
+    Returns each simulator's any-stopped flag."""
+    n = len(sims)
+    if n == 0:
+        return []
+    arrs = [np.ascontiguousarray(c, dtype=np.uint32) for c in cycles]
+    if len(arrs) != n:
+        raise ValueError("one cycle array per simulator")
+    lib = sims[0]._lib
+    hs = (ctypes.c_void_p * n)(*[s._h for s in sims])
+    cs = (_u32p * n)(*[a.ctypes.data_as(_u32p) for a in arrs])
+    st = (ctypes.c_int * n)()
+    before = [s._err() for s in sims]
+    rc = lib.tgs_run_batch_multi(hs, n, cs, st)
+    if rc != TGS_OK:  # the context that failed keeps the error text
+        after = [s._err() for s in sims]
+        raise SimError(rc, "; ".join(a for a, b in zip(after, before) if a and a != b) or after[0])
+    return [bool(v) for v in st]

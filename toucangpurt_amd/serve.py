@@ -11,20 +11,27 @@ The engine keeps one batched context per design resident in HBM (program
 image plus instance-minor state for a fixed number of instance slots) and,
 optionally, a second context per design on the streamed single-instance
 kernel for jobs of at most `small_jobs` instances. A batched launch-cycle
-costs about the same for 1 or 192 instances (one wave per 64-instance group
-walks a partition's tasks in series; about 1 ms on 200K LUT4 partitioned by
-256), while the streamed kernel spreads one instance's partition over a
-workgroup (64 us per cycle on the same design): on a lightly loaded GPU a
-small job finishes much sooner on it. Under load the batched context, which
+of a few instances costs a good part of one of many (each wave walks a
+partition's tasks in series for its group of instances: about 0.17 ms for
+64 instances and 1.0 ms for 1,024 on 200K LUT4 partitioned by 256), while
+the streamed kernel spreads one instance's partition over a workgroup
+(about 52 us per cycle on the same design): on a lightly loaded GPU a small
+job finishes sooner on it. Under load the batched context, which
 shares every program read among all its instances, gives the higher
 throughput, so `small_jobs` is 0 (off) by default. The engine runs in
 slices: every step makes one run per context with active slots, each active
-slot advancing min(remaining, slice) cycles.
+slot advancing min(remaining, slice) cycles; the slices of all batched
+contexts run in shared launches (tgs_run_batch_multi: every launch cycle
+runs region 0 of every design, then region 1), so the small batches of
+different designs occupy the GPU together.
 Requests that arrive while others run are admitted at the next step into
 slots that finished jobs freed, so jobs of one design overlap in the same
-launch and jobs of different designs alternate launch by launch. Admission
-is by slots and by an HBM budget: creating a design context charges its
-program image and its slot state; when the budget is short, idle design
+launch, and so do jobs of different designs. Admission
+is by slots and by an HBM budget: a design context is charged an estimate
+of its program image and slot state when it is created and, from its
+first run on, the device memory the runtime reports it holds
+(tgs_device_bytes; the estimate for later contexts is scaled by the
+largest ratio seen); when the budget is short, idle design
 contexts are evicted (least recently used) before a job waits. A job that
 has waited longer than `park_after` seconds for a context that does not fit
 parks the jobs of the least recently used busy context: each of their
@@ -84,6 +91,18 @@ def gpu_backend(design, kernel=KERNEL_BATCH):
     return sim
 
 
+def run_shared(sims, cycles):
+    """One slice of several batched contexts: in shared launches
+    (tgs_run_batch_multi) when they are device contexts, else one by one."""
+    from . import rt
+    if len(sims) > 1 and all(isinstance(s, rt.Simulator) for s in sims):
+        rt.run_batch_multi(sims, cycles)
+        return 1
+    for s, c in zip(sims, cycles):
+        s.run_batch(c)
+    return len(sims)
+
+
 class _Ctx:
     """One resident context of a design: `capacity` instance slots of one kind
     ("batch": the batched kernel; "small": the streamed kernel)."""
@@ -108,11 +127,13 @@ class Engine:
     """
 
     def __init__(self, designs, budget=250 << 30, capacity=128, slice_cycles=2000, backend=gpu_backend,
-                 clock=time.perf_counter, park_after=1.0, small_jobs=0, small_capacity=64):
+                 clock=time.perf_counter, park_after=1.0, small_jobs=0, small_capacity=64, shared=True):
         if small_jobs > small_capacity:
             # a job routed to the small context could never find enough free slots
             raise ValueError(f"small_jobs ({small_jobs}) exceeds small_capacity ({small_capacity})")
         self.designs = designs
+        self.shared = shared  # batched contexts' slices in shared launches
+        self.lut_cycles = 0   # instance-cycles weighted by the design's LUT4 count
         self.budget = budget
         self.park_after = park_after
         self.small_jobs = small_jobs
@@ -129,6 +150,7 @@ class Engine:
         self.steps = 0
         self.launches = 0
         self.parked = 0
+        self._ratio = 1.0  # device bytes / estimate, largest seen
         self._t0 = None
 
     # ---- HBM accounting -------------------------------------------------
@@ -139,9 +161,30 @@ class Engine:
         return int(st["reg_pool"]) + int(st["mem_pool"]) + int(st["exchange_pool"]) + 64
 
     def context_bytes(self, design, capacity=None):
-        """Program image (about the serialized size) plus all slots' state."""
+        """Estimate before a context exists: program image (about the
+        serialized size) plus all slots' state, scaled by the largest ratio
+        of device bytes to this estimate seen on the contexts so far."""
+        return int(self._estimate(design, capacity) * self._ratio)
+
+    def _estimate(self, design, capacity=None):
         cap = self.capacity if capacity is None else capacity
         return int(design.stats["design_bytes"]) + cap * self.state_bytes(design)
+
+    def calibrate(self, design, device_bytes, capacity=None):
+        """Scale later estimates by a context of `design` measured at
+        `device_bytes` (tgs_device_bytes), as after a context's first run."""
+        self._ratio = max(self._ratio, device_bytes / max(self._estimate(design, capacity), 1))
+
+    def _charge(self, c):
+        """Charge a context the device memory it holds (tgs_device_bytes),
+        once the runtime reports it; the estimate stands until then."""
+        q = getattr(c.sim, "device_bytes", None)
+        if q is None:
+            return
+        actual = q()
+        self.calibrate(c.design, actual, c.capacity)
+        self.used += actual - c.nbytes
+        c.nbytes = actual
 
     def kind_of(self, job):
         return "small" if self.small_jobs and job.instances <= self.small_jobs else "batch"
@@ -164,9 +207,12 @@ class Engine:
             return c
         d = self.designs[name]
         cap = self.small_capacity if kind == "small" else self.capacity
-        nbytes = self.context_bytes(d, cap)
-        if nbytes > self.budget:
-            raise ValueError(f"design {name} alone needs {nbytes} B, more than the budget {self.budget} B")
+        if self._estimate(d, cap) > self.budget:
+            raise ValueError(f"design {name} alone needs {self._estimate(d, cap)} B, more than the budget "
+                             f"{self.budget} B")
+        # the scaled estimate, at most the whole budget (a context that
+        # alone fills it runs when everything else is evicted or parked)
+        nbytes = min(self.context_bytes(d, cap), self.budget)
         if not self._make_room(nbytes, now):
             return None
         sim = self.backend(d, KERNEL_STREAM if kind == "small" else KERNEL_BATCH)
@@ -241,19 +287,30 @@ class Engine:
             self._t0 = self.clock()
         now = self.clock() - self._t0
         self._admit(now)
-        for c in list(self.ctx.values()):
-            if not c.jobs:
-                continue
+        live = [c for c in self.ctx.values() if c.jobs]
+        cycs = []
+        for c in live:
             cyc = np.zeros(c.capacity, np.uint32)
             for job in c.jobs:
                 n = np.minimum(job.remaining, self.slice)
                 cyc[job.slots] = n
                 job.remaining -= n
-            c.sim.run_batch(cyc)
-            self.launches += 1
+            cycs.append(cyc)
             self.instance_cycles += int(cyc.sum())
+            self.lut_cycles += int(cyc.sum()) * int(c.design.stats.get("n_luts", 0))
+        # every batched context's slice in shared launches, small-job
+        # (streamed-kernel) contexts one by one
+        batch = [k for k, c in enumerate(live) if c.key[1] == "batch"] if self.shared else []
+        if batch:
+            self.launches += run_shared([live[k].sim for k in batch], [cycs[k] for k in batch])
+        for k, c in enumerate(live):
+            if k not in batch:
+                c.sim.run_batch(cycs[k])
+                self.launches += 1
+        for c in live:
             t = self.clock() - self._t0
             c.last_used = t
+            self._charge(c)  # images are built on a context's first run
             keep = []
             for job in c.jobs:
                 if job.remaining.max() > 0:
@@ -300,6 +357,8 @@ class Engine:
             "instance_cycles": self.instance_cycles,
             "seconds": wall,
             "cycles_per_s": self.instance_cycles / wall if wall > 0 else None,
+            "lut4_cycles_per_s": self.lut_cycles / wall if wall > 0 else None,
+            "shared_launches": self.shared,
             "launches": self.launches,
             "parked": self.parked,
             "steps": self.steps,
