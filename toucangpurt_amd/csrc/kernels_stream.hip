@@ -22,6 +22,7 @@
 //     rows are never resident, so partitions of any size fit and several
 //     workgroups share a CU;
 //   * one atomic-counter grid barrier per region.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include "kernel_args.h"
@@ -385,8 +386,12 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
         loadRows<NC>(rs, recOf(j + 2 + kLoaders), W, lane);
       }
       if (recOf(j)[0] == S_REGION) {
-        __syncthreads();  // the two workgroup barriers of the grid barrier
-        __syncthreads();
+        if (c.gridSync) {
+          cooperative_groups::this_grid().sync();
+        } else {
+          __syncthreads();  // the two workgroup barriers of the grid barrier
+          __syncthreads();
+        }
       } else {
         __syncthreads();
       }
@@ -467,7 +472,14 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
         c.profOwner[part] = g;
       }
     } else {
-      barTicks += gridBarrier(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+      if (c.gridSync) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        cooperative_groups::this_grid().sync();
+        if (tid == 0) barTicks += __builtin_amdgcn_s_memrealtime() - t0;
+        ++bar;
+      } else {
+        barTicks += gridBarrier(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+      }
       if (pc) {
         if (tid == 0) c.profWg[2 * g + region] += shaderClock() - regionT0;
         regionT0 = shaderClock();
