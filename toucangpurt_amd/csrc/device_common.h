@@ -1,6 +1,5 @@
 // Device helpers shared by the gfx950 kernels.
 #pragma once
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -51,10 +50,6 @@ struct CommonArgs {
   uint64_t *profUseful, *profWg;
   uint32_t *profOwner;  // per partition: the workgroup that ran it
   uint32_t profStart, profCount;
-  // grid barrier of the persistent kernels: 0 the atomic arrival counter
-  // (gridBarrier), 1 cooperative groups' grid sync (every thread of the
-  // grid calls it; kept for the measured comparison)
-  uint32_t gridSync;
 };
 
 // Whether launch-local cycle `cyc` is inside the profile window.
@@ -237,17 +232,6 @@ __device__ __forceinline__ uint64_t gridBarrier(uint32_t *counter, uint32_t targ
   }
   __syncthreads();
   return waited;
-}
-
-// The persistent kernels' grid barrier as a.gridSync chooses (every thread
-// of the grid calls it); returns the ticks thread 0 of the workgroup waited.
-__device__ __forceinline__ uint64_t gridSyncChosen(const CommonArgs &a, uint32_t target) {
-  if (a.gridSync) {
-    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-    cooperative_groups::this_grid().sync();
-    return threadIdx.x == 0 ? __builtin_amdgcn_s_memrealtime() - t0 : 0;
-  }
-  return gridBarrier(&a.ctl[C_BARRIER], target, true);
 }
 
 }  // namespace tgs

@@ -22,7 +22,6 @@
 //     rows are never resident, so partitions of any size fit and several
 //     workgroups share a CU;
 //   * one atomic-counter grid barrier per region.
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include "kernel_args.h"
@@ -339,7 +338,9 @@ constexpr uint32_t kLoaders = 3;
 // At least six waves per SIMD (HIP's second launch bound): two NC = 8 or
 // three NC = 4 workgroups per CU, which keeps every partition slot of a
 // 1M-LUT4 design co-resident.
-template <int NC>
+// PROF: the per-partition profile's hooks are compiled in (the runtime
+// launches this instantiation only while the profile is on).
+template <int NC, bool PROF>
 __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamArgs a) {
   extern __shared__ __align__(16) uint8_t sLds[];
   constexpr uint32_t nTc = NC * 64;
@@ -386,12 +387,8 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
         loadRows<NC>(rs, recOf(j + 2 + kLoaders), W, lane);
       }
       if (recOf(j)[0] == S_REGION) {
-        if (c.gridSync) {
-          cooperative_groups::this_grid().sync();
-        } else {
-          __syncthreads();  // the two workgroup barriers of the grid barrier
-          __syncthreads();
-        }
+        __syncthreads();  // the two workgroup barriers of the grid barrier
+        __syncthreads();
       } else {
         __syncthreads();
       }
@@ -443,9 +440,11 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
       __syncthreads();
       if (g == 0)
         for (uint32_t k = tid; k < c.nIn; k += nTc) c.reg[c.inIds[k]] = uint8_t(stimBit(draws, k));
-      pc = profCycle(c, cyc);
-      region = 0;
-      if (pc) regionT0 = shaderClock();
+      if (PROF) {
+        pc = profCycle(c, cyc);
+        region = 0;
+        if (pc) regionT0 = shaderClock();
+      }
     }
     readStep(nxt, j + 1);
     const uint32_t type = cur.ey & 15u, kind = (cur.ey >> 4) & 7u;
@@ -466,21 +465,14 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     } else if (type == S_PEND) {
       partEnd(cur, c, W, vp, c.cycleBase + cyc, ts, nTs);
       __syncthreads();
-      if (pc && ts == 0 && kind != kKindNone) {
+      if (PROF && pc && ts == 0 && kind != kKindNone) {
         const uint32_t part = __builtin_amdgcn_readlane(cur.r0, S_PART_ID);
         c.profUseful[part] += shaderClock() - regionT0;
         c.profOwner[part] = g;
       }
     } else {
-      if (c.gridSync) {
-        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        cooperative_groups::this_grid().sync();
-        if (tid == 0) barTicks += __builtin_amdgcn_s_memrealtime() - t0;
-        ++bar;
-      } else {
-        barTicks += gridBarrier(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
-      }
-      if (pc) {
+      barTicks += gridBarrier(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+      if (PROF && pc) {
         if (tid == 0) c.profWg[2 * g + region] += shaderClock() - regionT0;
         regionT0 = shaderClock();
         region ^= 1u;
@@ -500,10 +492,14 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
 }
 
 namespace {
-const void *streamFn(uint32_t nc) {
-  return nc == 2 ? reinterpret_cast<const void *>(streamCycles<2>)
-         : nc == 8 ? reinterpret_cast<const void *>(streamCycles<8>)
-                   : reinterpret_cast<const void *>(streamCycles<4>);
+const void *streamFn(uint32_t nc, bool prof = false) {
+  if (prof)
+    return nc == 2 ? reinterpret_cast<const void *>(streamCycles<2, true>)
+           : nc == 8 ? reinterpret_cast<const void *>(streamCycles<8, true>)
+                     : reinterpret_cast<const void *>(streamCycles<4, true>);
+  return nc == 2 ? reinterpret_cast<const void *>(streamCycles<2, false>)
+         : nc == 8 ? reinterpret_cast<const void *>(streamCycles<8, false>)
+                   : reinterpret_cast<const void *>(streamCycles<4, false>);
 }
 }  // namespace
 
@@ -521,18 +517,27 @@ uint32_t streamBlock(uint32_t nc) { return (nc + kLoaders) * 64; }
 hipError_t launchStream(const StreamArgs &args, uint32_t nc, uint32_t grid, uint32_t lds, hipStream_t stream) {
   StreamArgs a = args;
   void *params[] = {&a};
-  return hipLaunchCooperativeKernel(streamFn(nc), dim3(grid), dim3((nc + kLoaders) * 64), params, lds, stream);
+  return hipLaunchCooperativeKernel(streamFn(nc, args.c.profUseful != nullptr), dim3(grid),
+                                    dim3((nc + kLoaders) * 64), params, lds, stream);
 }
 
+// The smaller of the two instantiations' figures, so a grid sized by it is
+// co-resident with the profile on or off.
 hipError_t streamMaxBlocksPerCU(int *blocks, uint32_t nc, uint32_t lds) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, streamFn(nc), (nc + kLoaders) * 64, lds);
+  int b0 = 0, b1 = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b0, streamFn(nc, false), (nc + kLoaders) * 64, lds);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, streamFn(nc, true), (nc + kLoaders) * 64, lds);
+  *blocks = std::min(b0, b1);
+  return e;
 }
 
 hipError_t streamSetLds(uint32_t lds) {
-  for (uint32_t nc : {2u, 4u, 8u}) {
-    hipError_t e = hipFuncSetAttribute(streamFn(nc), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-    if (e != hipSuccess) return e;
-  }
+  for (uint32_t nc : {2u, 4u, 8u})
+    for (bool prof : {false, true}) {
+      hipError_t e = hipFuncSetAttribute(streamFn(nc, prof), hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+      if (e != hipSuccess) return e;
+    }
   return hipSuccess;
 }
 

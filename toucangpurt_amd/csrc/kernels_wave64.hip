@@ -286,14 +286,14 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
       w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
       partDone(a.schedPart[s], t0, pc);
     }
-    barTicks += gridSyncChosen(a.c, (++bar) * gridDim.x);
+    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
     if (pc && tid == 0) a.c.profWg[2 * g] += shaderClock() - t0;
     t0 = pc ? shaderClock() : 0;
     for (uint32_t s = s1; s < s2; ++s) {
       w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
       partDone(a.schedPart[s], t0, pc);
     }
-    barTicks += gridSyncChosen(a.c, (++bar) * gridDim.x);
+    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
     if (pc && tid == 0) a.c.profWg[2 * g + 1] += shaderClock() - t0;
   }
   if (tid == 0) {

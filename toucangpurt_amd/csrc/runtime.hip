@@ -519,10 +519,6 @@ CommonArgs commonArgs(tgs_ctx *c, Instance &in, uint32_t cycles) {
     a.profStart = c->profStart;
     a.profCount = c->profCount;
   }
-  // TGS_GRID_SYNC=cg: cooperative groups' grid sync instead of the atomic
-  // arrival counter in the persistent kernels (the measured comparison)
-  const char *gs = std::getenv("TGS_GRID_SYNC");
-  a.gridSync = gs && std::strcmp(gs, "cg") == 0 ? 1u : 0u;
   return a;
 }
 
