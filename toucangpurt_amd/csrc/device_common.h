@@ -234,4 +234,42 @@ __device__ __forceinline__ uint64_t gridBarrier(uint32_t *counter, uint32_t targ
   return waited;
 }
 
+// Cross-workgroup pools (register, exchange and memory pools) accessed with
+// agent-coherent loads and stores: relaxed agent-scope atomics, which gfx950
+// performs with sc1 (a store is written through to the device coherence
+// point, a load does not hit another XCD's stale L2 line). A kernel that
+// touches those pools only this way can use gridBarrierShared.
+__device__ __forceinline__ uint32_t ldShared8(const uint8_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stShared8(uint8_t *p, uint32_t v) {
+  __hip_atomic_store(p, uint8_t(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stShared32(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier for a kernel whose data shared between workgroups moves only
+// through ldShared8 / stShared8 / stShared32: every wave waits for its own
+// stores to complete (vmcnt(0); a completed sc1 store is visible device-wide),
+// the workgroup synchronizes, one thread arrives and spins on the counter. No
+// L2 write-back before the arrival and no invalidate after it: gridBarrier's
+// agent-scope release writes back every dirty line of the XCD's L2, which on
+// the 1M-LUT4 design held the last arrival's counter increment back by 5-10 us
+// per region. `target` = (barrier index + 1) * gridDim.x.
+__device__ __forceinline__ uint64_t gridBarrierShared(uint32_t *counter, uint32_t target, bool measure) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __syncthreads();
+  uint64_t waited = 0;
+  if (threadIdx.x == 0) {
+    const uint64_t t0 = measure ? __builtin_amdgcn_s_memrealtime() : 0;
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(1);
+    if (measure) waited = __builtin_amdgcn_s_memrealtime() - t0;
+  }
+  __syncthreads();
+  return waited;
+}
+
 }  // namespace tgs

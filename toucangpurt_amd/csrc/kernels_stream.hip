@@ -21,7 +21,9 @@
 //   * the LUT, the value pool and the stimulus draws live in LDS, program
 //     rows are never resident, so partitions of any size fit and several
 //     workgroups share a CU;
-//   * one atomic-counter grid barrier per region.
+//   * one atomic-counter grid barrier per region; the register, exchange and
+//     memory pools (the only data workgroups exchange) are read and written
+//     with agent-coherent accesses, so the barrier needs no L2 write-back.
 #include <hip/hip_runtime.h>
 
 #include "kernel_args.h"
@@ -178,14 +180,14 @@ __device__ __forceinline__ void memReadS(const Pf &p, uint8_t *vp, const CommonA
   const uint32_t en = rdOpS(vp, p.r2 & 0xFFFFu);
   if (en == 0) return;
   const uint64_t at = memAddress(vp, p.r2 >> 16, p.r3 >> 16, base);
-  vp[p.r3 & 0xFFFFu] = at < c.memSize ? c.mem[at] : uint8_t(0);
+  vp[p.r3 & 0xFFFFu] = at < c.memSize ? uint8_t(ldShared8(c.mem + at)) : uint8_t(0);
 }
 
 // L0 source (register, exchange or stimulus input).
 __device__ __forceinline__ uint32_t l0Value(const CommonArgs &c, const uint64_t *draws, uint32_t src) {
   const uint32_t kind = src >> 30, idx = src & 0x3FFFFFFFu;
-  if (kind == 0) return c.reg[idx];
-  if (kind == 1) return c.exg[idx];
+  if (kind == 0) return ldShared8(c.reg + idx);
+  if (kind == 1) return ldShared8(c.exg + idx);
   if (kind == 2) return stimBit(draws, idx);
   return 0;
 }
@@ -251,7 +253,7 @@ __device__ __forceinline__ void partEnd(const Pf &p, const CommonArgs &c, const 
     const uint32_t exg = __builtin_amdgcn_readlane(p.r0, S_EW_EXG), dat = ewdc & 0xFFFFu;
     const uint32_t words = ((ewdc >> 16) + 3) / 4;
     for (uint32_t k = tid; k < words; k += nTc)
-      reinterpret_cast<uint32_t *>(c.exg)[(exg >> 2) + k] = reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k];
+      stShared32(reinterpret_cast<uint32_t *>(c.exg) + (exg >> 2) + k, reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k]);
     return;
   }
   const uint32_t rwdc = __builtin_amdgcn_readlane(p.r0, S_RW_DC);
@@ -259,7 +261,7 @@ __device__ __forceinline__ void partEnd(const Pf &p, const CommonArgs &c, const 
     const uint32_t reg = __builtin_amdgcn_readlane(p.r0, S_RW_REG), dat = rwdc & 0xFFFFu;
     const uint32_t words = ((rwdc >> 16) + 3) / 4;
     for (uint32_t k = tid; k < words; k += nTc)
-      reinterpret_cast<uint32_t *>(c.reg)[(reg >> 2) + k] = reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k];
+      stShared32(reinterpret_cast<uint32_t *>(c.reg) + (reg >> 2) + k, reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k]);
   }
   const uint32_t nmw = __builtin_amdgcn_readlane(p.r0, S_NMW), npr = __builtin_amdgcn_readlane(p.r0, S_NPR);
   const uint32_t nst = __builtin_amdgcn_readlane(p.r0, S_NST);
@@ -271,7 +273,7 @@ __device__ __forceinline__ void partEnd(const Pf &p, const CommonArgs &c, const 
       if (en != 0) {
         const uint32_t dat = rdOpS(vp, o[2] >> 16);
         const uint64_t at = memAddress(vp, o[2] & 0xFFFFu, o[3] >> 16, uint64_t(o[0]) | uint64_t(o[1]) << 32);
-        if (at < c.memSize) c.mem[at] = uint8_t(dat);
+        if (at < c.memSize) stShared8(c.mem + at, dat);
       }
     }
     L += 4 * nmw;
@@ -439,7 +441,7 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
       if (tid == 0) stimulusDraws(xs, c.nIn, draws);
       __syncthreads();
       if (g == 0)
-        for (uint32_t k = tid; k < c.nIn; k += nTc) c.reg[c.inIds[k]] = uint8_t(stimBit(draws, k));
+        for (uint32_t k = tid; k < c.nIn; k += nTc) stShared8(c.reg + c.inIds[k], stimBit(draws, k));
       if (PROF) {
         pc = profCycle(c, cyc);
         region = 0;
@@ -471,7 +473,7 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
         c.profOwner[part] = g;
       }
     } else {
-      barTicks += gridBarrier(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+      barTicks += gridBarrierShared(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
       if (PROF && pc) {
         if (tid == 0) c.profWg[2 * g + region] += shaderClock() - regionT0;
         regionT0 = shaderClock();
