@@ -115,20 +115,28 @@ void runRows(Model &m, Area &A) {
     }
 }
 
-// T_LUTFAST, as lutFastTask: byte offsets from the lane's column.
+// T_LUTFAST, as lutFastTask: byte offsets from the lane's column; a lane
+// holds four instances (32-bit words of the rows), so a round is 256 / I ops
+// and every offset must be a multiple of 4.
 void runFast(Model &m, Area &A) {
-  const uint32_t I = A.I, W = 64 / I, V = A.V;
+  const uint32_t I = A.I, W = 256 / I, V = A.V;
   const uint32_t w0 = m.word(0), w1 = m.word(1);
   const uint32_t nMid = (w0 >> 8) & 0xFFFFu, nt = w1 & 0xFFFFu, nl = w1 >> 16;
   const uint32_t topOff = 4 + ((((nMid + 3) >> 2) + 1) & ~1u);
   const uint32_t midOff = m.word(2), lastOff = m.word(3);
   auto sizeOf = [&](uint32_t l) { return (m.word(4 + (l >> 2)) >> ((l & 3u) * 8)) & 0xFFu; };
+  auto aligned = [&](uint64_t off) {
+    if (off % 4) die("T_LUTFAST offset not a multiple of 4: " + std::to_string(off));
+  };
   auto nib = [&](uint64_t off) -> uint32_t {
     const uint32_t v = A.at(off, "operand");
     if (v > 15) die("operand above 15 at offset " + std::to_string(off));
     return v;
   };
   auto eval = [&](uint32_t x, uint32_t y, uint32_t ii) -> uint32_t {
+    aligned(x >> 16);
+    aligned(y & 0xFFFFu);
+    aligned(y >> 16);
     const uint32_t pos = (((nib(ii + (x >> 16)) << 4) | nib(ii + (y & 0xFFFFu))) << 4 | nib(ii + (y >> 16))) +
                          (x & 0xFFFFu);
     if (pos >= m.lut.size()) die("LUT position past the region: " + std::to_string(pos));
@@ -143,21 +151,23 @@ void runFast(Model &m, Area &A) {
     const uint64_t next = l == 0 ? midOff : cur + 2ull * n;
     const bool lastMid = l > 0 && l == nMid;
     const uint32_t wb = (l == 0 || ((l - 1) & 1u)) ? 0u : kFastBufY;
-    if (n % W) die("task level not padded to whole rounds");
+    if (n > 64) die("task level of more than 64 ops");
     for (uint32_t r = 0; r < n; r += 2 * W) {
       wr.clear();
       for (uint32_t half = 0; half < (r + W < n ? 2u : 1u); ++half)
         for (uint32_t sub = 0; sub < W; ++sub) {
-          const uint32_t q = r + half * W + sub;
+          const uint32_t lr = r + half * W + sub;  // the lane's row; past the level's end it takes the last op
+          const uint32_t q = std::min(lr, n - 1);
           const uint32_t x = m.word(cur + 2 * q), y = m.word(cur + 2 * q + 1);
           uint64_t dst;
           if (lastMid) {
             const uint32_t d = m.word(cur + 2ull * n + q / 2);
             dst = (q & 1u) ? d >> 16 : d & 0xFFFFu;
           } else {
-            if (q >= kFastBufY || wb + q >= kConstRow) die("level row past its level buffer");
-            dst = uint64_t(V + wb + q) * I;
+            if (lr >= kFastBufY || wb + lr >= kConstRow) die("level row past its level buffer");
+            dst = uint64_t(V + wb + lr) * I;
           }
+          aligned(dst);
           for (uint32_t ii = 0; ii < I; ++ii) wr.push_back({dst + ii, eval(x, y, ii)});
         }
       for (auto &w : wr) A.at(w.off, "write") = uint8_t(w.v);
@@ -165,10 +175,11 @@ void runFast(Model &m, Area &A) {
     cur = next;
     n = nn;
   }
-  if (nl % W) die("last ops not padded to whole rounds");
   for (uint32_t r = 0; r < nl; r += W)
     for (uint32_t sub = 0; sub < W; ++sub) {
-      const uint32_t w = m.word(lastOff + r + sub);
+      const uint32_t w = m.word(lastOff + std::min(r + sub, nl - 1));
+      aligned(w & 0xFFFFu);
+      aligned(w >> 16);
       for (uint32_t ii = 0; ii < I; ++ii) A.at((w & 0xFFFFu) + ii, "last write") = uint8_t(nib((w >> 16) + ii));
     }
 }

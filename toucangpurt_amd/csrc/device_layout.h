@@ -77,7 +77,8 @@ enum W64Task : uint32_t { T_LUT2 = 0, T_VECREAD = 1, T_VECOP = 2, T_MEMREAD = 3,
 //   last ops, 1 word each: value-pool slot | source level row << 16
 constexpr uint32_t kLevelRows = 176, kConstRow = 144;
 // T_LUTFAST's level buffers: X at level row 0, Y at kFastBufY, each with
-// room for 64 ops plus a round's padding (at most 7 ops)
+// room for 64 ops (a kernel lane past a level's end writes a slack row below
+// 64: rounds are 256 / I ops, which divides 64)
 constexpr uint32_t kFastBufY = 72;
 // T_LUTFAST (batched image only, built for one instance count I per wave):
 // LUT micropartitions of one exec level packed into one task, with every
@@ -87,10 +88,8 @@ constexpr uint32_t kFastBufY = 72;
 // check. Parts with fewer middle levels are aligned at the end: with nMid
 // the largest count, a part with m middle levels has its top level on task
 // level nMid - m and its last middle level on task level nMid; at most 64
-// ops on a task level, each task level padded to a multiple of the round
-// width 64 / I with copies of its last op (a copy writes the slack row
-// after the level's ops, or the same destination on task level nMid), and
-// the last ops padded the same way. Only for programs in which every value is a nibble
+// ops on a task level (levels are not padded: the kernel clamps a lane's op
+// index to the level's last op). Only for programs in which every value is a nibble
 // (nibbleProgram), and only for parts whose ops satisfy: top-op LUT index 0
 // or <= 65536 - 4096 (no 16-bit wrap), no middle-op LUT operand an
 // immediate >= 16 (operands <= 15 keep the index <= 0xFFF).

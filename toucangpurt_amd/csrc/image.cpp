@@ -563,8 +563,6 @@ static bool lutFastTask(const std::vector<const CGMicroPartInfo *> &H, uint32_t 
   }
   auto off = [&](uint32_t row) -> uint32_t { return row * I; };
   const uint32_t zeroRow = V + kConstRow;
-  const uint32_t W = 64 / I;  // ops per round: levels are padded to a multiple
-  auto padded = [&](uint32_t n) -> uint32_t { return (n + W - 1) / W * W; };
   // level buffer written by task level L (0: X; then alternately Y, X, ...)
   auto bufOf = [](uint32_t L) -> uint32_t { return L == 0 ? 0u : ((L - 1) & 1u) ? 0u : kFastBufY; };
   // row read through reference lane o by an op of part h whose sources are
@@ -585,7 +583,7 @@ static bool lutFastTask(const std::vector<const CGMicroPartInfo *> &H, uint32_t 
   t.push_back(0);
   for (uint32_t l = 0; l < nMid; l += 4) {
     uint32_t v = 0;
-    for (uint32_t k = 0; k < 4 && l + k < nMid; ++k) v |= padded(tot[l + k + 1]) << (8 * k);
+    for (uint32_t k = 0; k < 4 && l + k < nMid; ++k) v |= tot[l + k + 1] << (8 * k);
     t.push_back(v);
   }
   if (t.size() & 1) t.push_back(0);
@@ -635,13 +633,6 @@ static bool lutFastTask(const std::vector<const CGMicroPartInfo *> &H, uint32_t 
         }
       }
     }
-    // padding: copies of the level's last op (and of its destination)
-    for (uint32_t k = tot[L]; k < padded(tot[L]); ++k) {
-      const uint32_t x = t[t.size() - 2], y = t[t.size() - 1];
-      t.push_back(x);
-      t.push_back(y);
-      if (L == nMid && L > 0) dsts.push_back(dsts.back());
-    }
     for (size_t j = 0; j < dsts.size(); j += 2)
       t.push_back(dsts[j] | (j + 1 < dsts.size() ? dsts[j + 1] : dsts[j]) << 16);
   }
@@ -661,8 +652,7 @@ static bool lutFastTask(const std::vector<const CGMicroPartInfo *> &H, uint32_t 
       ++nl;
     }
   }
-  for (; nl % W; ++nl) t.push_back(t.back());
-  t[1] = padded(tot[0]) | nl << 16;
+  t[1] = tot[0] | nl << 16;
   if (t.size() & 1) t.push_back(0);
   return true;
 }

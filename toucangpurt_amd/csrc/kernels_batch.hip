@@ -3,12 +3,15 @@
 //
 // Semantics per instance are the reference evaluator's (src/SimEval.cu:304-761,
 // :765-864, :889-965). The execution is laid out for 64-wide waves:
-//   * a wave serves a group of I instances (I = 16, 32 or 64) and W = 64 / I
+//   * a wave serves a group of I instances (I = 8, 16, 32 or 64) and W = 64 / I
 //     ops at once: lane l evaluates op sub = l / I of the current round for
 //     instance ii = l % I. A round of W ops costs what one op costs when
 //     every lane is an instance, so small I cuts the latency of one
 //     (partition, group) item by W and its LDS by W, which lets W times more
-//     waves share a CU;
+//     waves share a CU. The LUT tasks of nibble programs (T_LUTFAST) go
+//     further: a lane holds four instances in the bytes of 32-bit LDS words,
+//     so a round is 256 / I ops and an op's program words, operand reads and
+//     write are shared by four instances;
 //   * every pool is instance-minor (nibble k of slot s at k * instPad + s),
 //     so a group's nibble k is I contiguous bytes, moved 16 bytes per lane by
 //     the L0 reads and the last-level word copies;
@@ -163,90 +166,128 @@ __device__ __forceinline__ void lutTask(const Grp<I> &g, const uint32_t *T) {
   }
 }
 
-// LUT position of a T_LUTFAST op: (a << 4 | b) << 4 | c plus the low half
-// of the op's first word, in three VALU instructions (two shift-ors and one
-// add of a selected 16-bit half), which instruction selection does not
-// find on its own for this expression.
-__device__ __forceinline__ uint32_t lutPos(uint32_t w, uint32_t a, uint32_t b, uint32_t c) {
-  uint32_t t, idx, pos;
-  asm("v_lshl_or_b32 %0, %1, 4, %2" : "=v"(t) : "v"(a), "v"(b));
-  asm("v_lshl_or_b32 %0, %1, 4, %2" : "=v"(idx) : "v"(t), "v"(c));
-  asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:WORD_0"
+// The four LUT indices of a T_LUTFAST op for four instances whose operand
+// nibbles sit in the bytes of A, B and C (byte k = instance k): index k =
+// a_k << 8 | b_k << 4 | c_k. B << 4 | C puts b_k << 4 | c_k in byte k (the
+// operands are nibbles, so no byte carries into the next); two byte
+// permutes then interleave those bytes with A's, which gives index k in a
+// 16-bit half: i01 = {index 0, index 1}, i23 = {index 2, index 3}.
+struct QuadIdx {
+  uint32_t i01, i23;
+};
+__device__ __forceinline__ QuadIdx quadIndex(uint32_t A, uint32_t B, uint32_t C) {
+  uint32_t bc;
+  asm("v_lshl_or_b32 %0, %1, 4, %2" : "=v"(bc) : "v"(B), "v"(C));
+  // v_perm_b32 byte selectors: 0-3 pick bytes of the second source (bc),
+  // 4-7 bytes of the first (A)
+  return {__builtin_amdgcn_perm(A, bc, 0x05010400u), __builtin_amdgcn_perm(A, bc, 0x07030602u)};
+}
+
+// LUT position = one 16-bit half of an index pair plus the low half of the
+// op's first word (its LUT position of index 0), in one SDWA add each.
+__device__ __forceinline__ uint32_t posLo(uint32_t pair, uint32_t w) {
+  uint32_t pos;
+  asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_0 src1_sel:WORD_0"
       : "=v"(pos)
-      : "v"(idx), "v"(w));
+      : "v"(pair), "v"(w));
+  return pos;
+}
+__device__ __forceinline__ uint32_t posHi(uint32_t pair, uint32_t w) {
+  uint32_t pos;
+  asm("v_add_u32_sdwa %0, %1, %2 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:WORD_1 src1_sel:WORD_0"
+      : "=v"(pos)
+      : "v"(pair), "v"(w));
   return pos;
 }
 
-// A T_LUTFAST task: the same two micropartitions as T_LUTROWS with every
+__device__ __forceinline__ uint32_t ldsByte(uint32_t pos) {
+  return *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(pos);
+}
+
+// A T_LUTFAST task: the same two micropartitions as T_LUTROWS, with every
 // operand and destination a byte offset from the lane's column of the wave's
 // LDS area and every LUT index a position in the LDS LUT region that needs
 // no bounds check (the image builder emits it only where every value is a
-// nibble). An op is its program words, three operand reads, the index
-// (op0 << 8 | op1 << 4 | op2) plus the op's LUT position, one lookup and one
-// write; top and middle ops share that form. Level sizes, prefetch and the
-// two rounds per iteration are as in lutTask.
+// nibble). A lane holds four consecutive instances of the group: lane l
+// evaluates op sub = l / P of a round for instances 4q .. 4q + 3, q = l % P,
+// P = I / 4, so a round is W = 64 / P = 256 / I ops. An op is its program
+// words, three 32-bit operand reads (a byte per instance), four index
+// computations from those words, four lookups and one 32-bit write; top and
+// middle ops share that form. A lane past the end of a level takes the
+// level's last op: it rewrites that op's destination with the same value on
+// the last task level, and elsewhere writes level row r + sub, a slack row
+// past the level's ops that no op reads (r + sub < 64: a level has at most
+// 64 ops and W divides 64). Level sizes, prefetch and the two rounds per
+// iteration are as in lutTask.
 template <int I>
 __device__ __forceinline__ void lutFastTask(const Grp<I> &g, const uint32_t *T) {
-  constexpr uint32_t W = Grp<I>::W;
+  static_assert(I % 4 == 0, "four instances per lane");
+  constexpr uint32_t P = I / 4, W = 64 / P;
+  const uint32_t lane = threadIdx.x & 63u, q = lane % P, sub = lane / P;
   const uint32_t w0 = uni(T[0]), w1 = uni(T[1]);
   const uint32_t nMid = (w0 >> 8) & 0xFFFFu;
   const uint32_t nt = w1 & 0xFFFFu, nl = w1 >> 16;
   const uint32_t *S = T + 4;
-  // this lane's op in every round: op words at + sub, level rows at + sub * I
-  const uint2 *cur = reinterpret_cast<const uint2 *>(S + (((nMid + 3) >> 2) + 1 & ~1u)) + g.sub;
-  const uint2 *mids = reinterpret_cast<const uint2 *>(T + uni(T[2])) + g.sub;
-  const uint32_t *last = T + uni(T[3]) + g.sub;
-  uint8_t *vb = g.vp + g.ii;                        // offsets are relative to this lane's column
-  uint8_t *lb = g.lvl + g.ii + g.sub * uint32_t(I);  // level row `sub` of buffer X
+  const uint2 *cur = reinterpret_cast<const uint2 *>(S + (((nMid + 3) >> 2) + 1 & ~1u));
+  const uint2 *mids = reinterpret_cast<const uint2 *>(T + uni(T[2]));
+  const uint32_t *last = T + uni(T[3]);
+  uint8_t *vb = g.vp + 4 * q;                        // offsets are relative to this lane's column
+  uint8_t *lb = g.lvl + 4 * q + sub * uint32_t(I);  // level row `sub` of buffer X
   auto sizeOf = [&](uint32_t m) -> uint32_t { return uni((S[m >> 2] >> ((m & 3u) * 8)) & 0xFFu); };
-  // operands are nibbles here, so (op0 << 4 | op1) << 4 | op2 is the index;
+  // this lane's op in round r of a level of n >= 1 ops
+  auto opOf = [&](uint32_t r, uint32_t n) -> uint32_t { return min(r + sub, n - 1); };
+  // offsets are multiples of I (a multiple of 4), so every access is an
+  // aligned 32-bit word
+  auto rd = [&](uint32_t off) -> uint32_t { return *reinterpret_cast<const uint32_t *>(vb + off); };
+  auto wr = [&](uint8_t *p, uint32_t v) { *reinterpret_cast<uint32_t *>(p) = v; };
   // the LUT region starts at LDS address 0 (the kernel has no static LDS,
   // which the runtime checks before the first launch)
   auto eval = [&](uint2 w) -> uint32_t {
-    const uint32_t pos = lutPos(w.x, vb[w.x >> 16], vb[w.y & 0xFFFFu], vb[w.y >> 16]);
-    return *reinterpret_cast<const __attribute__((address_space(3))) uint8_t *>(pos);
+    const QuadIdx x = quadIndex(rd(w.x >> 16), rd(w.y & 0xFFFFu), rd(w.y >> 16));
+    const uint32_t r0 = ldsByte(posLo(x.i01, w.x)), r1 = ldsByte(posHi(x.i01, w.x));
+    const uint32_t r2 = ldsByte(posLo(x.i23, w.x)), r3 = ldsByte(posHi(x.i23, w.x));
+    return r0 | r1 << 8 | r2 << 16 | r3 << 24;
   };
-  // levels are padded to whole rounds: round r's op of this lane is r + sub
   const uint2 zero = make_uint2(0, 0);
   uint2 c0 = zero, c1 = zero;
   uint32_t n = nt;
-  if (n) c0 = cur[0];
-  if (n > W) c1 = cur[W];
+  if (n) c0 = cur[opOf(0, n)];
+  if (n > W) c1 = cur[opOf(W, n)];
   // level l: 0 = the first task level (writes X), then Y, X, ... (the last
   // one writes through its destination list)
   for (uint32_t l = 0; l <= nMid; ++l) {
     const uint32_t nn = l < nMid ? sizeOf(l) : 0;
     const uint2 *next = l == 0 ? mids : cur + n;
     uint2 d0 = zero, d1 = zero;
-    if (nn) d0 = next[0];  // next level's first words, read before this level's chain
-    if (nn > W) d1 = next[W];
+    if (nn) d0 = next[opOf(0, nn)];  // next level's first words, read before this level's chain
+    if (nn > W) d1 = next[opOf(W, nn)];
     // rounds of W ops: two per iteration while a second round remains, then
     // at most one more
     if (n) {
       if (l > 0 && l == nMid) {
-        const uint16_t *dst = reinterpret_cast<const uint16_t *>(cur - g.sub + n) + g.sub;
+        const uint16_t *dst = reinterpret_cast<const uint16_t *>(cur + n);
         uint32_t r = 0;
         for (; r + W < n; r += 2 * W) {
-          const uint2 a = r ? cur[r] : c0, b = r ? cur[r + W] : c1;
-          const uint32_t o0 = dst[r], o1 = dst[r + W];
+          const uint2 a = r ? cur[opOf(r, n)] : c0, b = r ? cur[opOf(r + W, n)] : c1;
+          const uint32_t o0 = dst[opOf(r, n)], o1 = dst[opOf(r + W, n)];
           const uint32_t va = eval(a), vbb = eval(b);
-          vb[o0] = uint8_t(va);
-          vb[o1] = uint8_t(vbb);
+          wr(vb + o0, va);
+          wr(vb + o1, vbb);
         }
         if (r < n) {
-          const uint32_t o0 = dst[r];
-          vb[o0] = uint8_t(eval(r ? cur[r] : c0));
+          const uint32_t o0 = dst[opOf(r, n)];
+          wr(vb + o0, eval(r ? cur[opOf(r, n)] : c0));
         }
       } else {
-        uint8_t *wr = lb + ((l == 0 || ((l - 1) & 1u)) ? 0u : kFastBufY * uint32_t(I));
+        uint8_t *wl = lb + ((l == 0 || ((l - 1) & 1u)) ? 0u : kFastBufY * uint32_t(I));
         uint32_t r = 0;
         for (; r + W < n; r += 2 * W) {
-          const uint2 a = r ? cur[r] : c0, b = r ? cur[r + W] : c1;
+          const uint2 a = r ? cur[opOf(r, n)] : c0, b = r ? cur[opOf(r + W, n)] : c1;
           const uint32_t va = eval(a), vbb = eval(b);
-          wr[r * uint32_t(I)] = uint8_t(va);
-          wr[(r + W) * uint32_t(I)] = uint8_t(vbb);
+          wr(wl + r * uint32_t(I), va);
+          wr(wl + (r + W) * uint32_t(I), vbb);
         }
-        if (r < n) wr[r * uint32_t(I)] = uint8_t(eval(r ? cur[r] : c0));
+        if (r < n) wr(wl + r * uint32_t(I), eval(r ? cur[opOf(r, n)] : c0));
       }
     }
     cur = next;
@@ -256,8 +297,8 @@ __device__ __forceinline__ void lutFastTask(const Grp<I> &g, const uint32_t *T) 
   }
   // last level: the copies not folded into the last middle level
   for (uint32_t r = 0; r < nl; r += W) {
-    const uint32_t w = last[r];
-    vb[w & 0xFFFFu] = vb[w >> 16];
+    const uint32_t w = last[opOf(r, nl)];
+    wr(vb + (w & 0xFFFFu), rd(w >> 16));
   }
 }
 
