@@ -57,6 +57,46 @@ def test_step_matches_oracle(gpu_available, kernel, n):
     sim.cleanup()
 
 
+def test_one_launch_steps_across_kernels_and_free_runs(gpu_available):
+    """A one-instance step on the streamed or wave64 kernel is one launch that
+    applies the writes before its first cycle and takes the reads after its
+    last; the grid-barrier counter goes on from launch to launch and is
+    cleared when the grid changes (a kernel switch). Steps, free runs and
+    switches interleaved stay equal to the oracle, and the profile's barrier
+    and loop tick totals grow with every launch (barrier <= total)."""
+    d = design()
+    rng = np.random.default_rng(23)
+    wids, rids = _ids(d, rng)
+    print("case one-launch steps across kernels", flush=True)
+    sim = gpu_sim(d.design, d.symbols, True, None, kernel=KERNEL_STREAM)
+    sim.set_stimulus(d.inputs, 7)
+    orc = OracleRun(d.design, d.symbols, True, d.inputs, seed=7)
+    last = (0, 0)
+    plan = [(KERNEL_STREAM, "step", 1), (KERNEL_STREAM, "run", 5), (KERNEL_STREAM, "step", 2),
+            (KERNEL_WAVE64, "step", 1), (KERNEL_WAVE64, "run", 3), (KERNEL_WAVE64, "step", 2),
+            (KERNEL_STREAM, "step", 4), (KERNEL_STREAM, "run", 1), (KERNEL_STREAM, "step", 1)]
+    for i, (kernel, what, cycles) in enumerate(plan):
+        sim.set_kernel(kernel)
+        if what == "step":
+            vals = rng.integers(0, 16, size=(1, wids.size)).astype(np.uint8)
+            got, _ = sim.step(wids, vals, cycles, rids)
+            reg = orc.st.pool(0)
+            reg[wids] = vals[0]
+            orc.st.set_pool(0, reg)
+            orc.run(cycles)
+            want = orc.st.pool(0)[rids]
+            assert np.array_equal(got[0], want), f"{i}: {got[0].tolist()} != {want.tolist()}"
+        else:
+            sim.eval_free_running(cycles)
+            orc.run(cycles)
+        assert_pools_equal(gpu_pools(sim), orc.pools(), f"after plan entry {i}")
+        prof = sim.profile()
+        assert prof["total_ticks"] > last[1] and prof["barrier_ticks"] >= last[0], f"{i}: {prof}"
+        assert prof["barrier_ticks"] <= prof["total_ticks"]
+        last = (prof["barrier_ticks"], prof["total_ticks"])
+    sim.cleanup()
+
+
 def test_step_equals_register_calls(gpu_available):
     """Same results as tgs_write_regs / tgs_run_batch / tgs_read_regs per instance."""
     d = design()

@@ -12,13 +12,13 @@ enum Ctl : uint32_t {
   C_STOP = 0,        // sticky stop flag (SimEval.cu:194 shouldStop)
   C_LOGCOUNT = 2,    // u64: print-log entries appended since the host last drained the log
   C_STOPCYC = 4,     // u64: first cycle a stop op fired (UINT64_MAX if none)
-  C_BARTICKS = 6,    // u64: real-time ticks spent in grid barriers, summed over workgroups (last launch)
-  C_TOTTICKS = 8,    // u64: real-time ticks of the cycle loop, summed over workgroups (last launch)
-  C_BARRIER = 10,    // grid-barrier arrival counter
+  C_BARTICKS = 6,    // u64: real-time ticks spent in grid barriers, summed over workgroups and launches
+  C_TOTTICKS = 8,    // u64: real-time ticks of the cycle loop, summed over workgroups and launches
+  C_BARRIER = 10,    // grid-barrier arrival counter (kept across launches of one shape)
   C_WORDS = 16
 };
-// Words the host zeroes before every launch of a persistent kernel.
-constexpr uint32_t kCtlLaunchFirst = C_BARTICKS, kCtlLaunchWords = C_BARRIER + 1 - C_BARTICKS;
+// Control words a launch copies to its mailbox (stop flag through the ticks).
+constexpr uint32_t kMbCtlWords = C_TOTTICKS + 2;
 
 constexpr uint32_t kMaxDraws = 64;  // up to 4096 stimulus inputs
 
@@ -50,6 +50,17 @@ struct CommonArgs {
   uint64_t *profUseful, *profWg;
   uint32_t *profOwner;  // per partition: the workgroup that ran it
   uint32_t profStart, profCount;
+  // testbench mailbox (pinned host memory; counts 0 and mbCtl null when
+  // none): register mbWriteId[k] takes mbWriteVal[k] before the first cycle,
+  // mbReadVal[k] takes register mbReadId[k] after the last, and mbCtl the
+  // control words 0 .. kMbCtlWords - 1, so a testbench step is one launch
+  const uint32_t *mbWriteId;
+  const uint8_t *mbWriteVal;
+  uint32_t mbWrites;
+  const uint32_t *mbReadId;
+  uint8_t *mbReadVal;
+  uint32_t mbReads;
+  uint32_t *mbCtl;
 };
 
 // Whether launch-local cycle `cyc` is inside the profile window.
@@ -232,6 +243,50 @@ __device__ __forceinline__ uint64_t gridBarrier(uint32_t *counter, uint32_t targ
   }
   __syncthreads();
   return waited;
+}
+
+// This workgroup's share of the mailbox's register writes (tid of nt
+// threads). The caller passes a grid barrier before any partition runs: a
+// region-0 partition may write a register the mailbox writes, and no
+// workgroup may still be writing the mailbox's value then.
+__device__ __forceinline__ void mailboxWrites(const CommonArgs &a, uint32_t tid, uint32_t nt) {
+  for (uint32_t k = blockIdx.x * nt + tid; k < a.mbWrites; k += gridDim.x * nt)
+    __hip_atomic_store(a.reg + a.mbWriteId[k], a.mbWriteVal[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The CommonArgs at the start of the running kernel's argument block (every
+// persistent kernel's argument struct begins with one), through a pointer
+// the compiler cannot relate to its entry loads: what a kernel reads through
+// it after its cycle loop is loaded there, not held in scalar registers
+// through the loop.
+__device__ __forceinline__ CommonArgs kernargCommon() {
+  const CommonArgs *p = (const CommonArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+  asm volatile("" : "+s"(p));
+  return *p;
+}
+
+// End of a launch, called by one whole wave of every workgroup after the
+// workgroup has added its ticks: one more arrival on the grid-barrier counter
+// (target = (bar + 1) * gridDim.x, bar the index the launch's last grid
+// barrier waited for: its count was bar * gridDim.x). The
+// workgroup whose arrival completes it is the last: it sees every
+// workgroup's stores and ticks and copies the mailbox reads and the control
+// words to the mailbox. Everything it orders is an agent-scope atomic or an
+// agent-coherent access (the tick sums, the pools after the last grid
+// barrier), so waiting for the wave's own atomics to complete (vmcnt(0))
+// stands in for a release fence, which would write back the XCD's L2.
+__device__ __forceinline__ void finishLaunch(const CommonArgs &a, uint32_t target) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t last = 0;
+  if (lane == 0) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    last = __hip_atomic_fetch_add(&a.ctl[C_BARRIER], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == target;
+  }
+  if (!__builtin_amdgcn_readfirstlane(last)) return;
+  for (uint32_t k = lane; k < a.mbReads; k += 64)
+    a.mbReadVal[k] = __hip_atomic_load(a.reg + a.mbReadId[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (a.mbCtl != nullptr && lane < kMbCtlWords)
+    a.mbCtl[lane] = __hip_atomic_load(&a.ctl[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Cross-workgroup pools (register, exchange and memory pools) accessed with

@@ -24,7 +24,7 @@ struct W64Args {
   const uint32_t *schedPart;
   const uint32_t *schedLds;
   uint32_t vpBytes;          // value-pool bytes in LDS (16-byte multiple)
-  uint32_t barrierBase;      // barrier index offset (counter is zeroed per launch)
+  uint32_t barrierBase;      // the barrier counter holds barrierBase * grid when the launch starts
 };
 
 struct StreamArgs {
@@ -35,7 +35,7 @@ struct StreamArgs {
   const uint32_t *stepCount;  // per workgroup: records per cycle
   uint32_t vpBytes;           // value-pool bytes in LDS per slot (16-byte multiple)
   uint32_t slots;             // partitions side by side per workgroup (NC / slots waves each)
-  uint32_t barrierBase;
+  uint32_t barrierBase;       // the barrier counter holds barrierBase * grid when the launch starts
 };
 
 // Batched kernel: pools are instance-minor (nibble k of slot s at k * instPad + s).

@@ -376,6 +376,10 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     }
     loadRows<NC>(rs, recOf(2 + k), W, lane);
     __syncthreads();  // start barrier
+    if (c.mbWrites) {  // the workgroup barriers of the mailbox's grid barrier
+      __syncthreads();
+      __syncthreads();
+    }
     uint32_t j = 0, si = 0, cyc = 0;
     for (;;) {
       if (si == 0) {
@@ -419,6 +423,10 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
   bool pc = false;
   uint32_t bar = a.barrierBase;
   __syncthreads();  // start barrier: steps 0 and 1 are in the ring
+  if (c.mbWrites) {  // testbench register writes, complete before any partition runs
+    mailboxWrites(c, tid, nTc);
+    barTicks += gridBarrierShared(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+  }
   const uint32_t *mySlot = ring + wave * kStreamRows * 64 + lane;
   auto readStep = [&](Pf &p, uint32_t j) {
     const uint32_t *r = recOf(j);
@@ -491,6 +499,7 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     if (g == 0) *c.xs = xs;
     addBarrierTicks(c, barTicks, __builtin_amdgcn_s_memrealtime() - tStart);
   }
+  if (wave == 0) finishLaunch(kernargCommon(), (bar + 1) * gridDim.x);
 }
 
 namespace {

@@ -265,6 +265,10 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
   uint64_t barTicks = 0;
   const uint64_t tStart = __builtin_amdgcn_s_memrealtime();
   uint32_t bar = a.barrierBase;
+  if (a.c.mbWrites) {  // testbench register writes, complete before any partition runs
+    mailboxWrites(a.c, tid, nt);
+    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+  }
   // per-partition profile (SimEval.cu:894-983): ticks from the region's
   // start on this workgroup to the end of each partition, and to the end of
   // the region's grid barrier
@@ -300,6 +304,7 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
     if (g == 0) *a.c.xs = xs;
     addBarrierTicks(a.c, barTicks, __builtin_amdgcn_s_memrealtime() - tStart);
   }
+  if (tid < 64) finishLaunch(kernargCommon(), (bar + 1) * gridDim.x);
 }
 
 namespace {

@@ -76,6 +76,41 @@ struct DBuf {
   }
 };
 
+// Pinned host memory that kernels read and write directly (coherent, mapped
+// into the device's address space); `d` is its device-side address.
+struct PBuf {
+  void *h = nullptr, *d = nullptr;
+  size_t n = 0;
+  PBuf() = default;
+  PBuf(const PBuf &) = delete;
+  PBuf &operator=(const PBuf &) = delete;
+  ~PBuf() { reset(); }
+  void reset() {
+    if (h) (void)hipHostFree(h);
+    h = d = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t bytes) {
+    reset();
+    hipError_t e = hipHostMalloc(&h, bytes ? bytes : 16, hipHostMallocCoherent);
+    if (e != hipSuccess) {
+      h = nullptr;
+      return e;
+    }
+    std::memset(h, 0, bytes ? bytes : 16);
+    n = bytes;
+    return hipHostGetDevicePointer(&d, h, 0);
+  }
+  template <typename T>
+  T *host() const {
+    return static_cast<T *>(h);
+  }
+  template <typename T>
+  T *dev() const {
+    return static_cast<T *>(d);
+  }
+};
+
 // Waveform snapshots (SimVCD): the register pool of one instance is copied
 // to a device staging slot on the context stream, ordered after the cycle
 // launches already queued, and from there to pinned host memory on a side
@@ -104,7 +139,21 @@ struct SnapRing {
 
 struct Instance {
   DBuf reg, mem, exg, xs, ctl, logCycle, logMsg, probe;
+  PBuf ctlPin;  // the control words as the last launch left them (written by the launch's last workgroup)
   uint64_t cycles = 0;
+  // The grid-barrier counter is not cleared between launches of one grid
+  // size: the next launch starts at barrier index barNext (the counter holds
+  // barNext * barGrid); barGrid 0 means the counter is unknown and the next
+  // launch clears it first. ticksSeen: the control block's barrier and loop
+  // tick totals already added to the profile.
+  uint32_t barNext = 0, barGrid = 0;
+  uint64_t ticksSeen[2] = {0, 0};
+  // the host rewrote the control block (counter and tick totals zero)
+  void ctlRewritten() {
+    barNext = 0;
+    barGrid = 0;
+    ticksSeen[0] = ticksSeen[1] = 0;
+  }
 };
 
 // Instance-minor state of the batched kernel: pool nibble k of slot s at
@@ -226,9 +275,10 @@ struct tgs_ctx {
   void *scratchDev = nullptr;  // this device's static scratch array (tgs_run_batch_noalloc)
   SnapRing snap;
   bool timingPending = false;  // the last launch's events are not accounted yet
-  // testbench mailbox (tgs_step): pinned host staging and its device twin
-  void *boxHost = nullptr;
-  size_t boxBytes = 0;
+  // testbench mailbox (tgs_step): pinned host memory (read and written by
+  // the streamed and wave64 kernels directly) and a device twin (the copy
+  // path of the other kernels)
+  PBuf box;
   DBuf boxDev;
   // signal probes: request order -> CSR position (CSR = grouped by partition)
   std::vector<uint32_t> probeCsrOf;
@@ -292,6 +342,7 @@ int resetInstance(tgs_ctx *c, Instance &in, uint32_t index, uint64_t seed) {
   HIPCHK(c, in.exg.alloc(up4(d.exchangePoolSize) + 16));
   HIPCHK(c, in.xs.alloc(8));
   HIPCHK(c, in.ctl.alloc(C_WORDS * 4));
+  if (!in.ctlPin.h) HIPCHK(c, in.ctlPin.alloc(C_WORDS * 4));
   HIPCHK(c, in.logCycle.alloc(kLogCap * 8));
   HIPCHK(c, in.logMsg.alloc(kLogCap * 4));
   HIPCHK(c, in.probe.alloc(std::max<uint32_t>(c->nProbe(), 1)));
@@ -311,6 +362,7 @@ int resetInstance(tgs_ctx *c, Instance &in, uint32_t index, uint64_t seed) {
   HIPCHK(c, hipMemcpyAsync(in.xs.p, xs.data(), 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   in.cycles = 0;
+  in.ctlRewritten();
   return TGS_OK;
 }
 
@@ -584,17 +636,59 @@ int profBegin(tgs_ctx *c, Instance &in, uint32_t n) {
   return TGS_OK;
 }
 
-// Largest cycle count per launch that keeps the barrier counter in range.
+// Largest cycle count per launch that keeps the barrier counter in range
+// (two grid barriers per cycle, plus the mailbox's and the closing arrival).
 uint32_t maxCyclesPerLaunch(uint32_t groups) {
-  uint64_t m = (0xF0000000ull / (2ull * std::max<uint32_t>(groups, 1)));
+  uint64_t m = (0xF0000000ull / (2ull * std::max<uint32_t>(groups, 1))) - 2;
   return uint32_t(std::min<uint64_t>(m, 1u << 30));
 }
+
+// Barrier base of a launch of `grid` workgroups that waits at `barriers`
+// grid barriers and then arrives once more (finishLaunch). The counter is
+// cleared first, by a fill kernel, when it counted for another grid or is
+// unknown, or when this launch would carry it out of range; otherwise it
+// goes on from the last launch and no clear precedes the launch.
+int barrierBaseFor(tgs_ctx *c, Instance &in, uint32_t grid, uint64_t barriers, uint32_t *base) {
+  if (in.barGrid != grid || (uint64_t(in.barNext) + barriers + 1) * grid >= 0xF0000000ull) {
+    HIPCHK(c, launchFill32(in.ctl.as<uint32_t>() + C_BARRIER, 1, 0, c->stream));
+    in.barNext = 0;
+    in.barGrid = grid;
+  }
+  *base = in.barNext;
+  in.barNext += uint32_t(barriers + 1);
+  return TGS_OK;
+}
+
+// Barrier and loop ticks of the instance's launches so far: `ctlWords` are
+// the control words its last completed launch copied out (cumulative
+// totals); the profile takes what was not yet counted.
+void accountTicks(tgs_ctx *c, Instance &in, const uint32_t *ctlWords) {
+  uint64_t t[2];
+  std::memcpy(t, ctlWords + C_BARTICKS, 16);
+  c->prof.barrier_ticks += t[0] - in.ticksSeen[0];
+  c->prof.total_ticks += t[1] - in.ticksSeen[1];
+  in.ticksSeen[0] = t[0];
+  in.ticksSeen[1] = t[1];
+}
+
+// A testbench step's mailbox (device addresses of pinned host memory): the
+// register writes go with the step's first launch, the register reads with
+// its last (streamed and wave64 kernels).
+struct StepBox {
+  const uint32_t *writeId;
+  const uint8_t *writeVal;
+  uint32_t nWrite;
+  const uint32_t *readId;
+  uint8_t *readVal;
+  uint32_t nRead;
+};
 
 // Launch `cycles` cycles of one instance. With `wait` each launch is waited
 // for and timed at once; without it the last launch's events stay pending
 // (c->timingPending) and finishTiming() accounts them after the caller's own
-// synchronization, so a testbench step costs one round trip.
-int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
+// synchronization, so a testbench step costs one round trip. Every streamed
+// or wave64 launch leaves its control words in in.ctlPin.
+int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true, const StepBox *mb = nullptr) {
   if (c->timingPending && cycles > 0) {  // a step left its launch untimed: account it before reusing the events
     c->timingPending = false;
     HIPCHK(c, hipEventSynchronize(c->ev1));
@@ -603,7 +697,22 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
     c->prof.kernel_ms += ms;
   }
   const uint32_t printCap = printCycleCap(c, kLogCap, 1);
-  uint32_t *ctlLaunch = in.ctl.as<uint32_t>() + kCtlLaunchFirst;
+  bool firstLaunch = true;
+  // mailbox and control-word arguments of a streamed or wave64 launch of n cycles
+  auto boxArgs = [&](CommonArgs &a, uint32_t n) {
+    a.mbCtl = in.ctlPin.dev<uint32_t>();
+    if (!mb) return;
+    if (firstLaunch) {
+      a.mbWriteId = mb->writeId;
+      a.mbWriteVal = mb->writeVal;
+      a.mbWrites = mb->nWrite;
+    }
+    if (n == cycles) {
+      a.mbReadId = mb->readId;
+      a.mbReadVal = mb->readVal;
+      a.mbReads = mb->nRead;
+    }
+  };
   while (cycles > 0) {
     int rc = drainLogs(c);  // every launch starts with an empty device log
     if (rc) return rc;
@@ -648,10 +757,13 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
       a.stepCount = c->sStepCount.as<uint32_t>();
       a.vpBytes = up16(std::max<uint32_t>(c->simg.maxVp, 16));
       a.slots = c->streamSlots;
-      a.barrierBase = 0;
-      HIPCHK(c, hipMemsetAsync(ctlLaunch, 0, 4 * kCtlLaunchWords, c->stream));
+      boxArgs(a.c, n);
+      if ((rc = barrierBaseFor(c, in, c->streamGrid, 2ull * n + (a.c.mbWrites ? 1 : 0), &a.barrierBase))) return rc;
       HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-      HIPCHK(c, launchStream(a, c->streamNCUsed, c->streamGrid, c->streamLds, c->stream));
+      if (hipError_t e = launchStream(a, c->streamNCUsed, c->streamGrid, c->streamLds, c->stream); e != hipSuccess) {
+        in.barGrid = 0;  // the counter is unknown after a failed launch
+        HIPCHK(c, e);
+      }
       HIPCHK(c, hipEventRecord(c->ev1, c->stream));
       c->prof.grid = c->streamGrid;
       c->prof.block = streamBlock(c->streamNCUsed);
@@ -675,10 +787,15 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
       a.schedPart = c->wSchedPart.as<uint32_t>();
       a.schedLds = c->wSchedLds.as<uint32_t>();
       a.vpBytes = up16(std::max<uint32_t>(c->w64.maxVp, 16));
-      a.barrierBase = 0;
-      HIPCHK(c, hipMemsetAsync(ctlLaunch, 0, 4 * kCtlLaunchWords, c->stream));
+      boxArgs(a.c, n);
+      if ((rc = barrierBaseFor(c, in, c->sched.groups, 2ull * n + (a.c.mbWrites ? 1 : 0), &a.barrierBase)))
+        return rc;
       HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-      HIPCHK(c, launchWave64(a, c->w64Mode, c->sched.groups, c->w64Block, c->w64Lds, c->stream));
+      if (hipError_t e = launchWave64(a, c->w64Mode, c->sched.groups, c->w64Block, c->w64Lds, c->stream);
+          e != hipSuccess) {
+        in.barGrid = 0;  // the counter is unknown after a failed launch
+        HIPCHK(c, e);
+      }
       HIPCHK(c, hipEventRecord(c->ev1, c->stream));
       c->prof.grid = c->sched.groups;
       c->prof.block = c->w64Block;
@@ -689,6 +806,7 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
       in.cycles += n;
       c->prof.cycles += n;
     }
+    firstLaunch = false;
     c->prof.launches += 1;
     c->prof.kernel = c->kernel;
     if (c->printEnable && c->stats.n_print) c->logDirty = true;
@@ -700,12 +818,7 @@ int runInstance(tgs_ctx *c, Instance &in, uint32_t cycles, bool wait = true) {
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->prof.kernel_ms += ms;
-    if (c->kernel == TGS_KERNEL_WAVE64 || c->kernel == TGS_KERNEL_STREAM) {
-      uint64_t t[2];
-      HIPCHK(c, hipMemcpy(t, in.ctl.as<uint32_t>() + C_BARTICKS, 16, hipMemcpyDeviceToHost));
-      c->prof.barrier_ticks += t[0];
-      c->prof.total_ticks += t[1];
-    }
+    if (c->kernel == TGS_KERNEL_WAVE64 || c->kernel == TGS_KERNEL_STREAM) accountTicks(c, in, in.ctlPin.host<uint32_t>());
   }
   return TGS_OK;
 }
@@ -747,6 +860,7 @@ int instUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
     ctl[C_STOP] = x.stop;
     std::memcpy(&ctl[C_STOPCYC], &x.stopCycle, 8);
     HIPCHK(c, hipMemcpy(in.ctl.p, ctl, sizeof ctl, hipMemcpyHostToDevice));
+    in.ctlRewritten();
     in.cycles = x.cycles;
   }
   return TGS_OK;
@@ -1387,7 +1501,7 @@ void tgs_destroy(tgs_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->snap.release();
-  if (c->boxHost) (void)hipHostFree(c->boxHost);
+  c->box.reset();
   c->inst.clear();
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1806,20 +1920,40 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
   const size_t oRead = 4ull * nWrite, oWv = oRead + 4ull * nRead, oRv = oWv + size_t(ni) * nWrite;
   const size_t oCtl = (oRv + size_t(ni) * nRead + 7) & ~size_t(7);
   const size_t bytes = oCtl + 4ull * kCtlBack * ni;
-  if (bytes > c->boxBytes) {
+  if (bytes > c->box.n) {
     HIPCHK(c, hipStreamSynchronize(c->stream));  // no copy of a failed earlier step may still use the old box
-    if (c->boxHost) HIPCHK(c, hipHostFree(c->boxHost));
-    c->boxHost = nullptr;
-    c->boxBytes = 0;
-    HIPCHK(c, hipHostMalloc(&c->boxHost, bytes, hipHostMallocDefault));
+    HIPCHK(c, c->box.alloc(bytes));
     HIPCHK(c, c->boxDev.alloc(bytes));
-    c->boxBytes = bytes;
   }
   c->noteWrite(writeVals, size_t(ni) * nWrite);
-  uint8_t *h = static_cast<uint8_t *>(c->boxHost), *d = c->boxDev.as<uint8_t>();
+  uint8_t *h = c->box.host<uint8_t>(), *d = c->boxDev.as<uint8_t>();
   std::memcpy(h, writeIds, 4ull * nWrite);
   std::memcpy(h + oRead, readIds, 4ull * nRead);
   std::memcpy(h + oWv, writeVals, size_t(ni) * nWrite);
+  // One instance on the streamed or wave64 kernel: the step is one launch
+  // (the kernel applies the writes before its first cycle and takes the
+  // reads after its last, through the pinned mailbox) and one
+  // synchronization; no copies, no mailbox kernels.
+  if (!c->batched && ni == 1 && cycles > 0 && c->kernel != TGS_KERNEL_PLAIN) {
+    Instance &in = *c->inst[0];
+    const uint8_t *hd = c->box.dev<uint8_t>();
+    const StepBox mb{reinterpret_cast<const uint32_t *>(hd), hd + oWv, nWrite,
+                     reinterpret_cast<const uint32_t *>(hd + oRead), const_cast<uint8_t *>(hd) + oRv, nRead};
+    if ((rc = runInstance(c, in, cycles, false, &mb))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->timingPending) {
+      c->timingPending = false;
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      c->prof.kernel_ms += ms;
+    }
+    const uint32_t *ctl = in.ctlPin.host<uint32_t>();
+    accountTicks(c, in, ctl);
+    if (nRead) std::memcpy(readVals, h + oRv, nRead);
+    if (anyStopped) *anyStopped = ctl[C_STOP] != 0;
+    c->stepFailed = false;
+    return TGS_OK;
+  }
   auto box = [&](const uint32_t *ids, uint32_t n, uint8_t *vals, bool toPool) -> int {
     if (c->batched) {
       HIPCHK(c, launchRegMailbox(c->bs.reg.as<uint8_t>(), c->bs.instPad, c->bs.instOfD.as<uint32_t>(), c->bs.n, ids,
@@ -1873,12 +2007,8 @@ int tgs_step(tgs_ctx *c, const uint32_t *writeIds, uint32_t nWrite, const uint8_
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->prof.kernel_ms += ms;
-    if (c->kernel == TGS_KERNEL_WAVE64 || c->kernel == TGS_KERNEL_STREAM) {
-      uint64_t t[2];
-      std::memcpy(t, h + oCtl + 4ull * (size_t(kCtlBack) * (ni - 1) + C_BARTICKS), 16);
-      c->prof.barrier_ticks += t[0];
-      c->prof.total_ticks += t[1];
-    }
+    if (c->kernel == TGS_KERNEL_WAVE64 || c->kernel == TGS_KERNEL_STREAM)
+      accountTicks(c, *c->inst[ni - 1], reinterpret_cast<const uint32_t *>(h + oCtl) + size_t(kCtlBack) * (ni - 1));
   }
   if (!c->batched)
     for (uint32_t i = 0; i < ni; ++i) {
