@@ -12,7 +12,8 @@ steps cover the 100,000 cycles. Inputs are resident in HBM before timing.
 Batched leg: the 200,000-LUT4 design (depth 48, 8 SRAMs of 1024 x 32 bit)
 run as --instances instances (default 1,024) with distinct xorshift64 seeds
 and cycle counts drawn uniformly from [1,000, 50,000], in one persistent
-launch per slice of --slice cycles. An autotuner first sweeps the partition
+launch (instances end at their own counts inside it; --slice N cuts the run
+into launches of at most N cycles per instance instead). An autotuner first sweeps the partition
 size (256..4,096 LUT4s) and the workgroup size on a short run; the logical design is identical for every
 partition size, so results do not depend on its choice.
 
@@ -89,7 +90,8 @@ def parse():
     ap.add_argument("--tune-cycles", type=int, default=100)
     ap.add_argument("--part-luts", type=int, default=0,
                     help="batched leg: partition size in LUT4s (0: the autotuner picks)")
-    ap.add_argument("--slice", type=int, default=5_000, help="cycles per instance per batched launch")
+    ap.add_argument("--slice", type=int, default=0,
+                    help="cycles per instance per batched launch (0: the whole run in one launch)")
     ap.add_argument("--seed", type=int, default=1)
     return ap.parse_args()
 
@@ -373,13 +375,15 @@ def run_batched(args):
     sim.set_instances(n)
     sim.set_stimulus(d.inputs, args.seed)
     sim.reset_profile()
-    # the run is issued in slices of --slice cycles per instance so that
-    # progress is visible; each slice is one persistent launch
+    # one persistent launch for the whole run (or slices of --slice cycles per
+    # instance, each one launch)
+    if args.slice <= 0:
+        log(f"batched: one launch over {int(cycles.sum())} instance-cycles, longest {int(cycles.max())} cycles")
     left = cycles.astype(np.int64).copy()
     launch_cycles = 0
     t0 = time.perf_counter()
     while left.max() > 0:
-        step = np.minimum(left, args.slice).astype(np.uint32)
+        step = (left if args.slice <= 0 else np.minimum(left, args.slice)).astype(np.uint32)
         sim.run_batch(step)
         launch_cycles += int(step.max())
         left -= step
@@ -392,7 +396,7 @@ def run_batched(args):
     value = total / dt
     log(f"batched: {value:.3e} stimulus-cycles/s over {total} instance-cycles in {dt:.2f} s")
     out = {"value": value, "unit": "stimulus-cycles/s", "instances": n, "total_cycles": total, "seconds": dt,
-           "kernel_ms": prof["kernel_ms"], "part_luts": part, "block": prof["block"], "grid": prof["grid"],
+           "kernel_ms": prof["kernel_ms"], "launches": prof["launches"], "part_luts": part, "block": prof["block"], "grid": prof["grid"],
            "instances_per_wave": prof["batch_group"], "lds_bytes": prof["lds_bytes"], "tuning": tuning,
            "workload": "lut200k: 200,000 LUT4, depth 48, 8 SRAM 1024x32b, 1,024 instances, "
                        f"cycles uniform in [{args.min_cycles}, {args.max_cycles}]; " + FANIN,
