@@ -183,9 +183,13 @@ int tgs_write_regs(tgs_ctx *ctx, uint32_t instance, const uint32_t *ids, uint32_
  * write_vals[i * n_write + k] (ids distinct), runs `cycles` cycles of every
  * instance (0: none), then reads register read_ids[k] of instance i into
  * read_vals[i * n_read + k]. The ids and values travel through one pinned
- * mailbox: one host-to-device copy and one scatter kernel before the run, one
- * gather kernel and one device-to-host copy after it, one synchronization.
- * *any_stopped (may be null): some instance's sticky stop flag is set. */
+ * mailbox. One instance on the streamed or wave64 kernel with cycles > 0:
+ * one launch, which applies the writes before its first cycle and takes the
+ * reads after its last from and to the mailbox, and one synchronization.
+ * Otherwise one host-to-device copy and one scatter kernel before the run,
+ * one gather kernel and one device-to-host copy after it, one
+ * synchronization. *any_stopped (may be null): some instance's sticky stop
+ * flag is set. */
 int tgs_step(tgs_ctx *ctx, const uint32_t *write_ids, uint32_t n_write, const uint8_t *write_vals, uint32_t cycles,
              const uint32_t *read_ids, uint32_t n_read, uint8_t *read_vals, int *any_stopped);
 /* Whole-pool access (checkpoint/restore of the architectural state). */
