@@ -633,7 +633,7 @@ __device__ __forceinline__ void batchRegion(const BatchLaunch &L, uint32_t r, ui
 // region 0 of every context, a grid barrier, region 1, a grid barrier.
 // groupsActive[c] of a context counts its groups of I instances.
 template <int I>
-__global__ __launch_bounds__(512) void batchCycles(BatchLaunch L) {
+__global__ __launch_bounds__(1024) void batchCycles(BatchLaunch L) {
   const uint32_t NW = blockDim.x >> 6;
   const uint32_t rowBytes = (L.vpRows + kLevelRows) * uint32_t(I);
   uint8_t *vpAll = bLds + kLutRegionB;  // per wave: value-pool rows, then level rows

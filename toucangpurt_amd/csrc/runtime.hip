@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -225,6 +226,7 @@ struct tgs_ctx {
   // image build), waves per workgroup (0 = chosen) and the effective values.
   uint32_t batchInst = 0, batchWaves = 0, batchInstEff = 64, batchBlockEff = 128;
   uint32_t batchGrid = 0, batchLds = 0, batchGridOverride = 0, batchProgWords = 0;
+  uint32_t batchShapeN = 0;  // instance count the batched launch shape was chosen for
   bool batchReady = false;
   // Every byte of every pool is a nibble (true after a load whose initial
   // pools are nibbles, false from the first wider byte written through the
@@ -1000,9 +1002,19 @@ uint32_t instanceCount(const tgs_ctx *c) { return c->batched ? c->bs.n : uint32_
 // shape: instances per wave I (a wave evaluates 64 / I ops per instruction
 // round for I instances) and waves per workgroup. Each wave holds its group's
 // value pool and level buffers in LDS ([slot][I], [2][2][32][I]); the LUT and
-// the staged partition program are shared by the workgroup. Among the shapes
-// that fit, the one with the most co-resident waves per CU (up to 16) runs,
-// then the one with the smallest I (shortest (partition, group) items).
+// the staged partition program are shared by the workgroup. The kernel is
+// bound by the latency of each wave's dependent LDS chain, so its throughput
+// follows the instances in flight per CU: co-resident workgroups (their
+// waves at most 16 per CU) times the instances one item carries on average
+// (an item is one partition for `waves` consecutive groups of I; this
+// context's instances fill ceil(groups / waves) items), weighted for the
+// longer LUT rounds of wide groups (a T_LUTFAST round is 256 / I ops: 0.8
+// at I = 32, 0.5 at I = 64, as measured against I = 8 and 16). The shape
+// with the most of them runs; ties go to more waves per CU, then to the
+// smaller I. The LUT and the program buffers are paid per workgroup, so one
+// large workgroup per CU can carry more instances than two small ones
+// (1,024 instances of the 200K-LUT4 design: 12 waves of I = 16 against two
+// workgroups of 8 waves of I = 8, 310 against 377 ms on one box).
 int ensureBatchImage(tgs_ctx *c) {
   if (c->batchReady) return TGS_OK;
   const uint32_t imageWaves = c->batchWaves ? c->batchWaves : 4;
@@ -1022,6 +1034,16 @@ int ensureBatchImage(tgs_ctx *c) {
     return m;
   };
   uint32_t maxPart = maxWords(c->w64);
+  // T_LUTFAST image, when every value is a nibble: its tasks' word counts do
+  // not depend on I (levels are unpadded), so one build sizes the program
+  // buffers, and every shape below is sized for the larger of the two images
+  const char *envFast = std::getenv("TGS_BATCH_FAST");
+  const bool fastAllowed = !(envFast && envFast[0] == '0');
+  Wave64Image fastImg;
+  if (fastAllowed && c->nibbleState && (vpRows + kLevelRows) * 8u <= 65536u && nibbleProgram(c->design)) {
+    fastImg = buildWave64Image(c->design, c->stim, imageWaves, vpRows, 8);
+    if (fastImg.fastTasks) maxPart = std::max(maxPart, maxWords(fastImg));
+  }
   const uint32_t ldsMax = uint32_t(c->prop.sharedMemPerBlock) - kLdsGranuleMargin;
   HIPCHK(c, batchCheckNoStaticLds());
   const char *envInst = std::getenv("TGS_BATCH_INST");
@@ -1031,9 +1053,12 @@ int ensureBatchImage(tgs_ctx *c) {
     if (!forcedInst || forcedInst == i) insts.push_back(i);
   if (insts.empty()) return c->fail(TGS_E_ARG, "instances per wave must be 8, 16, 32 or 64 (%u given)", forcedInst);
   if (c->batchWaves) waveList.push_back(c->batchWaves);
-  else waveList = {2u, 4u, 8u};
+  else waveList = {2u, 4u, 8u, 12u, 16u};
+  const uint64_t nInst = std::max<uint32_t>(c->bs.n, 1);
+  c->batchShapeN = uint32_t(nInst);
   bool have = false;
-  uint32_t bestScore = 0, bestInst = 0, bestWaves = 0, bestLds = 0;
+  uint32_t bestScore = 0, bestInst = 0, bestWaves = 0, bestLds = 0;  // bestScore: waves per CU
+  double bestCols = 0;
   int bestPerCU = 0;
   for (uint32_t inst : insts)
     for (uint32_t waves : waveList) {
@@ -1044,9 +1069,17 @@ int ensureBatchImage(tgs_ctx *c) {
       HIPCHK(c, batchMaxBlocksPerCU(&perCU, inst, waves * 64, lds));
       perCU = residentPerCU(c, perCU, lds);
       if (perCU < 1) continue;
-      const uint32_t score = std::min<uint32_t>(16, uint32_t(perCU) * waves);
-      if (!have || score > bestScore || (score == bestScore && inst < bestInst)) {
+      perCU = std::min<int>(perCU, std::max<uint32_t>(1, 16 / waves));
+      const uint32_t score = uint32_t(perCU) * waves;
+      const uint64_t groups = (nInst + inst - 1) / inst, items = (groups + waves - 1) / waves;
+      const double roundWeight = inst >= 64 ? 0.5 : inst >= 32 ? 0.8 : 1.0;
+      const double cols = double(perCU) * double(std::min<uint64_t>(nInst, items * waves * inst)) / double(items) *
+                          roundWeight;
+      const bool same = std::abs(cols - bestCols) <= 1e-6 * cols;
+      if (!have || (!same && cols > bestCols) ||
+          (same && (score > bestScore || (score == bestScore && inst < bestInst)))) {
         have = true;
+        bestCols = cols;
         bestScore = score;
         bestInst = inst;
         bestWaves = waves;
@@ -1058,26 +1091,12 @@ int ensureBatchImage(tgs_ctx *c) {
     return c->fail(TGS_E_LIMIT,
                    "batched execution needs %u B of LDS (value pool %u slots x %u instances); device allows %u B",
                    batchLdsBytes(insts[0], 1, vpBytes, maxPart), c->w64.maxVp, insts[0], ldsMax);
-  // T_LUTFAST tasks for the chosen instance count, when every value is a
-  // nibble and the rows' byte offsets fit 16 bits (kept only if the shape
-  // still fits as many waves per CU)
-  const char *envFast = std::getenv("TGS_BATCH_FAST");
-  const bool fastAllowed = !(envFast && envFast[0] == '0');
-  if (fastAllowed && c->nibbleState && (vpRows + kLevelRows) * bestInst <= 65536u && nibbleProgram(c->design)) {
-    Wave64Image f = buildWave64Image(c->design, c->stim, imageWaves, vpRows, bestInst);
-    const uint32_t fPart = maxWords(f);
-    const uint32_t lds = batchLdsBytes(bestInst, bestWaves, vpBytes, fPart);
-    int perCU = 0;
-    if (f.fastTasks && lds <= ldsMax) {
-      HIPCHK(c, batchSetLds(lds));
-      HIPCHK(c, batchMaxBlocksPerCU(&perCU, bestInst, bestWaves * 64, lds));
-      perCU = residentPerCU(c, perCU, lds);
-    }
-    if (perCU >= 1 && std::min<uint32_t>(16, uint32_t(perCU) * bestWaves) >= bestScore) {
+  // T_LUTFAST tasks for the chosen instance count when the rows' byte
+  // offsets fit 16 bits (the shape was sized for this image's program)
+  if (fastImg.fastTasks && (vpRows + kLevelRows) * bestInst <= 65536u) {
+    Wave64Image f = bestInst == 8 ? std::move(fastImg) : buildWave64Image(c->design, c->stim, imageWaves, vpRows, bestInst);
+    if (f.fastTasks && maxWords(f) <= maxPart) {
       c->w64 = std::move(f);
-      maxPart = fPart;
-      bestLds = lds;
-      bestPerCU = perCU;
       c->batchFast = true;
     }
   }
@@ -1611,7 +1630,7 @@ int tgs_set_launch(tgs_ctx *c, int kernel, uint32_t block, uint32_t groups) {
       c->w64Ready = false;
       break;
     case TGS_KERNEL_BATCH:
-      if (block > 512) return c->fail(TGS_E_ARG, "batched kernel block must be at most 512 (eight waves)");
+      if (block > 1024) return c->fail(TGS_E_ARG, "batched kernel block must be at most 1024 (sixteen waves)");
       c->batchWaves = block / 64;
       c->batchGridOverride = groups;
       c->batchReady = false;
@@ -1651,6 +1670,7 @@ int tgs_set_instances(tgs_ctx *c, uint32_t n) {
   if ((rc = uploadAll(c, h))) return rc;
   c->logs.assign(n, HostLog());
   c->logDirty = false;
+  if (c->batched && c->batchShapeN != n) c->batchReady = false;  // the launch shape depends on the count
   return c->batched ? ensureBatchImage(c) : TGS_OK;
 }
 
