@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "device_common.h"
 
 namespace tgs {
@@ -37,6 +39,10 @@ struct StreamArgs {
   uint32_t slots;             // partitions side by side per workgroup (NC / slots waves each)
   uint32_t barrierBase;       // the barrier counter holds barrierBase * grid when the launch starts
 };
+
+// kernargCommon() reads the CommonArgs at offset 0 of a kernel's argument block.
+static_assert(offsetof(W64Args, c) == 0 && offsetof(StreamArgs, c) == 0 && offsetof(PlainArgs, c) == 0,
+              "every persistent kernel's argument struct begins with its CommonArgs");
 
 // Batched kernel: pools are instance-minor (nibble k of slot s at k * instPad + s).
 struct BatchArgs {
