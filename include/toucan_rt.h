@@ -73,6 +73,9 @@ typedef struct tgs_profile {
   uint32_t batch_group;   /* batched: instances per wave (each wave evaluates 64 / batch_group ops per round) */
   uint32_t batch_fast_tasks; /* batched: LUT micropartition pairs evaluated without index bounds checks
                                 (every value a nibble; 0 when the program or the state does not allow it) */
+  uint32_t batch_nibble_rows; /* batched: 1 when the rows hold a nibble per instance (every LUT task
+                                 evaluated without bounds checks), 0 when a byte */
+  uint32_t reserved;
 } tgs_profile;
 
 /* ---- program inspection (host only, no device needed) ---- */

@@ -50,3 +50,23 @@ def test_fast_lut_tasks_equal_bounds_checked(tmp_path, dname, inst):
     assert out.returncode == 0, out.stderr
     fast, rows = map(int, out.stdout.split()[1:3])
     assert fast > 0
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="batch_check not built")
+@pytest.mark.parametrize("dname", list(DESIGNS))
+@pytest.mark.parametrize("inst", [8, 16, 32, 64])
+def test_fast_lut_tasks_on_nibble_rows(tmp_path, dname, inst):
+    """The same check with the fast image laid out for nibble rows (two
+    instances per byte, offsets row * I / 2), compared per instance with the
+    byte rows of the bounds-checked tasks; constant rows 16..31 are not
+    representable in a nibble and hold random nibbles, so a task that read
+    them would differ."""
+    p = design_file(tmp_path, dname)
+    out = subprocess.run([EXE, str(p), str(inst), "11", "2", "nib"], capture_output=True, text=True, timeout=300)
+    if "do not fit 16-bit offsets" in out.stderr:
+        pytest.skip(f"value pool x {inst // 2} bytes per row exceeds 16-bit offsets")
+    if "nibble rows need every LUT task fast" in out.stderr:
+        pytest.skip("some LUT pair needs the bounds-checked task (the runtime keeps byte rows)")
+    assert out.returncode == 0, out.stderr
+    fast, rows = map(int, out.stdout.split()[1:3])
+    assert fast > 0 and rows == 0

@@ -132,6 +132,42 @@ def test_fast_lut_tasks_equal_bounds_checked_ones(gpu_available, monkeypatch):
         assert_pools_equal(pools[True][k], orc.pools(), f"instance {k}")
 
 
+def test_nibble_rows_equal_byte_rows(gpu_available, monkeypatch):
+    """On a nibble program whose LUT tasks are all T_LUTFAST the batched rows
+    hold a nibble per instance (batch_nibble_rows); TGS_BATCH_NIB=0 keeps a
+    byte per instance and TGS_BATCH_FAST=0 the bounds-checked tasks. All three
+    agree on every instance, and with the oracle, for instance counts that
+    give different shapes (nibble rows of 16 and 32 instances per wave)."""
+    d = design("mixed")
+    for n in (70, 1024):
+        cycles = np.random.default_rng(n).integers(1, 24, size=n).astype(np.uint32)
+        pools = {}
+        for mode in ("nib", "byte", "rows"):
+            monkeypatch.delenv("TGS_BATCH_NIB", raising=False)
+            monkeypatch.delenv("TGS_BATCH_FAST", raising=False)
+            if mode == "byte":
+                monkeypatch.setenv("TGS_BATCH_NIB", "0")
+            if mode == "rows":
+                monkeypatch.setenv("TGS_BATCH_FAST", "0")
+            print(f"case batched rows={mode} n={n}", flush=True)
+            sim = gpu_sim(d.design, d.symbols, True, None, kernel=KERNEL_BATCH)
+            sim.set_instances(n)
+            sim.set_stimulus(d.inputs, 23)
+            sim.run_batch(cycles)
+            prof = sim.profile()
+            assert prof["batch_nibble_rows"] == (1 if mode == "nib" else 0), prof
+            assert (prof["batch_fast_tasks"] > 0) == (mode != "rows"), prof
+            pools[mode] = [gpu_pools(sim, k) for k in range(n)]
+            sim.cleanup()
+        for k in range(n):
+            assert_pools_equal(pools["nib"][k], pools["byte"][k], f"n={n} instance {k}: nibble vs byte rows")
+            assert_pools_equal(pools["nib"][k], pools["rows"][k], f"n={n} instance {k}: nibble vs bounds-checked")
+        for k in (0, n // 2, n - 1):
+            orc = OracleRun(d.design, d.symbols, True, d.inputs, seed=23, instance=k)
+            orc.run(int(cycles[k]))
+            assert_pools_equal(pools["nib"][k], orc.pools(), f"n={n} instance {k}")
+
+
 def test_wide_register_bytes_switch_off_fast_tasks(gpu_available):
     """Register bytes above 15 written through the API end the nibble state:
     the next run rebuilds the image with bounds-checked LUT tasks, and the

@@ -121,6 +121,8 @@ class Profile(ctypes.Structure):
         ("total_ticks", ctypes.c_uint64),
         ("batch_group", ctypes.c_uint32),
         ("batch_fast_tasks", ctypes.c_uint32),
+        ("batch_nibble_rows", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

@@ -42,14 +42,32 @@ std::vector<uint8_t> slurp(const char *path) {
   return std::vector<uint8_t>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
 }
 
-// One wave's LDS rows: [row][I] bytes, rows = V value-pool rows then
-// kLevelRows level rows (constant rows at kConstRow hold 0..31).
+// One wave's LDS rows: [row][RB] bytes, rows = V value-pool rows then
+// kLevelRows level rows (constant rows at kConstRow hold 0..31). RB = I (a
+// byte per instance) or, with nibble rows, I / 2 (instances 2j and 2j + 1 in
+// the low and high nibble of byte j).
 struct Area {
   uint32_t V, I;
+  bool nib;
   std::vector<uint8_t> b;
+  uint32_t rb() const { return nib ? I / 2 : I; }
   uint8_t &at(uint64_t off, const char *what) {
     if (off >= b.size()) die(std::string("offset out of the wave's rows: ") + what + " " + std::to_string(off));
     return b[off];
+  }
+  // value of instance ii in the row at byte offset `rowOff`
+  uint32_t get(uint64_t rowOff, uint32_t ii, const char *what) {
+    if (!nib) return at(rowOff + ii, what);
+    return (at(rowOff + ii / 2, what) >> (4 * (ii & 1u))) & 15u;
+  }
+  void set(uint64_t rowOff, uint32_t ii, uint32_t v, const char *what) {
+    if (!nib) {
+      at(rowOff + ii, what) = uint8_t(v);
+      return;
+    }
+    if (v > 15) die(std::string("value above 15 for a nibble row: ") + what);
+    uint8_t &x = at(rowOff + ii / 2, what);
+    x = uint8_t((ii & 1u) ? (x & 0x0Fu) | (v << 4) : (x & 0xF0u) | v);
   }
 };
 
@@ -69,6 +87,7 @@ uint32_t minOp(uint32_t q, uint32_t n) { return q < n ? q : n - 1; }
 // T_LUTROWS, as lutTask: rows relative to the value pool, level rows
 // relative to row V, the reference's 16-bit index and table bound.
 void runRows(Model &m, Area &A) {
+  if (A.nib) die("T_LUTROWS task on nibble rows");
   const uint32_t I = A.I, W = 64 / I, V = A.V;
   const uint32_t w0 = m.word(0), w1 = m.word(1);
   const uint32_t nMid = (w0 >> 8) & 0xFFFFu, nt = w1 & 0xFFFFu, nl = w1 >> 16;
@@ -116,10 +135,10 @@ void runRows(Model &m, Area &A) {
 }
 
 // T_LUTFAST, as lutFastTask: byte offsets from the lane's column; a lane
-// holds four instances (32-bit words of the rows), so a round is 256 / I ops
-// and every offset must be a multiple of 4.
+// holds one 32-bit word of the rows (four bytes or eight nibbles), so a
+// round is 256 / RB ops and every offset must be a multiple of 4.
 void runFast(Model &m, Area &A) {
-  const uint32_t I = A.I, W = 256 / I, V = A.V;
+  const uint32_t I = A.I, RB = A.rb(), W = 256 / RB, V = A.V;
   const uint32_t w0 = m.word(0), w1 = m.word(1);
   const uint32_t nMid = (w0 >> 8) & 0xFFFFu, nt = w1 & 0xFFFFu, nl = w1 >> 16;
   const uint32_t topOff = 4 + ((((nMid + 3) >> 2) + 1) & ~1u);
@@ -127,9 +146,10 @@ void runFast(Model &m, Area &A) {
   auto sizeOf = [&](uint32_t l) { return (m.word(4 + (l >> 2)) >> ((l & 3u) * 8)) & 0xFFu; };
   auto aligned = [&](uint64_t off) {
     if (off % 4) die("T_LUTFAST offset not a multiple of 4: " + std::to_string(off));
+    if (off % RB) die("T_LUTFAST offset not a whole row: " + std::to_string(off));
   };
-  auto nib = [&](uint64_t off) -> uint32_t {
-    const uint32_t v = A.at(off, "operand");
+  auto nib = [&](uint64_t off, uint32_t ii) -> uint32_t {
+    const uint32_t v = A.get(off, ii, "operand");
     if (v > 15) die("operand above 15 at offset " + std::to_string(off));
     return v;
   };
@@ -137,12 +157,11 @@ void runFast(Model &m, Area &A) {
     aligned(x >> 16);
     aligned(y & 0xFFFFu);
     aligned(y >> 16);
-    const uint32_t pos = (((nib(ii + (x >> 16)) << 4) | nib(ii + (y & 0xFFFFu))) << 4 | nib(ii + (y >> 16))) +
-                         (x & 0xFFFFu);
+    const uint32_t pos = (((nib(x >> 16, ii) << 4) | nib(y & 0xFFFFu, ii)) << 4 | nib(y >> 16, ii)) + (x & 0xFFFFu);
     if (pos >= m.lut.size()) die("LUT position past the region: " + std::to_string(pos));
     return m.lut[pos];
   };
-  struct Wr { uint64_t off; uint32_t v; };
+  struct Wr { uint64_t off; uint32_t ii, v; };
   std::vector<Wr> wr;
   uint64_t cur = topOff;
   uint32_t n = nt;
@@ -165,12 +184,12 @@ void runFast(Model &m, Area &A) {
             dst = (q & 1u) ? d >> 16 : d & 0xFFFFu;
           } else {
             if (lr >= kFastBufY || wb + lr >= kConstRow) die("level row past its level buffer");
-            dst = uint64_t(V + wb + lr) * I;
+            dst = uint64_t(V + wb + lr) * RB;
           }
           aligned(dst);
-          for (uint32_t ii = 0; ii < I; ++ii) wr.push_back({dst + ii, eval(x, y, ii)});
+          for (uint32_t ii = 0; ii < I; ++ii) wr.push_back({dst, ii, eval(x, y, ii)});
         }
-      for (auto &w : wr) A.at(w.off, "write") = uint8_t(w.v);
+      for (auto &w : wr) A.set(w.off, w.ii, w.v, "write");
     }
     cur = next;
     n = nn;
@@ -180,18 +199,19 @@ void runFast(Model &m, Area &A) {
       const uint32_t w = m.word(lastOff + std::min(r + sub, nl - 1));
       aligned(w & 0xFFFFu);
       aligned(w >> 16);
-      for (uint32_t ii = 0; ii < I; ++ii) A.at((w & 0xFFFFu) + ii, "last write") = uint8_t(nib((w >> 16) + ii));
+      for (uint32_t ii = 0; ii < I; ++ii) A.set(w & 0xFFFFu, ii, nib(w >> 16, ii), "last write");
     }
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
-  if (argc < 4) die("usage: batch_check design.bin I seed [rounds]");
+  if (argc < 4) die("usage: batch_check design.bin I seed [rounds] [nib]");
   auto bin = slurp(argv[1]);
   const uint32_t I = std::atoi(argv[2]);
   const uint64_t seed = std::strtoull(argv[3], nullptr, 10);
   const uint32_t rounds = argc > 4 ? std::atoi(argv[4]) : 2;
+  const bool nibRows = argc > 5 && std::string(argv[5]) == "nib";  // the fast image for nibble rows
   if (I != 8 && I != 16 && I != 32 && I != 64) die("I must be 8, 16, 32 or 64");
   SimDesignInfo d;
   std::string err;
@@ -202,12 +222,14 @@ int main(int argc, char **argv) {
   uint32_t maxVp = 16;
   for (const auto &p : d.parts) maxVp = std::max(maxVp, p.valuePoolSize);
   const uint32_t V = (maxVp + 15) & ~15u;
-  if ((V + kLevelRows) * I > 65536) die("rows do not fit 16-bit offsets for this I");
+  const uint32_t RB = nibRows ? I / 2 : I;  // bytes per row of the fast image's wave area
+  if ((V + kLevelRows) * RB > 65536) die("rows do not fit 16-bit offsets for this I");
   StimMap stim;
   const Wave64Image gi = buildWave64Image(d, stim, 4, V, 0);
-  const Wave64Image fi = buildWave64Image(d, stim, 4, V, I);
+  const Wave64Image fi = buildWave64Image(d, stim, 4, V, RB);
   if (gi.fastTasks != 0) die("general image holds T_LUTFAST tasks");
   if (gi.partOff.size() != fi.partOff.size()) die("partition counts differ");
+  if (nibRows && fi.rowTasks != 0) die("nibble rows need every LUT task fast");
   std::mt19937_64 rng(seed);
   std::vector<uint8_t> lut(kLutRegionB, 0);
   for (uint32_t k = 0; k < kLutSize; ++k) lut[k] = k < d.lut.size() ? d.lut[k] : 0;
@@ -218,14 +240,24 @@ int main(int argc, char **argv) {
       for (uint32_t k = 0; k < kLutSize; ++k) lut[k] = uint8_t(rng() & 15u);
     for (size_t p = 0; p < gi.partOff.size(); ++p) {
       const uint32_t *GP = gi.words.data() + gi.partOff[p], *FP = fi.words.data() + fi.partOff[p];
-      Area ga{V, I, std::vector<uint8_t>(size_t(V + kLevelRows) * I)};
+      // the same random nibbles in the value-pool rows of both areas, other
+      // random nibbles in their level rows (a row read before the task wrote
+      // it shows up as a difference), constant rows 0..31 (with nibble rows
+      // only 0..15 are representable: rows 16..31 stay random, so a read of
+      // them shows up too)
+      Area ga{V, I, false, std::vector<uint8_t>(size_t(V + kLevelRows) * I)};
+      Area fa{V, I, nibRows, std::vector<uint8_t>(size_t(V + kLevelRows) * RB)};
       for (auto &x : ga.b) x = uint8_t(rng() & 15u);
-      Area fa = ga;
+      for (auto &x : fa.b) x = uint8_t(rng() & 0xFFu);
+      for (uint32_t r = 0; r < V; ++r)
+        for (uint32_t ii = 0; ii < I; ++ii) fa.set(uint64_t(r) * RB, ii, ga.get(uint64_t(r) * I, ii, "init"), "init");
       for (uint32_t r = V; r < V + kLevelRows; ++r)
-        for (uint32_t ii = 0; ii < I; ++ii) fa.b[size_t(r) * I + ii] = uint8_t(rng() & 15u);
+        for (uint32_t ii = 0; ii < I; ++ii) fa.set(uint64_t(r) * RB, ii, uint32_t(rng() & 15u), "init");
       for (uint32_t r = kConstRow; r < kLevelRows; ++r)
-        for (uint32_t ii = 0; ii < I; ++ii)
-          ga.b[size_t(V + r) * I + ii] = fa.b[size_t(V + r) * I + ii] = uint8_t(r - kConstRow);
+        for (uint32_t ii = 0; ii < I; ++ii) {
+          ga.set(uint64_t(V + r) * I, ii, r - kConstRow, "const");
+          if (!nibRows || r - kConstRow < 16) fa.set(uint64_t(V + r) * RB, ii, r - kConstRow, "const");
+        }
       const uint32_t nLev = GP[H_NLEV];
       if (FP[H_NLEV] != nLev) die("exec level counts differ");
       // the LUT tasks of one exec level are independent: run all of them in
@@ -257,8 +289,11 @@ int main(int argc, char **argv) {
           }
         }
         if (gOther != fOther) die("other task counts differ");
-        if (!std::equal(ga.b.begin(), ga.b.begin() + size_t(V) * I, fa.b.begin()))
-          die("value pools differ after partition " + std::to_string(p) + " exec level " + std::to_string(e));
+        for (uint32_t r = 0; r < V; ++r)
+          for (uint32_t ii = 0; ii < I; ++ii)
+            if (ga.get(uint64_t(r) * I, ii, "cmp") != fa.get(uint64_t(r) * RB, ii, "cmp"))
+              die("value pools differ after partition " + std::to_string(p) + " exec level " + std::to_string(e) +
+                  " (slot " + std::to_string(r) + ", instance " + std::to_string(ii) + ")");
       }
     }
   }

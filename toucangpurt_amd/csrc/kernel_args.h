@@ -109,12 +109,14 @@ hipError_t streamMaxBlocksPerCU(int *blocks, uint32_t nc, uint32_t lds);
 hipError_t streamSetLds(uint32_t lds);
 
 // Batched kernel with `inst` instances per wave (8, 16, 32 or 64; 64 / inst ops
-// per round) and block / 64 waves per workgroup.
+// per round in per-instance tasks) and block / 64 waves per workgroup; `nib`:
+// rows hold a nibble per instance (nibble programs whose LUT tasks are all
+// T_LUTFAST), else a byte.
 bool batchSupportsGroup(uint32_t inst);
-uint32_t batchLdsBytes(uint32_t inst, uint32_t waves, uint32_t vpBytes, uint32_t progWords);
-hipError_t launchBatch(const BatchLaunch &launch, uint32_t inst, uint32_t grid, uint32_t block, uint32_t lds,
+uint32_t batchLdsBytes(uint32_t inst, uint32_t waves, uint32_t vpBytes, uint32_t progWords, bool nib);
+hipError_t launchBatch(const BatchLaunch &launch, uint32_t inst, bool nib, uint32_t grid, uint32_t block, uint32_t lds,
                        hipStream_t stream);
-hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t inst, uint32_t block, uint32_t lds);
+hipError_t batchMaxBlocksPerCU(int *blocks, uint32_t inst, bool nib, uint32_t block, uint32_t lds);
 hipError_t batchSetLds(uint32_t lds);
 // hipErrorInvalidKernelFile when a batched kernel was built with static LDS
 // (its LUT region must sit at LDS address 0).

@@ -227,6 +227,7 @@ struct tgs_ctx {
   uint32_t batchInst = 0, batchWaves = 0, batchInstEff = 64, batchBlockEff = 128;
   uint32_t batchGrid = 0, batchLds = 0, batchGridOverride = 0, batchProgWords = 0;
   uint32_t batchShapeN = 0;  // instance count the batched launch shape was chosen for
+  bool batchNib = false;     // the batched rows hold a nibble per instance
   bool batchReady = false;
   // Every byte of every pool is a nibble (true after a load whose initial
   // pools are nibbles, false from the first wider byte written through the
@@ -1054,52 +1055,68 @@ int ensureBatchImage(tgs_ctx *c) {
   if (insts.empty()) return c->fail(TGS_E_ARG, "instances per wave must be 8, 16, 32 or 64 (%u given)", forcedInst);
   if (c->batchWaves) waveList.push_back(c->batchWaves);
   else waveList = {2u, 4u, 8u, 12u, 16u};
+  // nibble rows (a nibble per instance) when every LUT task can be T_LUTFAST
+  const char *envNib = std::getenv("TGS_BATCH_NIB");
+  const bool nibOk = fastImg.fastTasks && fastImg.rowTasks == 0 && !(envNib && envNib[0] == '0');
   const uint64_t nInst = std::max<uint32_t>(c->bs.n, 1);
   c->batchShapeN = uint32_t(nInst);
-  bool have = false;
+  bool have = false, bestNib = false;
   uint32_t bestScore = 0, bestInst = 0, bestWaves = 0, bestLds = 0;  // bestScore: waves per CU
   double bestCols = 0;
   int bestPerCU = 0;
   for (uint32_t inst : insts)
-    for (uint32_t waves : waveList) {
-      const uint32_t lds = batchLdsBytes(inst, waves, vpBytes, maxPart);
-      if (lds > ldsMax) continue;
-      HIPCHK(c, batchSetLds(lds));
-      int perCU = 0;
-      HIPCHK(c, batchMaxBlocksPerCU(&perCU, inst, waves * 64, lds));
-      perCU = residentPerCU(c, perCU, lds);
-      if (perCU < 1) continue;
-      perCU = std::min<int>(perCU, std::max<uint32_t>(1, 16 / waves));
-      const uint32_t score = uint32_t(perCU) * waves;
-      const uint64_t groups = (nInst + inst - 1) / inst, items = (groups + waves - 1) / waves;
-      const double roundWeight = inst >= 64 ? 0.5 : inst >= 32 ? 0.8 : 1.0;
-      const double cols = double(perCU) * double(std::min<uint64_t>(nInst, items * waves * inst)) / double(items) *
-                          roundWeight;
-      const bool same = std::abs(cols - bestCols) <= 1e-6 * cols;
-      if (!have || (!same && cols > bestCols) ||
-          (same && (score > bestScore || (score == bestScore && inst < bestInst)))) {
-        have = true;
-        bestCols = cols;
-        bestScore = score;
-        bestInst = inst;
-        bestWaves = waves;
-        bestLds = lds;
-        bestPerCU = perCU;
+    for (bool nib : {false, true}) {
+      if (nib && !nibOk) continue;
+      const uint32_t rowBytes = nib ? inst / 2 : inst;
+      if (nib && (vpRows + kLevelRows) * rowBytes > 65536u) continue;
+      for (uint32_t waves : waveList) {
+        const uint32_t lds = batchLdsBytes(inst, waves, vpBytes, maxPart, nib);
+        if (lds > ldsMax) continue;
+        HIPCHK(c, batchSetLds(lds));
+        int perCU = 0;
+        HIPCHK(c, batchMaxBlocksPerCU(&perCU, inst, nib, waves * 64, lds));
+        perCU = residentPerCU(c, perCU, lds);
+        if (perCU < 1) continue;
+        perCU = std::min<int>(perCU, std::max<uint32_t>(1, 16 / waves));
+        const uint32_t score = uint32_t(perCU) * waves;
+        const uint64_t groups = (nInst + inst - 1) / inst, items = (groups + waves - 1) / waves;
+        // a T_LUTFAST round is 256 / rowBytes ops
+        const double roundWeight = rowBytes >= 64 ? 0.5 : rowBytes >= 32 ? 0.8 : rowBytes <= 4 ? 0.7 : 1.0;
+        const double cols = double(perCU) * double(std::min<uint64_t>(nInst, items * waves * inst)) / double(items) *
+                            roundWeight;
+        const bool same = std::abs(cols - bestCols) <= 1e-6 * cols;
+        if (!have || (!same && cols > bestCols) ||
+            (same && (score > bestScore || (score == bestScore && inst < bestInst)))) {
+          have = true;
+          bestCols = cols;
+          bestScore = score;
+          bestInst = inst;
+          bestNib = nib;
+          bestWaves = waves;
+          bestLds = lds;
+          bestPerCU = perCU;
+        }
       }
     }
   if (!have)
     return c->fail(TGS_E_LIMIT,
                    "batched execution needs %u B of LDS (value pool %u slots x %u instances); device allows %u B",
-                   batchLdsBytes(insts[0], 1, vpBytes, maxPart), c->w64.maxVp, insts[0], ldsMax);
-  // T_LUTFAST tasks for the chosen instance count when the rows' byte
-  // offsets fit 16 bits (the shape was sized for this image's program)
-  if (fastImg.fastTasks && (vpRows + kLevelRows) * bestInst <= 65536u) {
-    Wave64Image f = bestInst == 8 ? std::move(fastImg) : buildWave64Image(c->design, c->stim, imageWaves, vpRows, bestInst);
-    if (f.fastTasks && maxWords(f) <= maxPart) {
+                   batchLdsBytes(insts[0], 1, vpBytes, maxPart, false), c->w64.maxVp, insts[0], ldsMax);
+  // T_LUTFAST tasks for the chosen rows (offsets row * bytes per row) when
+  // those offsets fit 16 bits (the shape was sized for this image's program)
+  const uint32_t bestRowBytes = bestNib ? bestInst / 2 : bestInst;
+  c->batchNib = false;
+  if (fastImg.fastTasks && (vpRows + kLevelRows) * bestRowBytes <= 65536u) {
+    Wave64Image f = bestRowBytes == 8 ? std::move(fastImg)
+                                      : buildWave64Image(c->design, c->stim, imageWaves, vpRows, bestRowBytes);
+    if (f.fastTasks && maxWords(f) <= maxPart && (!bestNib || f.rowTasks == 0)) {
       c->w64 = std::move(f);
       c->batchFast = true;
+      c->batchNib = bestNib;
     }
   }
+  if (bestNib && !c->batchNib)
+    return c->fail(TGS_E_INVALID, "nibble rows chosen for an image that is not all T_LUTFAST");
   {
     auto &im = c->w64;
     int rc;
@@ -1228,12 +1245,14 @@ BatchArgs batchArgsOf(tgs_ctx *c, const BatchScratch &sc, uint32_t nCycles) {
 // the largest value pool and program do not fit), LDS sized for the largest.
 struct BatchShape {
   uint32_t inst = 0, block = 0, grid = 0, lds = 0, vpRows = 0, progWords = 0;
+  bool nib = false;
 };
 
 int batchShapeOf(tgs_ctx *const *cs, uint32_t nCtx, BatchShape *out) {
   tgs_ctx *c0 = cs[0];
   BatchShape sh;
   sh.inst = c0->batchInstEff;
+  sh.nib = c0->batchNib;
   for (uint32_t k = 0; k < nCtx; ++k) {
     sh.vpRows = std::max(sh.vpRows, up16(std::max<uint32_t>(cs[k]->w64.maxVp, 16)));
     sh.progWords = std::max(sh.progWords, cs[k]->batchProgWords);
@@ -1247,11 +1266,11 @@ int batchShapeOf(tgs_ctx *const *cs, uint32_t nCtx, BatchShape *out) {
   }
   const uint32_t ldsMax = uint32_t(c0->prop.sharedMemPerBlock) - kLdsGranuleMargin;
   for (uint32_t waves = c0->batchBlockEff / 64; waves >= 1; waves /= 2) {
-    const uint32_t lds = batchLdsBytes(sh.inst, waves, sh.vpRows, sh.progWords);
+    const uint32_t lds = batchLdsBytes(sh.inst, waves, sh.vpRows, sh.progWords, sh.nib);
     if (lds > ldsMax) continue;
     HIPCHK(c0, batchSetLds(lds));
     int perCU = 0;
-    HIPCHK(c0, batchMaxBlocksPerCU(&perCU, sh.inst, waves * 64, lds));
+    HIPCHK(c0, batchMaxBlocksPerCU(&perCU, sh.inst, sh.nib, waves * 64, lds));
     perCU = residentPerCU(c0, perCU, lds);
     if (perCU < 1) continue;
     perCU = std::min<int>(perCU, std::max<uint32_t>(1, 16 / waves));
@@ -1354,7 +1373,7 @@ int runBatchedMulti(tgs_ctx *const *cs, const uint32_t *const *cycles, uint32_t 
     HIPCHK(c0, launchFill32(barrier, 1, 0, c0->stream));  // a kernel, not a memset node
     BatchLaunch bl{argsDev, nCtx, L, barrier, sh.vpRows, sh.progWords};
     HIPCHK(c0, hipEventRecord(c0->ev0, c0->stream));
-    HIPCHK(c0, launchBatch(bl, sh.inst, sh.grid, sh.block, sh.lds, c0->stream));
+    HIPCHK(c0, launchBatch(bl, sh.inst, sh.nib, sh.grid, sh.block, sh.lds, c0->stream));
     HIPCHK(c0, hipEventRecord(c0->ev1, c0->stream));
     HIPCHK(c0, hipEventSynchronize(c0->ev1));
     float ms = 0;
@@ -1371,6 +1390,7 @@ int runBatchedMulti(tgs_ctx *const *cs, const uint32_t *const *cycles, uint32_t 
       c->prof.block = sh.block;
       c->prof.batch_group = sh.inst;
       c->prof.batch_fast_tasks = c->batchFast ? c->w64.fastTasks : 0;
+      c->prof.batch_nibble_rows = c->batchNib ? 1 : 0;
       c->prof.lds_bytes = sh.lds;
       c->prof.prog_bytes = c->w64.words.size() * 4;
       for (uint32_t s = 0; s < P; ++s) {
@@ -1769,19 +1789,21 @@ int tgs_run_batch_multi(tgs_ctx *const *ctxs, uint32_t n, const uint32_t *const 
   int rc;
   for (uint32_t k = 0; k < n; ++k)
     if ((rc = setDevice(ctxs[k])) || (rc = ensureBatchImage(ctxs[k]))) return rc;
-  // one launch per instance count per wave (the kernel is specialised on it)
+  // one launch per instance count per wave and row format (the kernel is
+  // specialised on both)
   std::vector<tgs_ctx *> grp;
   std::vector<const uint32_t *> cyc;
-  for (uint32_t inst : {8u, 16u, 32u, 64u}) {
-    grp.clear();
-    cyc.clear();
-    for (uint32_t k = 0; k < n; ++k)
-      if (ctxs[k]->batchInstEff == inst) {
-        grp.push_back(ctxs[k]);
-        cyc.push_back(cycles[k]);
-      }
-    if (!grp.empty() && (rc = runBatchedMulti(grp.data(), cyc.data(), uint32_t(grp.size())))) return rc;
-  }
+  for (uint32_t inst : {8u, 16u, 32u, 64u})
+    for (bool nib : {false, true}) {
+      grp.clear();
+      cyc.clear();
+      for (uint32_t k = 0; k < n; ++k)
+        if (ctxs[k]->batchInstEff == inst && ctxs[k]->batchNib == nib) {
+          grp.push_back(ctxs[k]);
+          cyc.push_back(cycles[k]);
+        }
+      if (!grp.empty() && (rc = runBatchedMulti(grp.data(), cyc.data(), uint32_t(grp.size())))) return rc;
+    }
   for (uint32_t k = 0; k < n; ++k) {
     tgs_ctx *c = ctxs[k];
     auto &st = c->bhStop;
