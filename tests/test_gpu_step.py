@@ -173,3 +173,31 @@ def test_step_reports_stop(gpu_available, kernel, n, stops):
         seen |= want
     assert seen == (stops > 0)
     sim.cleanup()
+
+
+@pytest.mark.parametrize("kernel", [KERNEL_STREAM, KERNEL_WAVE64])
+def test_one_launch_steps_keep_the_print_log(gpu_available, kernel):
+    """One-instance steps with printing on (each step one launch that also
+    writes and reads registers): the print log after steps of 1..7 cycles
+    with register writes in between, and its cycle numbers, equal the
+    oracle's run the same way."""
+    d = gen.generate(None, n_luts=2000, depth=6, n_regs=100, n_inputs=20, n_vecops=50, n_prints=3, n_stops=0,
+                     part_luts=400, seed=29)
+    rng = np.random.default_rng(4)
+    wids, rids = _ids(d, rng)
+    print(f"case one-launch steps with prints kernel={kernel}", flush=True)
+    sim = gpu_sim(d.design, d.symbols, False, d.inputs, seed=8, kernel=kernel, enable_print=True)
+    orc = OracleRun(d.design, d.symbols, False, d.inputs, seed=8)
+    for cycles in (1, 3, 2, 7, 1, 5):
+        vals = rng.integers(0, 16, size=(1, wids.size)).astype(np.uint8)
+        got, _ = sim.step(wids, vals, cycles, rids)
+        reg = orc.st.pool(0)
+        reg[wids] = vals[0]
+        orc.st.set_pool(0, reg)
+        orc.run(cycles, enable_print=True)
+        assert np.array_equal(got[0], orc.st.pool(0)[rids])
+    want = sorted(orc.st.print_log())
+    assert len(want) > 0
+    assert sim.print_log() == want
+    assert_pools_equal(gpu_pools(sim), orc.pools(), "after the steps")
+    sim.cleanup()
