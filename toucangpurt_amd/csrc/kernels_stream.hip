@@ -380,19 +380,23 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
       __syncthreads();
       __syncthreads();
     }
-    uint32_t j = 0, si = 0, cyc = 0;
+    // step counters kept without divisions: jk = j % kLoaders, si = j %
+    // nSteps, ln = the record index of this loader's next load
+    uint32_t j = 0, si = 0, cyc = 0, jk = 0, ln = (2 + k + kLoaders) % nSteps;
     for (;;) {
       if (si == 0) {
         if (cyc == c.nCycles) break;
         __syncthreads();  // cycle start (stimulus draws)
       }
-      if (j % kLoaders == k) {
+      if (jk == k) {
         storeRows<NC>(rs, ring + ((j + 2) % kSlots) * slotWords, lane);
         // the set's registers are reused by the next loads: keep the stores first
         __builtin_amdgcn_sched_barrier(0);
-        loadRows<NC>(rs, recOf(j + 2 + kLoaders), W, lane);
+        loadRows<NC>(rs, steps + ln * rec, W, lane);
+        for (ln += kLoaders; ln >= nSteps;) ln -= nSteps;
       }
-      if (recOf(j)[0] == S_REGION) {
+      if (++jk == kLoaders) jk = 0;
+      if (steps[si * rec] == S_REGION) {
         __syncthreads();  // the two workgroup barriers of the grid barrier
         __syncthreads();
       } else {
@@ -413,6 +417,7 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
   // compute waves: in a split step wave `wave` serves partition slot
   // wave / (NC / slots), otherwise every wave serves the one partition
   const uint32_t slotWaves = NC / a.slots;
+  const uint32_t splitSlot = wave / slotWaves;
   uint64_t xs = *c.xs;
   uint64_t barTicks = 0;
   const uint64_t tStart = __builtin_amdgcn_s_memrealtime();
@@ -428,8 +433,9 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     barTicks += gridBarrierShared(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
   }
   const uint32_t *mySlot = ring + wave * kStreamRows * 64 + lane;
-  auto readStep = [&](Pf &p, uint32_t j) {
-    const uint32_t *r = recOf(j);
+  // record ri of the cycle's steps, ring slot of step j
+  auto readStep = [&](Pf &p, uint32_t ri, uint32_t j) {
+    const uint32_t *r = steps + ri * rec;
     const uint32_t *sl = mySlot + (j % kSlots) * slotWords;
     p.ex = r[1 + 2 * wave];
     p.ey = r[2 + 2 * wave] | r[0];  // the step type rides in the entry's unused low nibble
@@ -441,7 +447,7 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     p.r5 = sl[320];
   };
   Pf cur, nxt;
-  readStep(cur, 0);
+  readStep(cur, 0, 0);
   uint32_t j = 0, si = 0, cyc = 0, constBase = 0;
   for (;;) {
     if (si == 0) {
@@ -456,11 +462,30 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
         if (pc) regionT0 = shaderClock();
       }
     }
-    readStep(nxt, j + 1);
+    // A run of LUT steps (most steps of a cycle) goes round a loop of its
+    // own: the general step dispatch below costs more scalar work per step.
+    // Every wave still meets one workgroup barrier per step, as there.
+    bool wrapped = false;
+    while ((cur.ey & 0x7Fu) == (S_LEVEL | T_LUT2 << 4)) {
+      readStep(nxt, si + 1 == nSteps ? 0u : si + 1, j + 1);
+      const bool split = (__builtin_amdgcn_readfirstlane(cur.ey) & kSplit) != 0;
+      lutPairS(cur, W, vpAll + (split ? splitSlot : 0u) * a.vpBytes, lut);
+      __syncthreads();
+      cur = nxt;
+      ++j;
+      if (++si == nSteps) {
+        si = 0;
+        ++cyc;
+        wrapped = true;
+        break;
+      }
+    }
+    if (wrapped) continue;
+    readStep(nxt, si + 1 == nSteps ? 0u : si + 1, j + 1);
     const uint32_t type = cur.ey & 15u, kind = (cur.ey >> 4) & 7u;
     const bool split = (__builtin_amdgcn_readfirstlane(cur.ey) & kSplit) != 0;
     const uint32_t sw = split ? slotWaves : uint32_t(NC);  // waves serving this wave's partition
-    const uint32_t slot = split ? wave / slotWaves : 0u;
+    const uint32_t slot = split ? splitSlot : 0u;
     uint8_t *vp = vpAll + slot * a.vpBytes;
     const uint32_t ts = (wave - slot * sw) * 64 + lane, nTs = sw * 64;
     if (type == S_LEVEL) {
