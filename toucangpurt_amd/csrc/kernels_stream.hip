@@ -302,10 +302,14 @@ struct RowSet {
 // readlane; a wave's rows are contiguous from its entry's x, so each load is
 // a uniform base plus the lane offset.
 template <int NC>
-__device__ __forceinline__ void loadRows(RowSet<NC> &s, const uint32_t *rec, const uint32_t *__restrict__ W,
-                                         uint32_t lane) {
+__device__ __forceinline__ uint32_t recWords(const uint32_t *rec, uint32_t lane) {
   constexpr uint32_t words = 1 + 2 * NC;
-  const uint32_t rv = rec[lane < words ? lane : 0];
+  return rec[lane < words ? lane : 0];
+}
+// rv: the record's words (recWords)
+template <int NC>
+__device__ __forceinline__ void loadRows(RowSet<NC> &s, uint32_t rv, const uint32_t *__restrict__ W,
+                                         uint32_t lane) {
   // unconditional loads: making them depend on the rows an entry needs (exec
   // masks or a uniform skip) measured 20-60% slower per step
 #pragma unroll
@@ -371,10 +375,10 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     RowSet<NC> rs;
     // steps 0 and 1 go straight into the ring; step 2 + k is this loader's first load
     if (k < kSlots) {
-      loadRows<NC>(rs, recOf(k), W, lane);
+      loadRows<NC>(rs, recWords<NC>(recOf(k), lane), W, lane);
       storeRows<NC>(rs, ring + k * slotWords, lane);
     }
-    loadRows<NC>(rs, recOf(2 + k), W, lane);
+    loadRows<NC>(rs, recWords<NC>(recOf(2 + k), lane), W, lane);
     __syncthreads();  // start barrier
     if (c.mbWrites) {  // the workgroup barriers of the mailbox's grid barrier
       __syncthreads();
@@ -383,20 +387,31 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     // step counters kept without divisions: jk = j % kLoaders, si = j %
     // nSteps, ln = the record index of this loader's next load
     uint32_t j = 0, si = 0, cyc = 0, jk = 0, ln = (2 + k + kLoaders) % nSteps;
+    // each step's type is read a step ahead into a VGPR (an opaque
+    // per-lane zero in the address keeps the compiler from waiting for it
+    // at once) and made scalar when that step starts
+    uint32_t zv;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zv));
+    uint32_t typeV = steps[zv];
     for (;;) {
       if (si == 0) {
         if (cyc == c.nCycles) break;
         __syncthreads();  // cycle start (stimulus draws)
       }
+      const bool region = __builtin_amdgcn_readfirstlane(typeV) == S_REGION;
+      typeV = steps[(si + 1 == nSteps ? 0u : si + 1) * rec + zv];
       if (jk == k) {
+        // the next load's record is read before the stores (LDS reads return
+        // in order: read after them, it would wait for them)
+        const uint32_t rv = recWords<NC>(steps + ln * rec, lane);
         storeRows<NC>(rs, ring + ((j + 2) % kSlots) * slotWords, lane);
         // the set's registers are reused by the next loads: keep the stores first
         __builtin_amdgcn_sched_barrier(0);
-        loadRows<NC>(rs, steps + ln * rec, W, lane);
+        loadRows<NC>(rs, rv, W, lane);
         for (ln += kLoaders; ln >= nSteps;) ln -= nSteps;
       }
       if (++jk == kLoaders) jk = 0;
-      if (steps[si * rec] == S_REGION) {
+      if (region) {
         __syncthreads();  // the two workgroup barriers of the grid barrier
         __syncthreads();
       } else {
