@@ -193,15 +193,28 @@ __device__ __forceinline__ void vpPutNibbles(uint8_t *vp, uint32_t res, uint32_t
   }
 }
 
-// The same for results that fit 64 bits (len <= 16).
+// The same for results that fit 64 bits (len <= 16). When every active lane
+// of the wave has a multiple of four nibbles, four slots go under one lane
+// mask at immediate offsets from the first slot; otherwise one slot at a
+// time. Slots res..res+len-1 lie inside the value pool (checked when the
+// image is built), so no slot index wraps.
 __device__ __forceinline__ void vpPutNibbles(uint8_t *vp, uint32_t res, uint32_t len, uint64_t r) {
-  for (uint32_t k = 0; k < len; k += 8) {
-    const uint32_t w = uint32_t(r);
-    r >>= 32;
+  uint8_t *d = vp + res;
+  if (__ballot((len & 3u) != 0) == 0) {
 #pragma unroll
-    for (uint32_t j = 0; j < 8; ++j)
-      if (k + j < len) vp[uint16_t(res + k + j)] = uint8_t((w >> (4 * j)) & 15u);
+    for (uint32_t q = 0; q < 4; ++q)
+      if (4 * q < len) {
+        const uint32_t w = uint32_t(r >> (16 * q));
+        d[4 * q] = uint8_t(w & 15u);
+        d[4 * q + 1] = uint8_t((w >> 4) & 15u);
+        d[4 * q + 2] = uint8_t((w >> 8) & 15u);
+        d[4 * q + 3] = uint8_t((w >> 12) & 15u);
+      }
+    return;
   }
+#pragma unroll
+  for (uint32_t j = 0; j < 16; ++j)
+    if (j < len) d[j] = uint8_t((r >> (4 * j)) & 15u);
 }
 
 // LUT lookup (SimEval.cu:349-356,406-413) with the u16 index arithmetic of the
