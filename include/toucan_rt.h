@@ -165,8 +165,8 @@ int tgs_device_bytes(tgs_ctx *ctx, uint64_t *bytes);
  * allocate device memory (serving loops, harnesses that forbid hipMalloc).
  * Same arguments and results as tgs_run_batch on a batched context; the
  * per-launch scratch of the batched kernel (slot cycle counts and bases, the
- * active-group table, the grid-barrier counter, the launch arguments) lives
- * in a static array of 312,064 bytes per GPU, held until the process ends, instead of context
+ * active-group table, the grid-barrier tree, the launch arguments) lives
+ * in a static array of 320,256 bytes per GPU, held until the process ends, instead of context
  * allocations, and no hipMalloc / hipFree or other allocating call is made.
  * At most 4,096 instances (TGS_E_LIMIT beyond); the batched program image
  * must be built, which tgs_set_kernel(TGS_KERNEL_BATCH), tgs_set_instances,

@@ -26,7 +26,7 @@ struct W64Args {
   const uint32_t *schedPart;
   const uint32_t *schedLds;
   uint32_t vpBytes;          // value-pool bytes in LDS (16-byte multiple)
-  uint32_t barrierBase;      // the barrier counter holds barrierBase * grid when the launch starts
+  uint32_t barrierBase;      // index of the last barrier the instance's barrier tree completed before the launch
 };
 
 struct StreamArgs {
@@ -37,7 +37,7 @@ struct StreamArgs {
   const uint32_t *stepCount;  // per workgroup: records per cycle
   uint32_t vpBytes;           // value-pool bytes in LDS per slot (16-byte multiple)
   uint32_t slots;             // partitions side by side per workgroup (NC / slots waves each)
-  uint32_t barrierBase;       // the barrier counter holds barrierBase * grid when the launch starts
+  uint32_t barrierBase;       // index of the last barrier the instance's barrier tree completed before the launch
 };
 
 // kernargCommon() reads the CommonArgs at offset 0 of a kernel's argument block.

@@ -810,8 +810,7 @@ __global__ __launch_bounds__(1024) void batchCycles(BatchLaunch L) {
     }
   };
   stimulus(0);
-  bar++;
-  gridBarrier(L.barrier, bar * gridDim.x, false);
+  gridBarrier(L.barrier, ++bar, false);
   for (uint32_t c = 0; c < L.nCycles; ++c) {
     uint32_t any = 0;
     for (uint32_t j = 0; j < L.nCtx; ++j) any |= activeOf(L.ctx[j], c);
@@ -827,12 +826,10 @@ __global__ __launch_bounds__(1024) void batchCycles(BatchLaunch L) {
             a.reg[size_t(a.inIds[i]) * a.instPad + s] = uint8_t((draws[size_t(i >> 6) * a.instPad + s] >> (i & 63)) & 1u);
     }
     batchRegion<I, NIB>(L, 0, c, vp, rowBytes, prog, curCtx);
-    bar++;
-    gridBarrier(L.barrier, bar * gridDim.x, false);
+    gridBarrier(L.barrier, ++bar, false);
     if (c + 1 < L.nCycles) stimulus(c + 1);
     batchRegion<I, NIB>(L, 1, c, vp, rowBytes, prog, curCtx);
-    bar++;
-    gridBarrier(L.barrier, bar * gridDim.x, false);
+    gridBarrier(L.barrier, ++bar, false);
   }
 }
 

@@ -445,7 +445,7 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
   __syncthreads();  // start barrier: steps 0 and 1 are in the ring
   if (c.mbWrites) {  // testbench register writes, complete before any partition runs
     mailboxWrites(c, tid, nTc);
-    barTicks += gridBarrierShared(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+    barTicks += gridBarrierShared(c.ctl + kCtlTree, ++bar, true);
   }
   const uint32_t *mySlot = ring + wave * kStreamRows * 64 + lane;
   // record ri of the cycle's steps, ring slot of step j
@@ -521,7 +521,7 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
         c.profOwner[part] = g;
       }
     } else {
-      barTicks += gridBarrierShared(&c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+      barTicks += gridBarrierShared(c.ctl + kCtlTree, ++bar, true);
       if (PROF && pc) {
         if (tid == 0) c.profWg[2 * g + region] += shaderClock() - regionT0;
         regionT0 = shaderClock();
@@ -539,7 +539,7 @@ __global__ __launch_bounds__((NC + kLoaders) * 64, 6) void streamCycles(StreamAr
     if (g == 0) *c.xs = xs;
     addBarrierTicks(c, barTicks, __builtin_amdgcn_s_memrealtime() - tStart);
   }
-  if (wave == 0) finishLaunch(kernargCommon(), (bar + 1) * gridDim.x);
+  if (wave == 0) finishLaunch(kernargCommon(), bar + 1);
 }
 
 namespace {

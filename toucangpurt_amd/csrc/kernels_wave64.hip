@@ -267,7 +267,7 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
   uint32_t bar = a.barrierBase;
   if (a.c.mbWrites) {  // testbench register writes, complete before any partition runs
     mailboxWrites(a.c, tid, nt);
-    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+    barTicks += gridBarrier(a.c.ctl + kCtlTree, ++bar, true);
   }
   // per-partition profile (SimEval.cu:894-983): ticks from the region's
   // start on this workgroup to the end of each partition, and to the end of
@@ -290,21 +290,21 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
       w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
       partDone(a.schedPart[s], t0, pc);
     }
-    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+    barTicks += gridBarrier(a.c.ctl + kCtlTree, ++bar, true);
     if (pc && tid == 0) a.c.profWg[2 * g] += shaderClock() - t0;
     t0 = pc ? shaderClock() : 0;
     for (uint32_t s = s1; s < s2; ++s) {
       w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
       partDone(a.schedPart[s], t0, pc);
     }
-    barTicks += gridBarrier(&a.c.ctl[C_BARRIER], (++bar) * gridDim.x, true);
+    barTicks += gridBarrier(a.c.ctl + kCtlTree, ++bar, true);
     if (pc && tid == 0) a.c.profWg[2 * g + 1] += shaderClock() - t0;
   }
   if (tid == 0) {
     if (g == 0) *a.c.xs = xs;
     addBarrierTicks(a.c, barTicks, __builtin_amdgcn_s_memrealtime() - tStart);
   }
-  if (tid < 64) finishLaunch(kernargCommon(), (bar + 1) * gridDim.x);
+  if (tid < 64) finishLaunch(kernargCommon(), bar + 1);
 }
 
 namespace {

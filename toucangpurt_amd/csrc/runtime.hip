@@ -31,7 +31,7 @@ using toucanGPUSim::SimDesignInfo;
 
 // Scratch memory of tgs_run_batch_noalloc: one static array per GPU, held
 // until the process ends. Layout: per-slot cycle counts (u32) and cycle bases
-// (u64) of up to kScratchSlots instance slots, the grid-barrier counter, the
+// (u64) of up to kScratchSlots instance slots, the grid-barrier tree, the
 // launch's BatchArgs, then the active-group table of up to kScratchCycles
 // launch-cycles. In host code
 // `scratch` names a host-side stand-in; its device address on each GPU comes
@@ -39,7 +39,7 @@ using toucanGPUSim::SimDesignInfo;
 constexpr uint32_t kScratchSlots = 4096, kScratchCycles = 65536;
 constexpr uint32_t kMaxBatchContexts = 64;  // contexts per tgs_run_batch_multi call
 constexpr size_t kScratchRunOff = 0, kScratchBaseOff = 4ull * kScratchSlots,
-                 kScratchBarrierOff = kScratchBaseOff + 8ull * kScratchSlots, kScratchArgsOff = kScratchBarrierOff + 256,
+                 kScratchBarrierOff = kScratchBaseOff + 8ull * kScratchSlots, kScratchArgsOff = kScratchBarrierOff + 4ull * kBarTreeWords,
                  kScratchActiveOff = kScratchArgsOff + 512, kScratchBytes = kScratchActiveOff + 4ull * kScratchCycles;
 static_assert(sizeof(BatchArgs) <= 512, "the scratch array holds one context's BatchArgs");
 static __device__ __align__(256) uint8_t scratch[kScratchBytes];
@@ -142,14 +142,14 @@ struct Instance {
   DBuf reg, mem, exg, xs, ctl, logCycle, logMsg, probe;
   PBuf ctlPin;  // the control words as the last launch left them (written by the launch's last workgroup)
   uint64_t cycles = 0;
-  // The grid-barrier counter is not cleared between launches of one grid
-  // size: the next launch starts at barrier index barNext (the counter holds
-  // barNext * barGrid); barGrid 0 means the counter is unknown and the next
-  // launch clears it first. ticksSeen: the control block's barrier and loop
-  // tick totals already added to the profile.
+  // The grid-barrier tree (after the control words) is not cleared between
+  // launches of one grid size: the next launch starts after barrier index
+  // barNext; barGrid 0 means the tree is unknown and the next launch clears
+  // it first. ticksSeen: the control block's barrier and loop tick totals
+  // already added to the profile.
   uint32_t barNext = 0, barGrid = 0;
   uint64_t ticksSeen[2] = {0, 0};
-  // the host rewrote the control block (counter and tick totals zero)
+  // the host rewrote the control words (tick totals zero; the tree is cleared before the next launch)
   void ctlRewritten() {
     barNext = 0;
     barGrid = 0;
@@ -344,7 +344,7 @@ int resetInstance(tgs_ctx *c, Instance &in, uint32_t index, uint64_t seed) {
   HIPCHK(c, in.mem.alloc(size_t(d.memPoolSize) + 16));
   HIPCHK(c, in.exg.alloc(up4(d.exchangePoolSize) + 16));
   HIPCHK(c, in.xs.alloc(8));
-  HIPCHK(c, in.ctl.alloc(C_WORDS * 4));
+  HIPCHK(c, in.ctl.alloc(kCtlAllocWords * 4));
   if (!in.ctlPin.h) HIPCHK(c, in.ctlPin.alloc(C_WORDS * 4));
   HIPCHK(c, in.logCycle.alloc(kLogCap * 8));
   HIPCHK(c, in.logMsg.alloc(kLogCap * 4));
@@ -647,13 +647,14 @@ uint32_t maxCyclesPerLaunch(uint32_t groups) {
 }
 
 // Barrier base of a launch of `grid` workgroups that waits at `barriers`
-// grid barriers and then arrives once more (finishLaunch). The counter is
+// grid barriers and then arrives once more (finishLaunch). The tree is
 // cleared first, by a fill kernel, when it counted for another grid or is
-// unknown, or when this launch would carry it out of range; otherwise it
-// goes on from the last launch and no clear precedes the launch.
+// unknown, or when this launch would carry a counter out of range (no
+// counter exceeds the barrier index times the grid); otherwise it goes on
+// from the last launch and no clear precedes the launch.
 int barrierBaseFor(tgs_ctx *c, Instance &in, uint32_t grid, uint64_t barriers, uint32_t *base) {
   if (in.barGrid != grid || (uint64_t(in.barNext) + barriers + 1) * grid >= 0xF0000000ull) {
-    HIPCHK(c, launchFill32(in.ctl.as<uint32_t>() + C_BARRIER, 1, 0, c->stream));
+    HIPCHK(c, launchFill32(in.ctl.as<uint32_t>() + kCtlTree, kBarTreeWords, 0, c->stream));
     in.barNext = 0;
     in.barGrid = grid;
   }
@@ -915,7 +916,7 @@ int batchUpload(tgs_ctx *c, const std::vector<HostInst> &h) {
   HIPCHK(c, b.logCount.alloc(8));
   HIPCHK(c, b.logCycle.alloc(b.logCap * 8));
   HIPCHK(c, b.logMsg.alloc(b.logCap * 8));
-  HIPCHK(c, b.barrier.alloc(4));
+  HIPCHK(c, b.barrier.alloc(kBarTreeWords * 4));
   HIPCHK(c, b.runCycles.alloc(4 * P));
   HIPCHK(c, b.cycleBase.alloc(8 * P));
   HIPCHK(c, b.instOfD.alloc(4 * P));
@@ -1370,7 +1371,7 @@ int runBatchedMulti(tgs_ctx *const *cs, const uint32_t *const *cycles, uint32_t 
       argsDev = c0->bs.argsDev.as<BatchArgs>();
     }
     HIPCHK(c0, hipMemcpyAsync(argsDev, args.data(), sizeof(BatchArgs) * nCtx, hipMemcpyHostToDevice, c0->stream));
-    HIPCHK(c0, launchFill32(barrier, 1, 0, c0->stream));  // a kernel, not a memset node
+    HIPCHK(c0, launchFill32(barrier, kBarTreeWords, 0, c0->stream));  // a kernel, not a memset node
     BatchLaunch bl{argsDev, nCtx, L, barrier, sh.vpRows, sh.progWords};
     HIPCHK(c0, hipEventRecord(c0->ev0, c0->stream));
     HIPCHK(c0, launchBatch(bl, sh.inst, sh.nib, sh.grid, sh.block, sh.lds, c0->stream));
