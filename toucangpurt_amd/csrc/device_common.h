@@ -239,7 +239,6 @@ __device__ __forceinline__ uint64_t memAddress(const uint8_t *vp, uint32_t addrV
   return base + addr;
 }
 
-// Grid-wide barrier over a monotonically increasing arrival counter.
 // Two-level arrival on a grid-barrier tree (kBarTreeWords words, 256-byte
 // aligned, all zero before its first barrier): workgroup g adds to the
 // counter of group g % kBarGroups, the last arrival of a group adds to the

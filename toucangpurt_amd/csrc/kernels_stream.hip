@@ -21,9 +21,10 @@
 //   * the LUT, the value pool and the stimulus draws live in LDS, program
 //     rows are never resident, so partitions of any size fit and several
 //     workgroups share a CU;
-//   * one atomic-counter grid barrier per region; the register, exchange and
-//     memory pools (the only data workgroups exchange) are read and written
-//     with agent-coherent accesses, so the barrier needs no L2 write-back.
+//   * one grid barrier per region on a two-level arrival tree; the register,
+//     exchange and memory pools (the only data workgroups exchange) are read
+//     and written with agent-coherent accesses, so the barrier needs no L2
+//     write-back.
 #include <hip/hip_runtime.h>
 
 #include "kernel_args.h"

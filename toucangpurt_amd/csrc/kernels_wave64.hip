@@ -13,9 +13,9 @@
 //     64-lane wave; operand shuffles are ds_bpermute within a half, and the
 //     next level's op word is loaded while the current level evaluates;
 //   * vector/memory ops of an exec level are regrouped 64 per wave;
-//   * one atomic-counter grid barrier per region (one agent-scope
-//     release/acquire per workgroup), launched cooperatively so every
-//     workgroup is resident.
+//   * one grid barrier per region on a two-level arrival tree (one
+//     agent-scope release/acquire per workgroup), launched cooperatively so
+//     every workgroup is resident.
 #include <hip/hip_runtime.h>
 
 #include "kernel_args.h"
