@@ -283,9 +283,22 @@ __device__ __forceinline__ uint64_t gridBarrier(uint32_t *tree, uint32_t b, bool
   if (threadIdx.x == 0) {
     const uint64_t t0 = measure ? __builtin_amdgcn_s_memrealtime() : 0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // The fence's L2 write-back (buffer_wbl2) must complete before the
+    // arrival: the compiler follows it with an LDS-only wait when it sees no
+    // vector memory operation outstanding, so the arrival's atomic could be
+    // performed first and a workgroup released by it read pool lines the
+    // write-back had not yet reached memory with.
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     treeArrive(tree, b);
     treeWait(tree, b);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // The fence's L2 invalidate (buffer_inv) orders this wave's later loads
+    // only; the other waves load after the workgroup barrier below, so it
+    // must have completed before this wave reaches that barrier (without
+    // the wait they could read pool lines the invalidate had not yet
+    // dropped: intermittently wrong registers with several partitions per
+    // workgroup).
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (measure) waited = __builtin_amdgcn_s_memrealtime() - t0;
   }
   __syncthreads();

@@ -962,7 +962,10 @@ Wave64Schedule scheduleWave64(const Wave64Image &img, uint32_t groups) {
       for (uint32_t id : per[g * 2 + r]) {
         s.parts.push_back(id);
         s.ldsOff.push_back(lds);
-        lds += img.partWords[id];
+        // whole 16-byte units: the kernel copies a program in 16-byte
+        // pieces (its last piece padded), so the next program must start
+        // past that piece, 16-byte aligned
+        lds += (img.partWords[id] + 3) & ~3u;
       }
       s.off.push_back(s.parts.size());
     }

@@ -13,9 +13,11 @@
 //     64-lane wave; operand shuffles are ds_bpermute within a half, and the
 //     next level's op word is loaded while the current level evaluates;
 //   * vector/memory ops of an exec level are regrouped 64 per wave;
-//   * one grid barrier per region on a two-level arrival tree (one
-//     agent-scope release/acquire per workgroup), launched cooperatively so
-//     every workgroup is resident.
+//   * one grid barrier per region on a two-level arrival tree, launched
+//     cooperatively so every workgroup is resident; the register, exchange
+//     and memory pools (the only data workgroups exchange) are read and
+//     written with agent-coherent accesses, so the barrier needs no L2
+//     write-back or invalidate.
 #include <hip/hip_runtime.h>
 
 #include "kernel_args.h"
@@ -164,7 +166,7 @@ __device__ __forceinline__ void memReadTask(const uint32_t *T, uint8_t *vp, cons
   const uint32_t en = rdOp(vp, o[2] & 0xFFFFu);
   if (en == 0) return;
   const uint64_t at = memAddress(vp, o[2] >> 16, o[3] >> 16, base);
-  vp[o[3] & 0xFFFFu] = at < c.memSize ? c.mem[at] : uint8_t(0);
+  vp[o[3] & 0xFFFFu] = at < c.memSize ? uint8_t(ldShared8(c.mem + at)) : uint8_t(0);
 }
 
 __device__ __forceinline__ void w64Partition(const W64Args &a, uint32_t part, const uint32_t *P, uint8_t *vp,
@@ -175,9 +177,9 @@ __device__ __forceinline__ void w64Partition(const W64Args &a, uint32_t part, co
   {
     const uint32_t nrr = P[H_NRR], ner = P[H_NER], nsr = P[H_NSR];
     const uint32_t *L = P + P[H_OFF_L0];
-    for (uint32_t k = tid; k < nrr; k += nt) vp[L[2 * k + 1]] = a.c.reg[L[2 * k]];
+    for (uint32_t k = tid; k < nrr; k += nt) vp[L[2 * k + 1]] = uint8_t(ldShared8(a.c.reg + L[2 * k]));
     L += 2 * nrr;
-    for (uint32_t k = tid; k < ner; k += nt) vp[L[2 * k + 1]] = a.c.exg[L[2 * k]];
+    for (uint32_t k = tid; k < ner; k += nt) vp[L[2 * k + 1]] = uint8_t(ldShared8(a.c.exg + L[2 * k]));
     L += 2 * ner;
     for (uint32_t k = tid; k < nsr; k += nt) vp[L[2 * k + 1]] = uint8_t(stimBit(draws, L[2 * k]));
   }
@@ -204,13 +206,13 @@ __device__ __forceinline__ void w64Partition(const W64Args &a, uint32_t part, co
   if (ewc) {
     const uint32_t exg = P[H_EW_EXG], dat = P[H_EW_DC] & 0xFFFFu, words = (ewc + 3) / 4;
     for (uint32_t k = tid; k < words; k += nt)
-      reinterpret_cast<uint32_t *>(a.c.exg)[(exg >> 2) + k] = reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k];
+      stShared32(reinterpret_cast<uint32_t *>(a.c.exg) + (exg >> 2) + k, reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k]);
   } else {
     const uint32_t rwc = P[H_RW_DC] >> 16;
     if (rwc) {
       const uint32_t reg = P[H_RW_REG], dat = P[H_RW_DC] & 0xFFFFu, words = (rwc + 3) / 4;
       for (uint32_t k = tid; k < words; k += nt)
-        reinterpret_cast<uint32_t *>(a.c.reg)[(reg >> 2) + k] = reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k];
+        stShared32(reinterpret_cast<uint32_t *>(a.c.reg) + (reg >> 2) + k, reinterpret_cast<const uint32_t *>(vp)[(dat >> 2) + k]);
     }
     const uint32_t nmw = P[H_NMW], npr = P[H_NPR], nst = P[H_NST];
     const uint32_t *L = P + P[H_OFF_LAST];
@@ -220,7 +222,7 @@ __device__ __forceinline__ void w64Partition(const W64Args &a, uint32_t part, co
       if (en != 0) {
         const uint32_t dat = rdOp(vp, o[2] >> 16);
         const uint64_t at = memAddress(vp, o[2] & 0xFFFFu, o[3] >> 16, uint64_t(o[0]) | uint64_t(o[1]) << 32);
-        if (at < a.c.memSize) a.c.mem[at] = uint8_t(dat);
+        if (at < a.c.memSize) stShared8(a.c.mem + at, dat);
       }
     }
     L += 4 * nmw;
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
   uint32_t bar = a.barrierBase;
   if (a.c.mbWrites) {  // testbench register writes, complete before any partition runs
     mailboxWrites(a.c, tid, nt);
-    barTicks += gridBarrier(a.c.ctl + kCtlTree, ++bar, true);
+    barTicks += gridBarrierShared(a.c.ctl + kCtlTree, ++bar, true);
   }
   // per-partition profile (SimEval.cu:894-983): ticks from the region's
   // start on this workgroup to the end of each partition, and to the end of
@@ -284,20 +286,20 @@ __global__ __launch_bounds__(1024) void w64Cycles(W64Args a) {
     if (tid == 0) stimulusDraws(xs, a.c.nIn, draws);
     __syncthreads();
     if (g == 0)
-      for (uint32_t k = tid; k < a.c.nIn; k += nt) a.c.reg[a.c.inIds[k]] = uint8_t(stimBit(draws, k));
+      for (uint32_t k = tid; k < a.c.nIn; k += nt) stShared8(a.c.reg + a.c.inIds[k], stimBit(draws, k));
     uint64_t t0 = pc ? shaderClock() : 0;
     for (uint32_t s = s0; s < s1; ++s) {
       w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
       partDone(a.schedPart[s], t0, pc);
     }
-    barTicks += gridBarrier(a.c.ctl + kCtlTree, ++bar, true);
+    barTicks += gridBarrierShared(a.c.ctl + kCtlTree, ++bar, true);
     if (pc && tid == 0) a.c.profWg[2 * g] += shaderClock() - t0;
     t0 = pc ? shaderClock() : 0;
     for (uint32_t s = s1; s < s2; ++s) {
       w64Partition(a, a.schedPart[s], progOf<MODE>(a, prog, s), vp, lut, draws, cycle);
       partDone(a.schedPart[s], t0, pc);
     }
-    barTicks += gridBarrier(a.c.ctl + kCtlTree, ++bar, true);
+    barTicks += gridBarrierShared(a.c.ctl + kCtlTree, ++bar, true);
     if (pc && tid == 0) a.c.profWg[2 * g + 1] += shaderClock() - t0;
   }
   if (tid == 0) {
