@@ -47,3 +47,26 @@ def test_barrier_tree_grids_match_oracle(gpu_available, kernel, block, groups):
     np.testing.assert_array_equal(sim.read_pool(POOL_REG), reg)
     np.testing.assert_array_equal(sim.read_pool(POOL_MEM), mem)
     sim.cleanup()
+
+
+def test_wave64_several_resident_programs_per_workgroup_repeated(gpu_available):
+    """Regression: with several partitions per workgroup the wave64 kernel keeps
+    their programs side by side in LDS, each copied in 16-byte pieces; programs
+    must start at whole pieces, or a copy's padded last piece overwrites the
+    next program's first words in a race (intermittently wrong registers: about
+    one run in ten on 16 workgroups before the fix)."""
+    d = design()
+    orc = OracleRun(d.design, inputs=d.inputs, seed=5)
+    orc.run(15)
+    reg, mem, _ = orc.pools()
+    for rep in range(12):
+        print(f"case wave64 16 workgroups repetition {rep}", flush=True)
+        sim = rt.Simulator(0)
+        assert sim.init(0, d.design, d.symbols, False, False) == 0
+        sim.set_kernel(KERNEL_WAVE64)
+        sim.set_launch(KERNEL_WAVE64, 512, 16)
+        sim.set_stimulus(d.inputs, 5)
+        sim.eval_free_running(15)
+        np.testing.assert_array_equal(sim.read_pool(POOL_REG), reg)
+        np.testing.assert_array_equal(sim.read_pool(POOL_MEM), mem)
+        sim.cleanup()
